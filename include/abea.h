@@ -1,0 +1,209 @@
+/*
+ * abea.h -- C ABI of the MI355X (gfx950) Adaptive Banded Event Alignment
+ * engine.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Drop-in boundary (reference interfaces replaced):
+ *   abea_align()        f5c per-read align(),  GPU/f5cmisc.h:48-50
+ *   abea_align_db()     f5c batched align_cuda(core_t*, db_t*),
+ *                       GPU/f5cmisc.h:99-102; the reference's OpenCL
+ *                       analogue is align_ocl(), GPU/align_cuda.c:391-943
+ *   abea_plan_*()       the same batch, device-resident, for callers that
+ *                       keep their data in HBM (the torch binding, benches)
+ *
+ * Semantics follow f5c's CPU align(): AlignedPair lists identical to it,
+ * band W = 100, k = 6, 4096-row pore model.
+ */
+#ifndef ABEA_H
+#define ABEA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ABEA_BANDWIDTH 100 /* ALN_BANDWIDTH, GPU/f5c.h:30 */
+#define ABEA_KMER_SIZE 6   /* KMER_SIZE, GPU/f5c.h:26 */
+#define ABEA_NUM_KMER 4096 /* NUM_KMER, GPU/f5c.h:27 */
+
+/* ---- f5c data types, byte-identical to GPU/f5c.h:112-181 (x86-64) ---- */
+typedef struct {
+  uint64_t start;
+  float length;
+  float mean;
+  float stdv;
+} abea_event_t; /* event_t, GPU/f5c.h:113-121 (24 bytes with tail padding) */
+
+typedef struct {
+  size_t n;
+  size_t start;
+  size_t end;
+  abea_event_t *event;
+} abea_event_table; /* event_table, GPU/f5c.h:124-130 */
+
+typedef struct {
+  float level_mean;
+  float level_stdv;
+  float level_log_stdv;
+} abea_model_t; /* model_t with CACHED_LOG, GPU/f5c.h:133-148 */
+
+typedef struct {
+  float scale;
+  float shift;
+  float var;
+  float log_var;
+} abea_scalings_t; /* scalings_t, GPU/f5c.h:151-167 */
+
+typedef struct {
+  int32_t ref_pos;
+  int32_t read_pos;
+} abea_aligned_pair_t; /* AlignedPair, GPU/f5c.h:177-181 */
+
+typedef enum {
+  ABEA_OK = 0,
+  ABEA_ERR_INVALID = 1,  /* bad argument */
+  ABEA_ERR_HIP = 2,      /* a HIP runtime call failed */
+  ABEA_ERR_NOMEM = 3,    /* device or host allocation failed */
+  ABEA_ERR_CAPACITY = 4, /* output capacity too small */
+  ABEA_ERR_NO_DEVICE = 5 /* no usable gfx950 device */
+} abea_status_t;
+
+/* Human-readable message for the last error raised on this thread. */
+const char *abea_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* (1) f5c align(): one read, host buffers.  GPU/f5cmisc.h:48-50.
+ * Returns the number of pairs written to out_2 (0 = QC failed, as f5c).
+ * out_2 must hold 2 * events.n pairs (f5c's allocation,
+ * GPU/align_cuda.c:496); a path that would not fit is reported as a failed
+ * read (0) instead of overrunning.  sample_rate is unused, as in the
+ * reference (GPU/align_cuda.cl:84-88).  Runs the plain per-read form on the
+ * calling thread's default context; returns -1 on a device error
+ * (abea_last_error() tells which). */
+int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_len,
+                   abea_event_table events, abea_model_t *models, abea_scalings_t scaling,
+                   float sample_rate);
+
+/* (2) f5c align_cuda(core, db): a batch in host buffers.
+ * GPU/f5cmisc.h:99-102 / align_ocl GPU/align_cuda.c:391-943.
+ * Reads read[i] (read_len[i] bases), et[i], scalings[i] and model[4096];
+ * writes n_event_align_pairs[i] and event_align_pairs[i], which is
+ * malloc()ed here per read (the caller frees it), as align_ocl does
+ * (GPU/align_cuda.c:926-931).  Uses the batched persistent form. */
+abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
+                            const abea_event_table *et, const abea_model_t *model,
+                            const abea_scalings_t *scalings,
+                            abea_aligned_pair_t **event_align_pairs,
+                            int32_t *n_event_align_pairs);
+
+/* ---------------------------------------------------------------------- */
+/* (3) Device-resident batches.  All pointers in abea_batch_t and
+ * abea_output_t are device pointers except the two *_host arrays. */
+
+typedef struct abea_ctx abea_ctx_t;
+typedef struct abea_plan abea_plan_t;
+
+/* One context per device per thread of use.  stream 0 = a new stream owned
+ * by the context; otherwise the caller's hipStream_t (cast to void*). */
+abea_status_t abea_ctx_create(abea_ctx_t **ctx, int device, void *stream);
+void abea_ctx_destroy(abea_ctx_t *ctx);
+void *abea_ctx_stream(abea_ctx_t *ctx);
+
+typedef struct {
+  int32_t n_reads;
+  const char *seq;               /* bases, CSR by seq_ptr (no NUL needed) */
+  const int64_t *seq_ptr;        /* [n_reads] */
+  const int32_t *read_len;       /* [n_reads] */
+  const abea_event_t *events;    /* [sum n_events] event_t records, CSR by ev_ptr */
+  const float *ev_mean;          /* alternative to events: means only (SoA); one of the two */
+  const int64_t *ev_ptr;         /* [n_reads] */
+  const int32_t *n_events;       /* [n_reads] */
+  const abea_scalings_t *scalings; /* [n_reads] */
+  const abea_model_t *model;     /* [4096] */
+  const int32_t *read_len_host;  /* host copies, used for planning */
+  const int32_t *n_events_host;
+  const int64_t *seq_ptr_host;
+  const int64_t *ev_ptr_host;
+} abea_batch_t;
+
+/* Per-read output slots: read i owns pairs [off[i], off[i] + n_events[i] +
+ * max(read_len[i]-5, 0)), where off is the exclusive prefix sum of those
+ * capacities in read order (abea_pair_offsets()); abea_pair_capacity()
+ * gives the total. */
+typedef struct {
+  abea_aligned_pair_t *pairs;      /* ascending pairs of the traced path (kept for QC-failed reads) */
+  uint8_t *path_from;              /* optional: 0 = D, 1 = U, 2 = L per pair */
+  int32_t *n_pairs;                /* f5c's n_event_align_pairs: path length, 0 when QC fails */
+  int32_t *path_len;               /* optional: path length before QC */
+  float *score;                    /* optional: best-path score (f5c float semantics) */
+  double *avg_log_emission;        /* optional: QC statistic */
+} abea_output_t;
+
+int64_t abea_pair_capacity(int32_t n_reads, const int32_t *read_len_host,
+                           const int32_t *n_events_host);
+void abea_pair_offsets(int32_t n_reads, const int32_t *read_len_host,
+                       const int32_t *n_events_host, int64_t *offsets);
+
+enum {
+  ABEA_PLAN_PLAIN = 0,   /* one pre/core/post launch set per read, in read order */
+  ABEA_PLAN_BATCHED = 1, /* length-sorted work queue, persistent core, graph replay */
+};
+
+/* Plan a batch: uploads per-read metadata and sizes the workspace (trace
+ * matrix, staged k-mer parameters, ...).  The batch's device buffers must
+ * stay valid (and may be updated in place, e.g. new scalings) for the plan's
+ * lifetime. */
+abea_status_t abea_plan_create(abea_ctx_t *ctx, const abea_batch_t *batch, int form,
+                               abea_plan_t **plan);
+void abea_plan_destroy(abea_plan_t *plan);
+
+/* Run pre -> core -> post on the context's stream (asynchronous). */
+abea_status_t abea_plan_forward(abea_plan_t *plan, const abea_output_t *out);
+
+/* Record the band moves (1 bit per band) of the last forward in
+ * moves[n_words], words laid out per read at abea_plan_move_offset(). */
+int64_t abea_plan_move_words(const abea_plan_t *plan);
+abea_status_t abea_plan_get_moves(abea_plan_t *plan, uint32_t *moves_host);
+/* Hold the band fixed: later forwards replay these moves instead of
+ * Suzuki's rule (NULL restores adaptive banding). */
+abea_status_t abea_plan_set_replay(abea_plan_t *plan, const uint32_t *moves_host);
+
+/* Adjoint of the summed weighted best-path score.  Needs the pairs and
+ * path_from of a forward on the same plan.  grad_score[n_reads] (device,
+ * f32) weights each read.  Outputs (device, caller-zeroed is not needed;
+ * they are overwritten):
+ *   grad_scalings[n_reads*3]  d/d(shift, scale, var), f32 (var column is 0:
+ *                             the reference emission ignores var,
+ *                             GPU/align_cuda.cl:95)
+ *   grad_model[4096*2]        d/d(level_mean, level_stdv), f64 accumulated,
+ *                             with level_log_stdv = log(level_stdv). */
+abea_status_t abea_plan_adjoint(abea_plan_t *plan, const abea_output_t *fwd,
+                                const float *grad_score, float *grad_scalings,
+                                double *grad_model);
+
+/* Tuning knobs chosen by the autotuner (or set by hand). */
+typedef struct {
+  int32_t core_waves_per_block; /* 1, 2 or 4 waves per workgroup */
+  int32_t core_waves_per_cu;    /* resident waves per CU for the persistent grid */
+  int32_t post_rows;            /* trace rows per LDS chunk in the traceback */
+  int32_t use_graph;            /* replay pre->core->post as one hipGraph */
+} abea_tuning_t;
+
+abea_status_t abea_plan_get_tuning(const abea_plan_t *plan, abea_tuning_t *t);
+abea_status_t abea_plan_set_tuning(abea_plan_t *plan, const abea_tuning_t *t);
+/* Time the candidate configurations on this plan (runs forwards) and keep
+ * the fastest; best_ms receives its time. */
+abea_status_t abea_plan_autotune(abea_plan_t *plan, const abea_output_t *out, int reps,
+                                 double *best_ms);
+
+/* Per-kernel device time (ms) of the last forward/adjoint, hipEvent-timed
+ * on the plan's stream: [0]=pre [1]=core [2]=post [3]=adjoint. */
+abea_status_t abea_plan_last_times(const abea_plan_t *plan, double times_ms[4]);
+/* Bytes of device workspace held by the plan. */
+int64_t abea_plan_workspace_bytes(const abea_plan_t *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABEA_H */

@@ -1,0 +1,197 @@
+/*
+ * abea_oracle.c -- CPU restatement of f5c's ABEA (see abea_oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+ * baseline.  Never linked into the product library.
+ *
+ * Built with -ffp-contract=off so that every a*b+c below rounds twice, as the
+ * x86-64 f5c build does.
+ */
+#include "abea_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define W 100            /* ALN_BANDWIDTH, GPU/f5c.h:30 */
+#define HALF_W 50
+#define KMER 6           /* KMER_SIZE, GPU/f5c.h:26 */
+#define FROM_D 0         /* GPU/align_cuda.cl:130-132 */
+#define FROM_U 1
+#define FROM_L 2
+#define MAX_GAP_THRESHOLD 50          /* GPU/align_cuda.cl:134 */
+#define MIN_AVG_LOG_EMISSION (-5.0)   /* GPU/align_cuda.cl:139 */
+#define LOG_INV_SQRT_2PI (-0.918938f) /* GPU/align_cuda.cl:62 */
+
+void orc_transition_constants(int32_t n_events, int32_t n_kmers, double out[4]) {
+  /* align_swi.cl:233-250: events_per_kmer and p_stay are double on the CPU. */
+#ifdef ORC_MUTATE_FLOAT_PSTAY /* planted mistake: GPU kernel's float p_stay */
+  float events_per_kmer = (float)n_events / n_kmers;
+  float p_stay = 1 - (1 / (events_per_kmer + 1));
+#else
+  double events_per_kmer = (double)n_events / n_kmers;
+  double p_stay = 1 - (1 / (events_per_kmer + 1));
+#endif
+  double lp_skip = log(1e-10);
+  double lp_stay = log(p_stay);
+  double lp_step = log(1.0 - exp(lp_skip) - exp(lp_stay));
+  double lp_trim = log(0.01);
+  out[0] = lp_skip;
+  out[1] = lp_stay;
+  out[2] = lp_step;
+  out[3] = lp_trim;
+}
+
+static inline uint32_t base_code(char b) {
+  /* GPU/align_cuda.cl:22-35 */
+  switch (b) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+    default: return 0;
+  }
+}
+
+uint32_t orc_kmer_rank(const char *s) {
+  /* GPU/align_cuda.cl:39-51: r += rank(str[k-i-1]) << 2i, so s[0] is the
+   * most significant base-4 digit. */
+  uint32_t r = 0;
+#ifdef ORC_MUTATE_RANK_LSB_FIRST /* planted mistake: reversed digit order */
+  for (uint32_t i = 0; i < KMER; ++i) r += base_code(s[i]) << (i << 1);
+#else
+  for (uint32_t i = 0; i < KMER; ++i) r += base_code(s[KMER - i - 1]) << (i << 1);
+#endif
+  return r;
+}
+
+float orc_emission_f32(float x, const orc_model_t *m, float scale, float shift) {
+  /* GPU/align_cuda.cl:511-528 with CACHED_LOG: the cached level_log_stdv is
+   * used as-is; scaling.var does not enter (gp_stdv = level_stdv * 1). */
+  float gp_mean = scale * m->level_mean + shift;
+  float gp_stdv = m->level_stdv;
+#ifdef ORC_MUTATE_RECOMPUTE_LOG /* planted mistake: log recomputed */
+  float gp_log_stdv = logf(m->level_stdv);
+#else
+  float gp_log_stdv = m->level_log_stdv;
+#endif
+  float a = (x - gp_mean) / gp_stdv;
+  return LOG_INV_SQRT_2PI - gp_log_stdv + (-0.5f * a * a);
+}
+
+static inline double emission_f64(float x, const orc_model_t *m, float scale, float shift) {
+  double gp_mean = (double)scale * (double)m->level_mean + (double)shift;
+  double a = ((double)x - gp_mean) / (double)m->level_stdv;
+  return (double)LOG_INV_SQRT_2PI - (double)m->level_log_stdv + (-0.5 * a * a);
+}
+
+/* ------------------------------------------------------------------------ */
+/* The band fill is written once over a cell type: float for the f5c
+ * semantics, double for the error oracle. */
+
+#define CELL float
+#define ORC_FILL orc_fill_f32
+#define EMIT(x, m, sc, sh) orc_emission_f32((x), (m), (sc), (sh))
+#define ROUND(v) ((float)(v))
+#include "abea_oracle_fill.inc"
+#undef CELL
+#undef ORC_FILL
+#undef EMIT
+#undef ROUND
+
+#define CELL double
+#define ORC_FILL orc_fill_f64
+#define EMIT(x, m, sc, sh) emission_f64((x), (m), (sc), (sh))
+#define ROUND(v) ((double)(v))
+#include "abea_oracle_fill.inc"
+#undef CELL
+#undef ORC_FILL
+#undef EMIT
+#undef ROUND
+
+int32_t orc_align(orc_pair_t *out, int32_t out_cap, const char *seq, int32_t seq_len,
+                  const float *ev_mean, int32_t n_events, const orc_model_t *models,
+                  float scale, float shift, int mode, const uint8_t *replay_moves,
+                  uint8_t *moves_out, orc_pair_t *path_out, uint8_t *from_out,
+                  uint8_t *trace_out, float *endcol_out, orc_info_t *info) {
+  if (mode == ORC_MODE_FP64)
+    return orc_fill_f64(out, out_cap, seq, seq_len, ev_mean, n_events, models, scale, shift,
+                        replay_moves, moves_out, path_out, from_out, trace_out, endcol_out, info);
+  return orc_fill_f32(out, out_cap, seq, seq_len, ev_mean, n_events, models, scale, shift,
+                      replay_moves, moves_out, path_out, from_out, trace_out, endcol_out, info);
+}
+
+void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n, const char *seq,
+                       const float *ev_mean, const orc_model_t *models, float scale, float shift,
+                       double weight, double *g_read, double *g_mu, double *g_sigma) {
+  /* score = sum over D/U path cells of em(e,k) + terms that do not depend on
+   * the scalings or the model.  With a = (x - (scale*mu + shift)) / sigma and
+   * em = c - log(sigma) - a^2/2:
+   *   d em/d shift = a/sigma, d em/d scale = a*mu/sigma,
+   *   d em/d mu = a*scale/sigma, d em/d sigma = (a^2 - 1)/sigma,
+   *   d em/d var = 0 (GPU/align_cuda.cl:95: gp_stdv = level_stdv * 1). */
+  double gs = 0, gc = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    if (from[i] == FROM_L) continue;
+    int32_t k = path[i].ref_pos, e = path[i].read_pos;
+    uint32_t r = orc_kmer_rank(seq + k);
+    const orc_model_t *m = &models[r];
+    double mu = m->level_mean, sg = m->level_stdv;
+    double a = ((double)ev_mean[e] - ((double)scale * mu + (double)shift)) / sg;
+    gs += a / sg;
+    gc += a * mu / sg;
+    if (g_mu) g_mu[r] += weight * (a * (double)scale / sg);
+    if (g_sigma) g_sigma[r] += weight * ((a * a - 1.0) / sg);
+  }
+  if (g_read) {
+    g_read[0] += weight * gs;
+    g_read[1] += weight * gc;
+    g_read[2] += 0.0;
+  }
+}
+
+typedef struct { int64_t key; int32_t idx; } sort_item;
+
+static int cmp_desc(const void *a, const void *b) {
+  const sort_item *x = (const sort_item *)a, *y = (const sort_item *)b;
+  if (x->key != y->key) return x->key > y->key ? -1 : 1;
+  return x->idx - y->idx;
+}
+
+int orc_align_batch(int32_t n_reads, const char *seqs, const int64_t *seq_ptr,
+                    const int32_t *seq_len, const float *ev_means, const int64_t *ev_ptr,
+                    const int32_t *n_events, const orc_model_t *models, const float *scale,
+                    const float *shift, orc_pair_t *out, const int64_t *pair_ptr,
+                    int32_t *n_pairs, float *scores, int n_threads) {
+  /* The reference's CPU path is OpenMP/pthreads over reads
+   * (GPU/f5c.h:57-59); longest reads first keeps the threads balanced. */
+  sort_item *order = (sort_item *)malloc(sizeof(sort_item) * (size_t)(n_reads > 0 ? n_reads : 1));
+  for (int32_t i = 0; i < n_reads; ++i) {
+    order[i].key = (int64_t)n_events[i] + seq_len[i];
+    order[i].idx = i;
+  }
+  qsort(order, (size_t)n_reads, sizeof(sort_item), cmp_desc);
+  int used = 1;
+#ifdef _OPENMP
+  if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+#endif
+  for (int32_t j = 0; j < n_reads; ++j) {
+    int32_t i = order[j].idx;
+    orc_info_t info;
+    int32_t cap = n_events[i] + (seq_len[i] > 5 ? seq_len[i] - 5 : 0);
+    orc_pair_t *o = out ? out + pair_ptr[i] : NULL;
+    n_pairs[i] = orc_fill_f32(o, o ? cap : 0, seqs + seq_ptr[i], seq_len[i], ev_means + ev_ptr[i],
+                              n_events[i], models, scale[i], shift[i], NULL, NULL, NULL, NULL,
+                              NULL, NULL, &info);
+    if (scores) scores[i] = (float)info.score;
+  }
+#ifdef _OPENMP
+  used = n_threads;
+#endif
+  free(order);
+  return used;
+}

@@ -1,0 +1,104 @@
+/*
+ * abea_oracle.h -- CPU restatement of f5c's Adaptive Banded Event Alignment.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  The product library (libabea) never links or calls it.
+ *
+ * Semantics follow the reference's CPU-shaped sources:
+ *   fill       FPGA/code/device/align_swi.cl:230-503
+ *   traceback  FPGA/code/host/src/align_swi.cpp:1139-1312
+ *   emission   GPU/align_cuda.cl:62-110 (cached level_log_stdv, CACHED_LOG
+ *              GPU/f5c.h:67)
+ *   types      GPU/f5c.h:112-181
+ * with the precision choices of upstream f5c align(): double transition
+ * constants and p_stay, float band cells, double QC emission sum.
+ */
+#ifndef ABEA_ORACLE_H
+#define ABEA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float level_mean, level_stdv, level_log_stdv; } orc_model_t; /* GPU/f5c.h:133-148 */
+typedef struct { int32_t ref_pos, read_pos; } orc_pair_t;                     /* GPU/f5c.h:177-181 */
+
+enum { ORC_MODE_F5C = 0,  /* f5c mixed precision (float cells, double transitions) */
+       ORC_MODE_FP64 = 1  /* every quantity in double: the error oracle */ };
+
+typedef struct {
+  double score;             /* best end score (float-valued in ORC_MODE_F5C) */
+  double avg_log_emission;  /* QC statistic, GPU/align_cuda.cl:868 */
+  int32_t path_len;         /* pairs on the traced path before QC */
+  int32_t max_gap;
+  int32_t spanned;
+  int32_t qc_pass;
+  int32_t end_event;        /* e* of the best end cell */
+  int32_t n_bands;
+  int64_t fills;            /* cells actually computed */
+} orc_info_t;
+
+/* Per-read transition log-probabilities, align_swi.cl:233-250. out[4] =
+ * {lp_skip, lp_stay, lp_step, lp_trim}. */
+void orc_transition_constants(int32_t n_events, int32_t n_kmers, double out[4]);
+
+/* 6-mer rank, base-4 with S[k] the most significant digit,
+ * GPU/align_cuda.cl:22-51. Non-ACGT maps to 0. */
+uint32_t orc_kmer_rank(const char *s);
+
+/* f5c-shaped emission (float), GPU/align_cuda.cl:511-528. */
+float orc_emission_f32(float x, const orc_model_t *m, float scale, float shift);
+
+/*
+ * Align one read.  Returns the number of pairs written to out (0 when QC
+ * fails, or when the read has fewer than 6 bases / no events).
+ *
+ * out, out_cap     ascending AlignedPairs after QC (may be NULL)
+ * replay_moves     if non-NULL, n_bands bytes: 1 = band b moved right,
+ *                  0 = down, used instead of Suzuki's rule (band held fixed)
+ * moves_out        if non-NULL, n_bands bytes of the moves taken
+ * path_out/from_out  pre-QC path in ascending order with from codes
+ *                  (0 = D, 1 = U, 2 = L); capacity n_events + n_kmers
+ * trace_out        if non-NULL, n_bands*100 trace bytes (reference layout)
+ * endcol_out       if non-NULL, n_events floats: band cell (e, n_kmers-1),
+ *                  -inf when outside the band
+ */
+int32_t orc_align(orc_pair_t *out, int32_t out_cap,
+                  const char *seq, int32_t seq_len,
+                  const float *ev_mean, int32_t n_events,
+                  const orc_model_t *models, float scale, float shift,
+                  int mode, const uint8_t *replay_moves, uint8_t *moves_out,
+                  orc_pair_t *path_out, uint8_t *from_out,
+                  uint8_t *trace_out, float *endcol_out, orc_info_t *info);
+
+/*
+ * FP64 gradient of the best-path score along a given path (from codes as
+ * returned by orc_align): only D and U cells carry an emission.
+ * g_read[0..2] += d/d(shift, scale, var)   (var term is identically zero)
+ * g_mu[rank], g_sigma[rank] += d/d level_mean, d/d level_stdv (with
+ * level_log_stdv = log(level_stdv) differentiated as well).
+ */
+void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n,
+                       const char *seq, const float *ev_mean,
+                       const orc_model_t *models, float scale, float shift,
+                       double weight, double *g_read, double *g_mu,
+                       double *g_sigma);
+
+/* Batched CPU baseline: OpenMP over reads, longest first.
+ * seq_ptr / ev_ptr are CSR offsets (int64) into seqs / ev_means.
+ * Writes n_pairs[i], scores[i]; pairs written at out + pair_ptr[i] when out
+ * is non-NULL.  Returns the number of threads used. */
+int orc_align_batch(int32_t n_reads, const char *seqs, const int64_t *seq_ptr,
+                    const int32_t *seq_len, const float *ev_means,
+                    const int64_t *ev_ptr, const int32_t *n_events,
+                    const orc_model_t *models, const float *scale,
+                    const float *shift, orc_pair_t *out,
+                    const int64_t *pair_ptr, int32_t *n_pairs, float *scores,
+                    int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,186 @@
+"""ctypes binding of the CPU oracle (oracle/abea_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product path never imports
+this module.
+
+The oracle restates f5c's align(): fill FPGA/code/device/align_swi.cl:230-503,
+traceback/QC FPGA/code/host/src/align_swi.cpp:1139-1312, emission
+GPU/align_cuda.cl:62-110 with the cached level_log_stdv.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
+
+MODE_F5C = 0
+MODE_FP64 = 1
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("score", C.c_double),
+        ("avg_log_emission", C.c_double),
+        ("path_len", C.c_int32),
+        ("max_gap", C.c_int32),
+        ("spanned", C.c_int32),
+        ("qc_pass", C.c_int32),
+        ("end_event", C.c_int32),
+        ("n_bands", C.c_int32),
+        ("fills", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_align.restype = C.c_int32
+        L.orc_align.argtypes = [P, C.c_int32, P, C.c_int32, P, C.c_int32, P, C.c_float,
+                                C.c_float, C.c_int, P, P, P, P, P, P, C.POINTER(Info)]
+        L.orc_kmer_rank.restype = C.c_uint32
+        L.orc_kmer_rank.argtypes = [C.c_char_p]
+        L.orc_transition_constants.restype = None
+        L.orc_transition_constants.argtypes = [C.c_int32, C.c_int32, P]
+        L.orc_emission_f32.restype = C.c_float
+        L.orc_emission_f32.argtypes = [C.c_float, P, C.c_float, C.c_float]
+        L.orc_path_gradient.restype = None
+        L.orc_path_gradient.argtypes = [P, P, C.c_int32, P, P, P, C.c_float, C.c_float,
+                                        C.c_double, P, P, P]
+        L.orc_align_batch.restype = C.c_int
+        L.orc_align_batch.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def transition_constants(n_events: int, n_kmers: int) -> np.ndarray:
+    out = np.zeros(4, np.float64)
+    lib().orc_transition_constants(n_events, n_kmers, _p(out))
+    return out
+
+
+def kmer_rank(s: bytes) -> int:
+    return int(lib().orc_kmer_rank(s))
+
+
+def emission(x: float, model_row: np.ndarray, scale: float, shift: float) -> float:
+    m = np.ascontiguousarray(model_row, np.float32)
+    return float(lib().orc_emission_f32(x, _p(m), scale, shift))
+
+
+def align(seq: bytes, ev_mean: np.ndarray, model: np.ndarray, scale: float, shift: float,
+          mode: int = MODE_F5C, replay_moves: np.ndarray | None = None, want_trace=False,
+          want_endcol=False):
+    """Align one read.  Returns a dict with pairs (after QC), path/from (before
+    QC), moves, info fields and optionally the reference-layout trace."""
+    seq = bytes(seq)
+    L = len(seq)
+    ev = np.ascontiguousarray(ev_mean, np.float32)
+    n_e = int(ev.shape[0])
+    model = np.ascontiguousarray(model, np.float32).reshape(4096, 3)
+    n_k = max(L - 5, 0)
+    n_bands = n_e + n_k + 2 if (n_k > 0 and n_e > 0) else 0
+    cap = n_e + n_k
+    out = np.zeros((max(cap, 1), 2), np.int32)
+    path = np.zeros((max(cap, 1), 2), np.int32)
+    frm = np.zeros(max(cap, 1), np.uint8)
+    moves = np.zeros(max(n_bands, 1), np.uint8)
+    trace = np.zeros(max(n_bands, 1) * 100, np.uint8) if want_trace else None
+    endcol = np.zeros(max(n_e, 1), np.float32) if want_endcol else None
+    rm = None
+    if replay_moves is not None:
+        rm = np.ascontiguousarray(replay_moves, np.uint8)
+        assert rm.shape[0] >= n_bands
+    info = Info()
+    sbuf = C.create_string_buffer(seq, L + 1)
+    n = lib().orc_align(_p(out), cap, C.cast(sbuf, C.c_void_p), L, _p(ev), n_e, _p(model),
+                        float(scale), float(shift), mode, _p(rm), _p(moves), _p(path),
+                        _p(frm), _p(trace), _p(endcol), C.byref(info))
+    pl = info.path_len
+    res = {
+        "n_pairs": int(n),
+        "pairs": out[:n].copy(),
+        "path": path[:pl].copy(),
+        "from": frm[:pl].copy(),
+        "moves": moves[:n_bands].copy(),
+        "score": info.score,
+        "avg_log_emission": info.avg_log_emission,
+        "path_len": pl,
+        "max_gap": info.max_gap,
+        "spanned": info.spanned,
+        "qc_pass": info.qc_pass,
+        "end_event": info.end_event,
+        "n_bands": info.n_bands,
+        "fills": info.fills,
+    }
+    if want_trace:
+        res["trace"] = trace[: n_bands * 100].reshape(n_bands, 100) if n_bands else trace[:0]
+    if want_endcol:
+        res["endcol"] = endcol[:n_e]
+    return res
+
+
+def path_gradient(path, frm, seq: bytes, ev_mean, model, scale, shift, weight=1.0):
+    """FP64 gradient of one read's best-path score along a fixed path.
+    Returns (g_read[3], g_mu[4096], g_sigma[4096])."""
+    path = np.ascontiguousarray(path, np.int32)
+    frm = np.ascontiguousarray(frm, np.uint8)
+    ev = np.ascontiguousarray(ev_mean, np.float32)
+    model = np.ascontiguousarray(model, np.float32).reshape(4096, 3)
+    g_read = np.zeros(3, np.float64)
+    g_mu = np.zeros(4096, np.float64)
+    g_sg = np.zeros(4096, np.float64)
+    sbuf = C.create_string_buffer(bytes(seq), len(seq) + 1)
+    lib().orc_path_gradient(_p(path), _p(frm), int(path.shape[0]), C.cast(sbuf, C.c_void_p),
+                            _p(ev), _p(model), float(scale), float(shift), float(weight),
+                            _p(g_read), _p(g_mu), _p(g_sg))
+    return g_read, g_mu, g_sg
+
+
+def align_batch(batch, n_threads: int = 0, want_pairs: bool = False):
+    """OpenMP CPU baseline over a Batch (abea_amd.synth.Batch layout).
+    Returns (n_pairs, scores, threads_used, pairs_or_None)."""
+    n = batch.n_reads
+    n_pairs = np.zeros(n, np.int32)
+    scores = np.zeros(n, np.float32)
+    pairs = None
+    pair_ptr = None
+    if want_pairs:
+        cap = batch.n_events.astype(np.int64) + np.maximum(batch.read_len.astype(np.int64) - 5, 0)
+        pair_ptr = np.zeros(n, np.int64)
+        pair_ptr[1:] = np.cumsum(cap)[:-1]
+        pairs = np.zeros((int(cap.sum()) + 1, 2), np.int32)
+    seqs = np.ascontiguousarray(batch.seq, np.uint8)
+    used = lib().orc_align_batch(
+        n, _p(seqs), _p(np.ascontiguousarray(batch.seq_ptr, np.int64)),
+        _p(np.ascontiguousarray(batch.read_len, np.int32)),
+        _p(np.ascontiguousarray(batch.ev_mean, np.float32)),
+        _p(np.ascontiguousarray(batch.ev_ptr, np.int64)),
+        _p(np.ascontiguousarray(batch.n_events, np.int32)),
+        _p(np.ascontiguousarray(batch.model, np.float32)),
+        _p(np.ascontiguousarray(batch.scale, np.float32)),
+        _p(np.ascontiguousarray(batch.shift, np.float32)),
+        _p(pairs), _p(pair_ptr), _p(n_pairs), _p(scores), int(n_threads))
+    return n_pairs, scores, int(used), (pairs, pair_ptr)
