@@ -1,0 +1,134 @@
+"""ctypes view of libabea's C ABI (include/abea.h).
+
+The library is loaded from abea-opencl_amd/lib/libabea.so (built in-tree by
+abea-opencl_amd/build.py).  A missing library is an error: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libabea.so")
+
+ABEA_OK = 0
+PLAN_PLAIN = 0
+PLAN_BATCHED = 1
+
+P = C.c_void_p
+
+
+class EventT(C.Structure):  # event_t, GPU/f5c.h:113-121
+    _fields_ = [("start", C.c_uint64), ("length", C.c_float), ("mean", C.c_float),
+                ("stdv", C.c_float)]
+
+
+class EventTable(C.Structure):  # event_table, GPU/f5c.h:124-130
+    _fields_ = [("n", C.c_size_t), ("start", C.c_size_t), ("end", C.c_size_t),
+                ("event", C.POINTER(EventT))]
+
+
+class ModelT(C.Structure):
+    _fields_ = [("level_mean", C.c_float), ("level_stdv", C.c_float),
+                ("level_log_stdv", C.c_float)]
+
+
+class ScalingsT(C.Structure):
+    _fields_ = [("scale", C.c_float), ("shift", C.c_float), ("var", C.c_float),
+                ("log_var", C.c_float)]
+
+
+class AlignedPair(C.Structure):
+    _fields_ = [("ref_pos", C.c_int32), ("read_pos", C.c_int32)]
+
+
+class BatchT(C.Structure):
+    _fields_ = [("n_reads", C.c_int32), ("seq", P), ("seq_ptr", P), ("read_len", P),
+                ("events", P), ("ev_mean", P), ("ev_ptr", P), ("n_events", P),
+                ("scalings", P), ("model", P), ("read_len_host", P), ("n_events_host", P),
+                ("seq_ptr_host", P), ("ev_ptr_host", P)]
+
+
+class OutputT(C.Structure):
+    _fields_ = [("pairs", P), ("path_from", P), ("n_pairs", P), ("path_len", P), ("score", P),
+                ("avg_log_emission", P)]
+
+
+class TuningT(C.Structure):
+    _fields_ = [("core_waves_per_block", C.c_int32), ("core_waves_per_cu", C.c_int32),
+                ("post_rows", C.c_int32), ("use_graph", C.c_int32)]
+
+
+# every symbol include/abea.h declares
+EXPORTS = [
+    "abea_last_error", "abea_align", "abea_align_db", "abea_ctx_create", "abea_ctx_destroy",
+    "abea_ctx_stream", "abea_pair_capacity", "abea_pair_offsets", "abea_plan_create",
+    "abea_plan_destroy", "abea_plan_forward", "abea_plan_move_words", "abea_plan_get_moves",
+    "abea_plan_set_replay", "abea_plan_adjoint", "abea_plan_get_tuning", "abea_plan_set_tuning",
+    "abea_plan_autotune", "abea_plan_last_times", "abea_plan_workspace_bytes",
+]
+
+_lib = None
+
+
+class AbeaError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AbeaError(f"libabea is not built: {LIB_PATH} missing "
+                        "(run python abea-opencl_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    L.abea_last_error.restype = C.c_char_p
+    L.abea_last_error.argtypes = []
+    L.abea_align.restype = C.c_int32
+    L.abea_align.argtypes = [P, C.c_char_p, C.c_int32, EventTable, P, ScalingsT, C.c_float]
+    L.abea_align_db.restype = C.c_int
+    L.abea_align_db.argtypes = [C.c_int32, P, P, P, P, P, P, P]
+    L.abea_ctx_create.restype = C.c_int
+    L.abea_ctx_create.argtypes = [C.POINTER(P), C.c_int, P]
+    L.abea_ctx_destroy.restype = None
+    L.abea_ctx_destroy.argtypes = [P]
+    L.abea_ctx_stream.restype = P
+    L.abea_ctx_stream.argtypes = [P]
+    L.abea_pair_capacity.restype = C.c_int64
+    L.abea_pair_capacity.argtypes = [C.c_int32, P, P]
+    L.abea_pair_offsets.restype = None
+    L.abea_pair_offsets.argtypes = [C.c_int32, P, P, P]
+    L.abea_plan_create.restype = C.c_int
+    L.abea_plan_create.argtypes = [P, C.POINTER(BatchT), C.c_int, C.POINTER(P)]
+    L.abea_plan_destroy.restype = None
+    L.abea_plan_destroy.argtypes = [P]
+    L.abea_plan_forward.restype = C.c_int
+    L.abea_plan_forward.argtypes = [P, C.POINTER(OutputT)]
+    L.abea_plan_move_words.restype = C.c_int64
+    L.abea_plan_move_words.argtypes = [P]
+    L.abea_plan_get_moves.restype = C.c_int
+    L.abea_plan_get_moves.argtypes = [P, P]
+    L.abea_plan_set_replay.restype = C.c_int
+    L.abea_plan_set_replay.argtypes = [P, P]
+    L.abea_plan_adjoint.restype = C.c_int
+    L.abea_plan_adjoint.argtypes = [P, C.POINTER(OutputT), P, P, P]
+    L.abea_plan_get_tuning.restype = C.c_int
+    L.abea_plan_get_tuning.argtypes = [P, C.POINTER(TuningT)]
+    L.abea_plan_set_tuning.restype = C.c_int
+    L.abea_plan_set_tuning.argtypes = [P, C.POINTER(TuningT)]
+    L.abea_plan_autotune.restype = C.c_int
+    L.abea_plan_autotune.argtypes = [P, C.POINTER(OutputT), C.c_int, C.POINTER(C.c_double)]
+    L.abea_plan_last_times.restype = C.c_int
+    L.abea_plan_last_times.argtypes = [P, C.POINTER(C.c_double)]
+    L.abea_plan_workspace_bytes.restype = C.c_int64
+    L.abea_plan_workspace_bytes.argtypes = [P]
+    _lib = L
+    return L
+
+
+def check(status: int, what: str) -> None:
+    if status != ABEA_OK:
+        msg = lib().abea_last_error().decode(errors="replace")
+        raise AbeaError(f"{what}: status {status}: {msg}")

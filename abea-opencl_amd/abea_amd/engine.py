@@ -1,0 +1,283 @@
+"""Device-resident ABEA batches over libabea's C ABI.
+
+torch provides device memory and the stream; every computation runs in
+libabea's HIP kernels.  Mirrors the reference's batch flow (align_ocl,
+GPU/align_cuda.c:391-943): flattened reads/events in, per-read
+n_event_align_pairs and AlignedPair lists out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .synth import Batch
+
+P = C.c_void_p
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else P(t.data_ptr())
+
+
+def _nptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+class Context:
+    """One libabea context bound to a device and torch's current stream."""
+
+    def __init__(self, device: int = 0, stream: torch.cuda.Stream | None = None):
+        if not torch.cuda.is_available():
+            raise _lib.AbeaError("libabea needs a HIP device; none is visible")
+        self.device = device
+        torch.cuda.set_device(device)
+        # a stream of our own (default priority): graphs cannot be captured on
+        # the legacy default stream; ordering with torch's current stream is
+        # made explicit around every launch (enter/leave)
+        self.stream = stream or torch.cuda.Stream(device=device)
+        h = P()
+        _lib.check(_lib.lib().abea_ctx_create(C.byref(h), device, P(self.stream.cuda_stream)),
+                   "abea_ctx_create")
+        self.handle = h
+
+    def enter(self):
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+
+    def leave(self):
+        torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+    def close(self):
+        if self.handle:
+            _lib.lib().abea_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class DeviceBatch:
+    """A Batch uploaded to HBM (inputs only)."""
+    host: Batch
+    seq: torch.Tensor
+    seq_ptr: torch.Tensor
+    read_len: torch.Tensor
+    ev_mean: torch.Tensor | None
+    events: torch.Tensor | None     # raw event_t records (uint8 view, 24 B each)
+    ev_ptr: torch.Tensor
+    n_events: torch.Tensor
+    scalings: torch.Tensor          # float32 [n, 4]
+    model: torch.Tensor             # float32 [4096, 3]
+    read_len_h: np.ndarray
+    n_events_h: np.ndarray
+    seq_ptr_h: np.ndarray
+    ev_ptr_h: np.ndarray
+
+    @staticmethod
+    def upload(b: Batch, device: int = 0, aos_events: bool = False) -> "DeviceBatch":
+        dev = torch.device("cuda", device)
+
+        def t(a, dtype=None):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(dev, non_blocking=False)
+
+        events = None
+        ev_mean = None
+        if aos_events:
+            events = t(b.events_aos().view(np.uint8))
+        else:
+            ev_mean = t(b.ev_mean.astype(np.float32))
+        return DeviceBatch(
+            host=b, seq=t(b.seq.astype(np.uint8)), seq_ptr=t(b.seq_ptr.astype(np.int64)),
+            read_len=t(b.read_len.astype(np.int32)), ev_mean=ev_mean, events=events,
+            ev_ptr=t(b.ev_ptr.astype(np.int64)), n_events=t(b.n_events.astype(np.int32)),
+            scalings=t(b.scalings()), model=t(b.model.astype(np.float32)),
+            read_len_h=np.ascontiguousarray(b.read_len, np.int32),
+            n_events_h=np.ascontiguousarray(b.n_events, np.int32),
+            seq_ptr_h=np.ascontiguousarray(b.seq_ptr, np.int64),
+            ev_ptr_h=np.ascontiguousarray(b.ev_ptr, np.int64))
+
+    @property
+    def n_reads(self) -> int:
+        return int(self.read_len_h.shape[0])
+
+    def c_struct(self) -> _lib.BatchT:
+        s = _lib.BatchT()
+        s.n_reads = self.n_reads
+        s.seq = _ptr(self.seq)
+        s.seq_ptr = _ptr(self.seq_ptr)
+        s.read_len = _ptr(self.read_len)
+        s.events = _ptr(self.events)
+        s.ev_mean = _ptr(self.ev_mean)
+        s.ev_ptr = _ptr(self.ev_ptr)
+        s.n_events = _ptr(self.n_events)
+        s.scalings = _ptr(self.scalings)
+        s.model = _ptr(self.model)
+        s.read_len_host = _nptr(self.read_len_h)
+        s.n_events_host = _nptr(self.n_events_h)
+        s.seq_ptr_host = _nptr(self.seq_ptr_h)
+        s.ev_ptr_host = _nptr(self.ev_ptr_h)
+        return s
+
+
+def pair_offsets(read_len: np.ndarray, n_events: np.ndarray) -> np.ndarray:
+    cap = n_events.astype(np.int64) + np.maximum(read_len.astype(np.int64) - 5, 0)
+    off = np.zeros(cap.shape[0], np.int64)
+    if cap.shape[0] > 1:
+        off[1:] = np.cumsum(cap)[:-1]
+    return off
+
+
+class Plan:
+    """A planned batch with its output buffers (device)."""
+
+    def __init__(self, ctx: Context, batch: DeviceBatch, form: int = _lib.PLAN_BATCHED):
+        self.ctx = ctx
+        self.batch = batch
+        self.form = form
+        self._cb = batch.c_struct()  # keep host arrays referenced
+        h = P()
+        _lib.check(_lib.lib().abea_plan_create(ctx.handle, C.byref(self._cb), form, C.byref(h)),
+                   "abea_plan_create")
+        self.handle = h
+        n = batch.n_reads
+        self.pair_off = pair_offsets(batch.read_len_h, batch.n_events_h)
+        cap = int(_lib.lib().abea_pair_capacity(n, _nptr(batch.read_len_h),
+                                                _nptr(batch.n_events_h)))
+        dev = torch.device("cuda", ctx.device)
+        self.capacity = cap
+        self.pairs = torch.zeros((max(cap, 1), 2), dtype=torch.int32, device=dev)
+        self.path_from = torch.zeros(max(cap, 1), dtype=torch.uint8, device=dev)
+        self.n_pairs = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.path_len = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.score = torch.zeros(max(n, 1), dtype=torch.float32, device=dev)
+        self.avg = torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
+        self.out = _lib.OutputT(_ptr(self.pairs), _ptr(self.path_from), _ptr(self.n_pairs),
+                                _ptr(self.path_len), _ptr(self.score), _ptr(self.avg))
+        torch.cuda.synchronize(ctx.device)  # inputs uploaded and outputs cleared
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().abea_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- runs
+    def forward(self):
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_forward(self.handle, C.byref(self.out)), "abea_plan_forward")
+        self.ctx.leave()
+
+    def adjoint(self, grad_score: torch.Tensor):
+        n = self.batch.n_reads
+        dev = grad_score.device
+        gs = grad_score.contiguous().to(torch.float32)
+        g_scal = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+        g_model = torch.empty((4096, 2), dtype=torch.float64, device=dev)
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_adjoint(self.handle, C.byref(self.out), _ptr(gs),
+                                                _ptr(g_scal), _ptr(g_model)), "abea_plan_adjoint")
+        self.ctx.leave()
+        gs.record_stream(self.ctx.stream)
+        return g_scal[:n], g_model
+
+    def times(self) -> list[float]:
+        t = (C.c_double * 4)()
+        _lib.check(_lib.lib().abea_plan_last_times(self.handle, t), "abea_plan_last_times")
+        return [float(x) for x in t]
+
+    def autotune(self, reps: int = 1) -> float:
+        ms = C.c_double()
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_autotune(self.handle, C.byref(self.out), reps, C.byref(ms)),
+                   "abea_plan_autotune")
+        self.ctx.leave()
+        return ms.value
+
+    def tuning(self) -> dict:
+        t = _lib.TuningT()
+        _lib.check(_lib.lib().abea_plan_get_tuning(self.handle, C.byref(t)), "get_tuning")
+        return {f: getattr(t, f) for f, _ in _lib.TuningT._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = _lib.TuningT(**cur)
+        _lib.check(_lib.lib().abea_plan_set_tuning(self.handle, C.byref(t)), "set_tuning")
+
+    def moves(self) -> np.ndarray:
+        nw = int(_lib.lib().abea_plan_move_words(self.handle))
+        m = np.zeros(max(nw, 1), np.uint32)
+        _lib.check(_lib.lib().abea_plan_get_moves(self.handle, _nptr(m)), "get_moves")
+        return m[:nw]
+
+    def set_replay(self, moves: np.ndarray | None):
+        if moves is None:
+            _lib.check(_lib.lib().abea_plan_set_replay(self.handle, None), "set_replay")
+            return
+        m = np.ascontiguousarray(moves, np.uint32)
+        self._replay = m
+        _lib.check(_lib.lib().abea_plan_set_replay(self.handle, _nptr(m)), "set_replay")
+
+    def workspace_bytes(self) -> int:
+        return int(_lib.lib().abea_plan_workspace_bytes(self.handle))
+
+    # ---- results
+    def results(self) -> dict:
+        """Copy outputs to host.  pairs_i = pairs[off[i]:off[i]+path_len[i]]."""
+        torch.cuda.synchronize(self.ctx.device)
+        n = self.batch.n_reads
+        return {
+            "pairs": self.pairs.cpu().numpy(),
+            "path_from": self.path_from.cpu().numpy(),
+            "n_pairs": self.n_pairs[:n].cpu().numpy(),
+            "path_len": self.path_len[:n].cpu().numpy(),
+            "score": self.score[:n].cpu().numpy(),
+            "avg": self.avg[:n].cpu().numpy(),
+            "pair_off": self.pair_off,
+        }
+
+
+def move_offsets(read_len: np.ndarray, n_events: np.ndarray) -> np.ndarray:
+    """Offsets (uint32 words) of each read's band-move bits in Plan.moves()."""
+    nk = read_len.astype(np.int64) - 5
+    ok = (nk > 0) & (n_events > 0)
+    nb = np.where(ok, n_events.astype(np.int64) + nk + 2, 0)
+    words = (nb + 31) // 32
+    off = np.zeros(words.shape[0], np.int64)
+    if words.shape[0] > 1:
+        off[1:] = np.cumsum(words)[:-1]
+    return off
+
+
+def unpack_moves(words: np.ndarray, off: int, n_bands: int) -> np.ndarray:
+    w = words[off:off + (n_bands + 31) // 32].astype(np.uint32)
+    bits = ((w[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1).astype(np.uint8)
+    return bits.reshape(-1)[:n_bands]
+
+
+def pack_moves(bits_per_read: list, read_len, n_events) -> np.ndarray:
+    off = move_offsets(np.asarray(read_len), np.asarray(n_events))
+    total = 0
+    out_parts = []
+    for bits in bits_per_read:
+        nb = bits.shape[0]
+        words = (nb + 31) // 32
+        pad = np.zeros(words * 32, np.uint32)
+        pad[:nb] = bits
+        out_parts.append((pad.reshape(words, 32) << np.arange(32, dtype=np.uint32)).sum(1)
+                         .astype(np.uint32))
+        total += words
+    assert len(off) == len(out_parts)
+    return np.concatenate(out_parts) if out_parts else np.zeros(0, np.uint32)

@@ -1,0 +1,767 @@
+// Host side of libabea: contexts, batch plans, graph replay, the autotuner
+// and the f5c drop-in entry points declared in include/abea.h.
+//
+// Orchestration replaces the reference's align_ocl() (GPU/align_cuda.c:
+// 391-943): flatten -> buffers -> pre/core/post -> readback.  Differences by
+// design: device buffers are planned once per batch and reused, reads are
+// queued longest-first for a persistent core, and the three launches are
+// replayed as one hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "abea_internal.h"
+
+using abea::KernelArgs;
+using abea::KmerParam;
+using abea::ReadMeta;
+
+namespace {
+
+thread_local std::string g_err;
+
+abea_status_t fail(abea_status_t st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define HIP_TRY(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ABEA_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  abea_status_t ensure(size_t bytes) {
+    if (bytes <= n && p) return ABEA_OK;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(ABEA_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+    }
+    n = want;
+    return ABEA_OK;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+// Per-read transition constants exactly as the CPU reference computes them
+// (FPGA/code/device/align_swi.cl:233-250): double p_stay, glibc log/exp.
+void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_step) {
+  double events_per_kmer = (double)n_e / n_k;
+  double p_stay = 1 - (1 / (events_per_kmer + 1));
+  double lp_skip = log(1e-10);
+  *lp_stay = log(p_stay);
+  *lp_step = log(1.0 - exp(lp_skip) - exp(*lp_stay));
+}
+
+}  // namespace
+
+struct abea_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int n_cu = 256;
+  std::string arch;
+};
+
+struct abea_plan {
+  abea_ctx *ctx = nullptr;
+  int form = ABEA_PLAN_BATCHED;
+  int32_t n = 0;
+  abea_batch_t batch{};
+  std::vector<ReadMeta> meta_h;
+  std::vector<int32_t> order_h;
+  int64_t total_pairs = 0, trace_bytes = 0, move_words = 0, km_count = 0, ev_count = 0;
+  int64_t n_bands_total = 0;
+  DevBuf meta, queue, counter, km, ev, endcol, trace, moves, replay, rev;
+  bool replay_on = false;
+  double lp_skip = 0, lp_trim = 0;
+  abea_tuning_t tune{1, 24, 64, 1};
+  // graph cache
+  hipGraphExec_t gexec = nullptr;
+  abea_output_t gout{};
+  abea_tuning_t gtune{};
+  bool greplay = false;
+  bool graph_timing_ok = true;
+  hipEvent_t evt[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double times[4] = {0, 0, 0, 0};
+  bool have_times = false;
+
+  ~abea_plan() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (auto &e : evt)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+extern "C" {
+
+const char *abea_last_error(void) { return g_err.c_str(); }
+
+int64_t abea_pair_capacity(int32_t n_reads, const int32_t *read_len_host,
+                           const int32_t *n_events_host) {
+  int64_t t = 0;
+  for (int32_t i = 0; i < n_reads; ++i)
+    t += (int64_t)n_events_host[i] + std::max<int64_t>((int64_t)read_len_host[i] - 5, 0);
+  return t;
+}
+
+void abea_pair_offsets(int32_t n_reads, const int32_t *read_len_host, const int32_t *n_events_host,
+                       int64_t *offsets) {
+  int64_t t = 0;
+  for (int32_t i = 0; i < n_reads; ++i) {
+    offsets[i] = t;
+    t += (int64_t)n_events_host[i] + std::max<int64_t>((int64_t)read_len_host[i] - 5, 0);
+  }
+}
+
+abea_status_t abea_ctx_create(abea_ctx_t **out, int device, void *stream) {
+  if (!out) return fail(ABEA_ERR_INVALID, "abea_ctx_create: null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+    return fail(ABEA_ERR_NO_DEVICE, "abea_ctx_create: no HIP device %d (found %d)", device, count);
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  std::string arch = prop.gcnArchName;
+  if (arch.rfind("gfx950", 0) != 0)
+    return fail(ABEA_ERR_NO_DEVICE, "abea_ctx_create: device %d is %s; libabea is built for gfx950",
+                device, arch.c_str());
+  abea_ctx *c = new abea_ctx();
+  c->device = device;
+  c->arch = arch;
+  c->n_cu = prop.multiProcessorCount;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(ABEA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return ABEA_OK;
+}
+
+void abea_ctx_destroy(abea_ctx_t *c) {
+  if (!c) return;
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void *abea_ctx_stream(abea_ctx_t *c) { return c ? (void *)c->stream : nullptr; }
+
+}  // extern "C"
+
+namespace {
+
+// (Re)plan a batch on an existing plan object; buffers only grow.
+abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
+  if (!b || b->n_reads < 0) return fail(ABEA_ERR_INVALID, "plan: bad batch");
+  if (b->n_reads > 0 && (!b->read_len_host || !b->n_events_host || !b->ev_ptr_host ||
+                         !b->seq || !b->seq_ptr || !b->scalings || !b->model ||
+                         (!b->events && !b->ev_mean)))
+    return fail(ABEA_ERR_INVALID, "plan: missing batch arrays");
+  const int32_t n = b->n_reads;
+  p->batch = *b;
+  p->n = n;
+  p->form = form;
+  p->meta_h.assign(n, ReadMeta{});
+  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    ReadMeta &m = p->meta_h[i];
+    const int32_t n_e = b->n_events_host[i];
+    const int32_t n_k = b->read_len_host[i] - ABEA_KMER_SIZE + 1;
+    if (n_e < 0) return fail(ABEA_ERR_INVALID, "plan: read %d has %d events", i, n_e);
+    m.n_e = n_e;
+    m.n_k = n_k;
+    m.ev_off = b->ev_ptr_host[i];
+    evmax = std::max<int64_t>(evmax, m.ev_off + n_e);
+    m.pair_off = pr;
+    pr += (int64_t)n_e + std::max<int32_t>(n_k, 0);
+    m.km_off = km;
+    m.trace_off = tr;
+    m.mv_off = mv;
+    if (n_k > 0 && n_e > 0) {
+      const int64_t bands = (int64_t)n_e + n_k + 2;
+      km += n_k;
+      tr += bands * abea::kRowBytes;
+      mv += (bands + 31) / 32;
+      nb += bands;
+      transition_constants(n_e, n_k, &m.lp_stay, &m.lp_step);
+    }
+  }
+  p->total_pairs = pr;
+  p->trace_bytes = tr;
+  p->move_words = mv;
+  p->km_count = km;
+  p->ev_count = evmax;
+  p->n_bands_total = nb;
+  p->lp_skip = log(1e-10);
+  p->lp_trim = log(0.01);
+
+  // processing order: the plain form walks reads in input order; the batched
+  // form queues them longest-first so the persistent core ends balanced
+  p->order_h.resize(n);
+  std::iota(p->order_h.begin(), p->order_h.end(), 0);
+  if (form == ABEA_PLAN_BATCHED) {
+    std::stable_sort(p->order_h.begin(), p->order_h.end(), [&](int32_t x, int32_t y) {
+      const ReadMeta &a = p->meta_h[x], &c = p->meta_h[y];
+      const int64_t ba = (int64_t)a.n_e + std::max(a.n_k, 0), bc = (int64_t)c.n_e + std::max(c.n_k, 0);
+      return ba > bc;
+    });
+  }
+
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  const size_t need = (size_t)tr + (size_t)km * sizeof(KmerParam) + (size_t)evmax * 8 +
+                      (size_t)mv * 8 + (size_t)pr + (size_t)n * (sizeof(ReadMeta) + 4);
+  if (free_b && need > free_b + p->trace.n + p->km.n + p->ev.n + p->endcol.n + p->moves.n +
+                           p->replay.n + p->rev.n + p->meta.n)
+    return fail(ABEA_ERR_NOMEM, "plan: workspace of %.2f GB exceeds free device memory (%.2f GB)",
+                need / 1e9, free_b / 1e9);
+
+  abea_status_t st;
+  if ((st = p->meta.ensure(sizeof(ReadMeta) * (size_t)std::max(n, 1)))) return st;
+  if ((st = p->queue.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
+  if ((st = p->counter.ensure(256))) return st;
+  if ((st = p->km.ensure(sizeof(KmerParam) * (size_t)std::max<int64_t>(km, 1)))) return st;
+  if (b->events) {
+    if ((st = p->ev.ensure(sizeof(float) * (size_t)std::max<int64_t>(evmax, 1)))) return st;
+  }
+  if ((st = p->endcol.ensure(sizeof(float) * (size_t)std::max<int64_t>(evmax, 1)))) return st;
+  if ((st = p->trace.ensure((size_t)std::max<int64_t>(tr, 1)))) return st;
+  if ((st = p->moves.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mv, 1)))) return st;
+  if ((st = p->rev.ensure((size_t)std::max<int64_t>(pr, 1)))) return st;
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(p->meta.p, p->meta_h.data(), sizeof(ReadMeta) * n,
+                           hipMemcpyHostToDevice, p->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(p->queue.p, p->order_h.data(), sizeof(int32_t) * n,
+                           hipMemcpyHostToDevice, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+  }
+  p->replay_on = false;
+  if (p->gexec) {
+    (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+  }
+  p->tune.use_graph = form == ABEA_PLAN_BATCHED ? 1 : 0;
+  return ABEA_OK;
+}
+
+KernelArgs make_args(const abea_plan *p, const abea_output_t *out) {
+  KernelArgs a{};
+  a.meta = p->meta.as<ReadMeta>();
+  a.queue = p->queue.as<int32_t>();
+  a.n_queue = p->n;
+  a.counter = p->counter.as<int32_t>();
+  a.seq = p->batch.seq;
+  a.seq_ptr = p->batch.seq_ptr;
+  a.events = p->batch.events;
+  a.ev_mean_in = p->batch.ev_mean;
+  a.scalings = p->batch.scalings;
+  a.model = p->batch.model;
+  a.km = p->km.as<KmerParam>();
+  a.ev = p->batch.events ? p->ev.as<float>() : const_cast<float *>(p->batch.ev_mean);
+  a.endcol = p->endcol.as<float>();
+  a.trace = p->trace.as<uint8_t>();
+  a.moves = p->moves.as<uint32_t>();
+  a.replay = p->replay_on ? p->replay.as<uint32_t>() : nullptr;
+  a.rev_from = p->rev.as<uint8_t>();
+  a.lp_skip = p->lp_skip;
+  a.lp_trim = p->lp_trim;
+  if (out) a.out = *out;
+  return a;
+}
+
+abea_status_t ensure_events(abea_plan *p) {
+  for (auto &e : p->evt)
+    if (!e) HIP_TRY(hipEventCreate(&e));
+  return ABEA_OK;
+}
+
+// The launch sequence of one forward; evt non-null records per-kernel
+// boundaries on the stream.
+abea_status_t enqueue_forward(abea_plan *p, const KernelArgs &a, bool with_events) {
+  hipStream_t s = p->ctx->stream;
+  const abea_tuning_t &t = p->tune;
+  if (with_events) HIP_TRY(hipEventRecord(p->evt[0], s));
+  if (p->form == ABEA_PLAN_PLAIN) {
+    // reference-shaped: one launch set per read, reads in input order
+    for (int32_t i = 0; i < p->n; ++i) {
+      HIP_TRY(abea::launch_pre(a, i, 1, s));
+      HIP_TRY(abea::launch_core(a, i, 1, false, 1, 1, s));
+      HIP_TRY(abea::launch_post(a, i, 1, t.post_rows, s));
+    }
+    if (with_events) {
+      HIP_TRY(hipEventRecord(p->evt[1], s));
+      HIP_TRY(hipEventRecord(p->evt[2], s));
+      HIP_TRY(hipEventRecord(p->evt[3], s));
+    }
+    return ABEA_OK;
+  }
+  HIP_TRY(abea::launch_pre(a, 0, p->n, s));
+  if (with_events) HIP_TRY(hipEventRecord(p->evt[1], s));
+  const int wpb = t.core_waves_per_block;
+  int grid = std::max(1, (p->ctx->n_cu * t.core_waves_per_cu) / wpb);
+  grid = std::min(grid, std::max(1, (p->n + wpb - 1) / wpb));
+  HIP_TRY(abea::launch_core(a, 0, p->n, true, wpb, grid, s));
+  if (with_events) HIP_TRY(hipEventRecord(p->evt[2], s));
+  HIP_TRY(abea::launch_post(a, 0, p->n, t.post_rows, s));
+  if (with_events) HIP_TRY(hipEventRecord(p->evt[3], s));
+  return ABEA_OK;
+}
+
+bool same_out(const abea_output_t &x, const abea_output_t &y) {
+  return std::memcmp(&x, &y, sizeof(x)) == 0;
+}
+bool same_tune(const abea_tuning_t &x, const abea_tuning_t &y) {
+  return std::memcmp(&x, &y, sizeof(x)) == 0;
+}
+
+abea_status_t check_out(const abea_plan *p, const abea_output_t *out) {
+  if (!out || (p->n > 0 && (!out->pairs || !out->n_pairs)))
+    return fail(ABEA_ERR_INVALID, "forward: output pairs and n_pairs are required");
+  return ABEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+abea_status_t abea_plan_create(abea_ctx_t *ctx, const abea_batch_t *batch, int form,
+                               abea_plan_t **plan) {
+  if (!ctx || !plan) return fail(ABEA_ERR_INVALID, "abea_plan_create: null argument");
+  if (form != ABEA_PLAN_PLAIN && form != ABEA_PLAN_BATCHED)
+    return fail(ABEA_ERR_INVALID, "abea_plan_create: unknown form %d", form);
+  HIP_TRY(hipSetDevice(ctx->device));
+  abea_plan *p = new abea_plan();
+  p->ctx = ctx;
+  abea_status_t st = plan_bind(p, batch, form);
+  if (st) {
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return ABEA_OK;
+}
+
+void abea_plan_destroy(abea_plan_t *p) { delete p; }
+
+abea_status_t abea_plan_forward(abea_plan_t *p, const abea_output_t *out) {
+  if (!p) return fail(ABEA_ERR_INVALID, "forward: null plan");
+  abea_status_t st = check_out(p, out);
+  if (st) return st;
+  if (p->n == 0) return ABEA_OK;
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  if ((st = ensure_events(p))) return st;
+  const KernelArgs a = make_args(p, out);
+  hipStream_t s = p->ctx->stream;
+  p->have_times = false;
+  if (p->tune.use_graph && p->form == ABEA_PLAN_BATCHED) {
+    if (!p->gexec || !same_out(p->gout, *out) || !same_tune(p->gtune, p->tune) ||
+        p->greplay != p->replay_on) {
+      if (p->gexec) {
+        (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      abea_status_t cst = enqueue_forward(p, a, p->graph_timing_ok);
+      hipError_t ce = hipStreamEndCapture(s, &g);
+      if (cst || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (p->graph_timing_ok) {
+          // event records inside a captured graph were refused: capture
+          // without them and time the graph as a whole
+          p->graph_timing_ok = false;
+          return abea_plan_forward(p, out);
+        }
+        return cst ? cst : fail(ABEA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+      }
+      hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        p->gexec = nullptr;
+        return fail(ABEA_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+      }
+      p->gout = *out;
+      p->gtune = p->tune;
+      p->greplay = p->replay_on;
+    }
+    if (!p->graph_timing_ok) HIP_TRY(hipEventRecord(p->evt[0], s));
+    HIP_TRY(hipGraphLaunch(p->gexec, s));
+    if (!p->graph_timing_ok) HIP_TRY(hipEventRecord(p->evt[3], s));
+    p->have_times = true;
+    return ABEA_OK;
+  }
+  st = enqueue_forward(p, a, true);
+  if (!st) p->have_times = true;
+  return st;
+}
+
+abea_status_t abea_plan_last_times(const abea_plan_t *pc, double times_ms[4]) {
+  abea_plan *p = const_cast<abea_plan *>(pc);
+  if (!p || !times_ms) return fail(ABEA_ERR_INVALID, "last_times: null argument");
+  for (int i = 0; i < 4; ++i) times_ms[i] = -1.0;
+  if (p->have_times && p->evt[0]) {
+    HIP_TRY(hipEventSynchronize(p->evt[3]));
+    float ms = 0;
+    const bool split = !(p->tune.use_graph && p->form == ABEA_PLAN_BATCHED && !p->graph_timing_ok);
+    if (split && p->form == ABEA_PLAN_BATCHED) {
+      HIP_TRY(hipEventElapsedTime(&ms, p->evt[0], p->evt[1]));
+      times_ms[0] = ms;
+      HIP_TRY(hipEventElapsedTime(&ms, p->evt[1], p->evt[2]));
+      times_ms[1] = ms;
+      HIP_TRY(hipEventElapsedTime(&ms, p->evt[2], p->evt[3]));
+      times_ms[2] = ms;
+    } else {
+      // whole forward only (plain form interleaves the kernels)
+      HIP_TRY(hipEventElapsedTime(&ms, p->evt[0], p->evt[3]));
+      times_ms[1] = ms;
+    }
+  }
+  times_ms[3] = p->times[3];
+  return ABEA_OK;
+}
+
+int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
+  if (!p) return 0;
+  return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->ev.n + p->endcol.n +
+                   p->trace.n + p->moves.n + p->replay.n + p->rev.n);
+}
+
+int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }
+
+abea_status_t abea_plan_get_moves(abea_plan_t *p, uint32_t *moves_host) {
+  if (!p || !moves_host) return fail(ABEA_ERR_INVALID, "get_moves: null argument");
+  if (p->move_words == 0) return ABEA_OK;
+  HIP_TRY(hipMemcpyAsync(moves_host, p->moves.p, sizeof(uint32_t) * p->move_words,
+                         hipMemcpyDeviceToHost, p->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_set_replay(abea_plan_t *p, const uint32_t *moves_host) {
+  if (!p) return fail(ABEA_ERR_INVALID, "set_replay: null plan");
+  if (!moves_host) {
+    p->replay_on = false;
+    return ABEA_OK;
+  }
+  abea_status_t st = p->replay.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(p->move_words, 1));
+  if (st) return st;
+  if (p->move_words)
+    HIP_TRY(hipMemcpyAsync(p->replay.p, moves_host, sizeof(uint32_t) * p->move_words,
+                           hipMemcpyHostToDevice, p->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+  p->replay_on = true;
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_adjoint(abea_plan_t *p, const abea_output_t *fwd, const float *grad_score,
+                                float *grad_scalings, double *grad_model) {
+  if (!p || !fwd || !grad_score || !grad_scalings || !grad_model)
+    return fail(ABEA_ERR_INVALID, "adjoint: null argument");
+  if (!fwd->pairs || !fwd->path_from || !fwd->path_len)
+    return fail(ABEA_ERR_INVALID, "adjoint: the forward must record pairs, path_from and path_len");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  abea_status_t st = ensure_events(p);
+  if (st) return st;
+  hipStream_t s = p->ctx->stream;
+  const KernelArgs a = make_args(p, fwd);
+  HIP_TRY(hipEventRecord(p->evt[4], s));
+  HIP_TRY(abea::launch_zero_f64(grad_model, 2 * ABEA_NUM_KMER, s));
+  HIP_TRY(abea::launch_adjoint(a, grad_score, grad_scalings, grad_model, s));
+  HIP_TRY(hipEventRecord(p->evt[3], s));
+  HIP_TRY(hipEventSynchronize(p->evt[3]));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, p->evt[4], p->evt[3]));
+  p->times[3] = ms;
+  p->have_times = false;
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_get_tuning(const abea_plan_t *p, abea_tuning_t *t) {
+  if (!p || !t) return fail(ABEA_ERR_INVALID, "get_tuning: null argument");
+  *t = p->tune;
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_set_tuning(abea_plan_t *p, const abea_tuning_t *t) {
+  if (!p || !t) return fail(ABEA_ERR_INVALID, "set_tuning: null argument");
+  const int w = t->core_waves_per_block;
+  if ((w != 1 && w != 2 && w != 4) || t->core_waves_per_cu < 1 || t->core_waves_per_cu > 64 ||
+      (t->post_rows != 32 && t->post_rows != 64 && t->post_rows != 128))
+    return fail(ABEA_ERR_INVALID, "set_tuning: unsupported values (%d, %d, %d)", w,
+                t->core_waves_per_cu, t->post_rows);
+  p->tune = *t;
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_autotune(abea_plan_t *p, const abea_output_t *out, int reps,
+                                 double *best_ms) {
+  // The project autotuner: times candidate launch shapes of the core
+  // (waves per workgroup x resident waves per CU of the persistent grid) and
+  // the traceback's LDS depth (trace rows per chunk) on this plan's own
+  // batch, each with event timing of that kernel alone, and keeps the best.
+  if (!p) return fail(ABEA_ERR_INVALID, "autotune: null plan");
+  if (p->form != ABEA_PLAN_BATCHED) return fail(ABEA_ERR_INVALID, "autotune: batched plans only");
+  abea_status_t st = check_out(p, out);
+  if (st) return st;
+  if (reps < 1) reps = 1;
+  const abea_tuning_t saved = p->tune;
+  abea_tuning_t t = saved;
+  t.use_graph = 0;
+  struct Cand { int wpb, wpc; };
+  const Cand cands[] = {{1, 8}, {1, 16}, {1, 24}, {1, 32}, {2, 16}, {2, 32}, {4, 16}, {4, 32}};
+  double best_core = 1e30;
+  Cand best_c = {saved.core_waves_per_block, saved.core_waves_per_cu};
+  for (const Cand &c : cands) {
+    t.core_waves_per_block = c.wpb;
+    t.core_waves_per_cu = c.wpc;
+    p->tune = t;
+    double acc = 0;
+    for (int r = 0; r < reps; ++r) {
+      if ((st = abea_plan_forward(p, out))) { p->tune = saved; return st; }
+      double tm[4];
+      if ((st = abea_plan_last_times(p, tm))) { p->tune = saved; return st; }
+      acc += tm[1];
+    }
+    if (acc / reps < best_core) {
+      best_core = acc / reps;
+      best_c = c;
+    }
+  }
+  t.core_waves_per_block = best_c.wpb;
+  t.core_waves_per_cu = best_c.wpc;
+  double best_post = 1e30;
+  int best_rows = saved.post_rows;
+  for (int rows : {32, 64, 128}) {
+    t.post_rows = rows;
+    p->tune = t;
+    double acc = 0;
+    for (int r = 0; r < reps; ++r) {
+      if ((st = abea_plan_forward(p, out))) { p->tune = saved; return st; }
+      double tm[4];
+      if ((st = abea_plan_last_times(p, tm))) { p->tune = saved; return st; }
+      acc += tm[2];
+    }
+    if (acc / reps < best_post) {
+      best_post = acc / reps;
+      best_rows = rows;
+    }
+  }
+  t.post_rows = best_rows;
+  t.use_graph = saved.use_graph;
+  p->tune = t;
+  if (best_ms) *best_ms = best_core + best_post;
+  return ABEA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// f5c drop-in entries over host buffers.
+namespace {
+
+struct HostRun {
+  abea_ctx *ctx = nullptr;
+  abea_plan *plan = nullptr;
+  DevBuf seq, seq_ptr, read_len, events, ev_ptr, n_events, scal, model, pairs, from, n_pairs,
+      path_len, score;
+  ~HostRun() {
+    delete plan;
+    abea_ctx_destroy(ctx);
+  }
+};
+
+thread_local HostRun *t_run = nullptr;
+
+abea_status_t host_run_get(HostRun **out) {
+  if (!t_run) {
+    HostRun *r = new HostRun();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    abea_status_t st = abea_ctx_create(&r->ctx, dev, nullptr);
+    if (st) {
+      delete r;
+      return st;
+    }
+    r->plan = new abea_plan();
+    r->plan->ctx = r->ctx;
+    t_run = r;
+  }
+  *out = t_run;
+  return ABEA_OK;
+}
+
+template <class T>
+abea_status_t upload(DevBuf &d, const T *h, size_t count, hipStream_t s) {
+  abea_status_t st = d.ensure(sizeof(T) * std::max<size_t>(count, 1));
+  if (st) return st;
+  if (count) HIP_TRY(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+  return ABEA_OK;
+}
+
+// Run a host-resident batch; pairs_h gets all pair slots (abea_pair_offsets layout).
+abea_status_t run_host_batch(int form, int32_t n, const std::vector<char> &seq,
+                             const std::vector<int64_t> &seq_ptr, const std::vector<int32_t> &read_len,
+                             const std::vector<abea_event_t> &events,
+                             const std::vector<int64_t> &ev_ptr, const std::vector<int32_t> &n_events,
+                             const abea_scalings_t *scal, const abea_model_t *model,
+                             std::vector<abea_aligned_pair_t> &pairs_h,
+                             std::vector<int32_t> &n_pairs_h, std::vector<int64_t> &pair_off) {
+  HostRun *r = nullptr;
+  abea_status_t st = host_run_get(&r);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(r->ctx->device));
+  hipStream_t s = r->ctx->stream;
+  if ((st = upload(r->seq, seq.data(), seq.size(), s))) return st;
+  if ((st = upload(r->seq_ptr, seq_ptr.data(), seq_ptr.size(), s))) return st;
+  if ((st = upload(r->read_len, read_len.data(), read_len.size(), s))) return st;
+  if ((st = upload(r->events, events.data(), events.size(), s))) return st;
+  if ((st = upload(r->ev_ptr, ev_ptr.data(), ev_ptr.size(), s))) return st;
+  if ((st = upload(r->n_events, n_events.data(), n_events.size(), s))) return st;
+  if ((st = upload(r->scal, scal, (size_t)n, s))) return st;
+  if ((st = upload(r->model, model, (size_t)ABEA_NUM_KMER, s))) return st;
+  abea_batch_t b{};
+  b.n_reads = n;
+  b.seq = r->seq.as<char>();
+  b.seq_ptr = r->seq_ptr.as<int64_t>();
+  b.read_len = r->read_len.as<int32_t>();
+  b.events = r->events.as<abea_event_t>();
+  b.ev_ptr = r->ev_ptr.as<int64_t>();
+  b.n_events = r->n_events.as<int32_t>();
+  b.scalings = r->scal.as<abea_scalings_t>();
+  b.model = r->model.as<abea_model_t>();
+  b.read_len_host = read_len.data();
+  b.n_events_host = n_events.data();
+  b.seq_ptr_host = seq_ptr.data();
+  b.ev_ptr_host = ev_ptr.data();
+  if ((st = plan_bind(r->plan, &b, form))) return st;
+  const int64_t cap = r->plan->total_pairs;
+  if ((st = r->pairs.ensure(sizeof(abea_aligned_pair_t) * (size_t)std::max<int64_t>(cap, 1)))) return st;
+  if ((st = r->n_pairs.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
+  abea_output_t out{};
+  out.pairs = r->pairs.as<abea_aligned_pair_t>();
+  out.n_pairs = r->n_pairs.as<int32_t>();
+  if ((st = abea_plan_forward(r->plan, &out))) return st;
+  pairs_h.resize((size_t)cap);
+  n_pairs_h.resize(n);
+  pair_off.resize(n);
+  abea_pair_offsets(n, read_len.data(), n_events.data(), pair_off.data());
+  if (cap)
+    HIP_TRY(hipMemcpyAsync(pairs_h.data(), out.pairs, sizeof(abea_aligned_pair_t) * cap,
+                           hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(n_pairs_h.data(), out.n_pairs, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return ABEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_len,
+                   abea_event_table events, abea_model_t *models, abea_scalings_t scaling,
+                   float sample_rate) {
+  (void)sample_rate;  // unused by the reference too, GPU/align_cuda.cl:84-88
+  if (!out_2 || !sequence || !models || (events.n && !events.event)) {
+    fail(ABEA_ERR_INVALID, "abea_align: null argument");
+    return -1;
+  }
+  const int32_t n_e = (int32_t)events.n;
+  std::vector<char> seq(sequence, sequence + std::max(sequence_len, 0));
+  std::vector<int64_t> seq_ptr{0}, ev_ptr{0};
+  std::vector<int32_t> read_len{sequence_len}, n_events{n_e};
+  std::vector<abea_event_t> ev(events.event, events.event + events.n);
+  std::vector<abea_aligned_pair_t> pairs;
+  std::vector<int32_t> n_pairs;
+  std::vector<int64_t> off;
+  if (run_host_batch(ABEA_PLAN_PLAIN, 1, seq, seq_ptr, read_len, ev, ev_ptr, n_events, &scaling,
+                     models, pairs, n_pairs, off))
+    return -1;
+  int32_t np = n_pairs[0];
+  if ((int64_t)np > 2 * (int64_t)n_e) return 0;  // would overrun f5c's 2*n_events buffer
+  std::memcpy(out_2, pairs.data(), sizeof(abea_aligned_pair_t) * np);
+  return np;
+}
+
+abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
+                            const abea_event_table *et, const abea_model_t *model,
+                            const abea_scalings_t *scalings,
+                            abea_aligned_pair_t **event_align_pairs,
+                            int32_t *n_event_align_pairs) {
+  if (n_bam_rec < 0 || (n_bam_rec > 0 && (!read || !read_len || !et || !model || !scalings ||
+                                          !event_align_pairs || !n_event_align_pairs)))
+    return fail(ABEA_ERR_INVALID, "abea_align_db: null argument");
+  // flatten, as GPU/align_cuda.c:432-492 does (no NUL terminators needed)
+  std::vector<int64_t> seq_ptr(n_bam_rec), ev_ptr(n_bam_rec);
+  std::vector<int32_t> rl(read_len, read_len + n_bam_rec), ne(n_bam_rec);
+  int64_t sl = 0, se = 0;
+  for (int32_t i = 0; i < n_bam_rec; ++i) {
+    seq_ptr[i] = sl;
+    sl += std::max(read_len[i], 0);
+    ev_ptr[i] = se;
+    ne[i] = (int32_t)et[i].n;
+    se += (int64_t)et[i].n;
+  }
+  std::vector<char> seq((size_t)sl);
+  std::vector<abea_event_t> ev((size_t)se);
+  for (int32_t i = 0; i < n_bam_rec; ++i) {
+    if (read_len[i] > 0) std::memcpy(seq.data() + seq_ptr[i], read[i], (size_t)read_len[i]);
+    if (et[i].n) std::memcpy(ev.data() + ev_ptr[i], et[i].event, sizeof(abea_event_t) * et[i].n);
+  }
+  std::vector<abea_aligned_pair_t> pairs;
+  std::vector<int32_t> n_pairs;
+  std::vector<int64_t> off;
+  abea_status_t st = run_host_batch(ABEA_PLAN_BATCHED, n_bam_rec, seq, seq_ptr, rl, ev, ev_ptr, ne,
+                                    scalings, model, pairs, n_pairs, off);
+  if (st) return st;
+  // unflatten into callee-malloc'd per-read arrays, GPU/align_cuda.c:926-931
+  for (int32_t i = 0; i < n_bam_rec; ++i) {
+    const int32_t np = n_pairs[i];
+    n_event_align_pairs[i] = np;
+    event_align_pairs[i] =
+        (abea_aligned_pair_t *)malloc(sizeof(abea_aligned_pair_t) * (size_t)std::max(np, 1));
+    if (!event_align_pairs[i]) return fail(ABEA_ERR_NOMEM, "abea_align_db: malloc failed");
+    if (np) std::memcpy(event_align_pairs[i], pairs.data() + off[i], sizeof(abea_aligned_pair_t) * np);
+  }
+  return ABEA_OK;
+}
+
+}  // extern "C"

@@ -28,6 +28,8 @@
 // 355-359), argmax with the reference's tie order (GPU/align_cuda.cl:536-542).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "abea_internal.h"
 
 namespace abea {
@@ -260,7 +262,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, int r, int lane) 
     *row = (uint16_t)(f0 | (f1 << 8));
 
     // end-cell column (k = n_k - 1) for the post kernel's end search
-    const int os = (js - K) & 127;
+    const int os = (n_k - 1) - K;  // offset of the last k-mer in this band
     if ((uint32_t)(os - lo) < span) {
       const float v = (js & 1) ? M1 : M0;
       if (lane == (js >> 1)) endcol[E - os] = v;
@@ -431,8 +433,9 @@ __global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin
         pairs[i].ref_pos = kk;
         pairs[i].read_pos = ee;
         if (from_out) from_out[i] = (uint8_t)f;
-        const KmerParam p = kmp[kk];
-        em_buf[lane] = emission(evp[ee], p.mean, p.stdv, p.c);
+        // indices stay inside the read even if the path were malformed
+        const KmerParam p = kmp[min(max(kk, 0), n_k - 1)];
+        em_buf[lane] = emission(evp[min(max(ee, 0), n_e - 1)], p.mean, p.stdv, p.c);
       }
       if (t0 + 64 >= P) first_k = __shfl(kk, (P - 1 - t0) & 63);
       ck -= __shfl(pk, 63);
@@ -480,11 +483,13 @@ __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const f
     for (int i = lane; i < P; i += 64) {
       if (from[i] == FROM_L) continue;
       const abea_aligned_pair_t pr = pairs[i];
-      const int rank = kmp[pr.ref_pos].rank;
+      const int kk = min(max(pr.ref_pos, 0), m.n_k - 1);
+      const int ee = min(max(pr.read_pos, 0), m.n_e - 1);
+      const int rank = kmp[kk].rank & (ABEA_NUM_KMER - 1);
       const abea_model_t md = a.model[rank];
       const double mu = (double)md.level_mean, sg = (double)md.level_stdv;
       const double inv = 1.0 / sg;
-      const double z = ((double)evp[pr.read_pos] - (scale * mu + shift)) * inv;
+      const double z = ((double)evp[ee] - (scale * mu + shift)) * inv;
       gs += z * inv;
       gc += z * mu * inv;
       atomicAdd(&grad_model[2 * rank], w * (z * scale * inv));
@@ -512,7 +517,7 @@ __global__ void abea_zero_f64_kernel(double *p, int64_t n) {
 // ---------------------------------------------------------------------------
 hipError_t launch_pre(const KernelArgs &a, int q_begin, int q_count, hipStream_t s) {
   if (q_count <= 0) return hipSuccess;
-  const int grid = min(q_count, 4096);
+  const int grid = q_count < 4096 ? q_count : 4096;
   hipLaunchKernelGGL(abea_pre_kernel, dim3(grid), dim3(256), 0, s, a, q_begin, q_count);
   return hipGetLastError();
 }
@@ -543,7 +548,7 @@ int core_max_waves_per_block() { return 4; }
 
 hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int rows, hipStream_t s) {
   if (q_count <= 0) return hipSuccess;
-  const int grid = min(q_count, 65536);
+  const int grid = q_count < 65536 ? q_count : 65536;
   switch (rows) {
     case 32:
       hipLaunchKernelGGL(abea_post_kernel<32>, dim3(grid), dim3(64), 0, s, a, q_begin, q_count);
@@ -564,7 +569,7 @@ hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *g
                           double *grad_model, hipStream_t s) {
   const int n = a.n_queue;
   if (n <= 0) return hipSuccess;
-  const int grid = min((n + 3) / 4, 16384);
+  const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
   hipLaunchKernelGGL(abea_adjoint_kernel, dim3(grid), dim3(256), 0, s, a, grad_score,
                      grad_scalings, grad_model, n);
   return hipGetLastError();
@@ -572,7 +577,8 @@ hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *g
 
 hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int grid = (int)min<int64_t>((n + 255) / 256, 1024);
+  const int64_t blocks = (n + 255) / 256;
+  const int grid = (int)(blocks < 1024 ? blocks : 1024);
   hipLaunchKernelGGL(abea_zero_f64_kernel, dim3(grid), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }

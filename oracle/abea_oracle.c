@@ -153,6 +153,37 @@ void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n, c
   }
 }
 
+double orc_path_score(const orc_pair_t *path, const uint8_t *from, int32_t n, const char *seq,
+                      const float *ev_mean, int32_t n_events, const orc_model_t *models,
+                      float scale, float shift) {
+  /* FP64 score of a fixed path: boundary cell + transitions + emissions of
+   * D/U cells + trimming of the events after the end cell.  Equals the DP's
+   * best score when the path is the DP's own (GPU/align_cuda.cl:532-542,
+   * 722-746). */
+  if (n <= 0) return -INFINITY;
+  int32_t n_kmers = (int32_t)strlen(seq) - KMER + 1;
+  double lp[4];
+  orc_transition_constants(n_events, n_kmers, lp);
+  const double lp_skip = lp[0], lp_stay = lp[1], lp_step = lp[2], lp_trim = lp[3];
+  int32_t k0 = path[0].ref_pos, e0 = path[0].read_pos;
+  int32_t pk = from[0] == FROM_U ? k0 : k0 - 1;
+  int32_t pe = from[0] == FROM_L ? e0 : e0 - 1;
+  double s;
+  if (pk == -1 && pe == -1) s = 0.0;
+  else if (pk == -1) s = lp_trim * (pe + 1);
+  else return -INFINITY;
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t k = path[i].ref_pos, e = path[i].read_pos;
+    if (from[i] == FROM_L) {
+      s += lp_skip;
+    } else {
+      s += from[i] == FROM_D ? lp_step : lp_stay;
+      s += emission_f64(ev_mean[e], &models[orc_kmer_rank(seq + k)], scale, shift);
+    }
+  }
+  return s + (n_events - path[n - 1].read_pos) * lp_trim;
+}
+
 typedef struct { int64_t key; int32_t idx; } sort_item;
 
 static int cmp_desc(const void *a, const void *b) {

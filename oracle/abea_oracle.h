@@ -86,6 +86,11 @@ void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n,
                        double weight, double *g_read, double *g_mu,
                        double *g_sigma);
 
+/* FP64 score of a fixed path (from codes as returned by orc_align). */
+double orc_path_score(const orc_pair_t *path, const uint8_t *from, int32_t n, const char *seq,
+                      const float *ev_mean, int32_t n_events, const orc_model_t *models,
+                      float scale, float shift);
+
 /* Batched CPU baseline: OpenMP over reads, longest first.
  * seq_ptr / ev_ptr are CSR offsets (int64) into seqs / ev_means.
  * Writes n_pairs[i], scores[i]; pairs written at out + pair_ptr[i] when out

@@ -65,6 +65,9 @@ def lib():
         L.orc_path_gradient.restype = None
         L.orc_path_gradient.argtypes = [P, P, C.c_int32, P, P, P, C.c_float, C.c_float,
                                         C.c_double, P, P, P]
+        L.orc_path_score.restype = C.c_double
+        L.orc_path_score.argtypes = [P, P, C.c_int32, C.c_char_p, P, C.c_int32, P, C.c_float,
+                                     C.c_float]
         L.orc_align_batch.restype = C.c_int
         L.orc_align_batch.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int]
         _lib = L
@@ -157,6 +160,16 @@ def path_gradient(path, frm, seq: bytes, ev_mean, model, scale, shift, weight=1.
                             _p(ev), _p(model), float(scale), float(shift), float(weight),
                             _p(g_read), _p(g_mu), _p(g_sg))
     return g_read, g_mu, g_sg
+
+
+def path_score(path, frm, seq: bytes, ev_mean, model, scale, shift) -> float:
+    """FP64 score of a fixed path (boundary + transitions + emissions + end trim)."""
+    path = np.ascontiguousarray(path, np.int32)
+    frm = np.ascontiguousarray(frm, np.uint8)
+    ev = np.ascontiguousarray(ev_mean, np.float32)
+    model = np.ascontiguousarray(model, np.float32).reshape(4096, 3)
+    return float(lib().orc_path_score(_p(path), _p(frm), int(path.shape[0]), bytes(seq), _p(ev),
+                                      int(ev.shape[0]), _p(model), float(scale), float(shift)))
 
 
 def align_batch(batch, n_threads: int = 0, want_pairs: bool = False):

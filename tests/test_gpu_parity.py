@@ -1,0 +1,275 @@
+"""GPU parity: libabea's HIP path against the reference's golden f5c output and
+against the CPU oracle (same seeded inputs).  Bit-exact for pairs, from-codes,
+band moves and float scores; the QC statistic is a double sum in the
+reference's order and must match exactly too."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    _need_gpu()
+    from abea_amd import engine
+    c = engine.Context(0)
+    yield c
+    c.close()
+
+
+def run(ctx, batch, form, aos=False, tune=None):
+    from abea_amd import engine
+    db = engine.DeviceBatch.upload(batch, 0, aos_events=aos)
+    plan = engine.Plan(ctx, db, form)
+    if tune:
+        plan.set_tuning(**tune)
+    plan.forward()
+    res = plan.results()
+    res["moves"] = plan.moves()
+    plan.close()
+    return res
+
+
+def read_path(res, i):
+    off = int(res["pair_off"][i])
+    n = int(res["path_len"][i])
+    return res["pairs"][off:off + n], res["path_from"][off:off + n]
+
+
+def compare_to_oracle(batch, res, check_moves=True, idx=None):
+    import oracle
+    from abea_amd import engine
+    mv_off = engine.move_offsets(batch.read_len, batch.n_events)
+    bad = []
+    for i in (range(batch.n_reads) if idx is None else idx):
+        r = oracle.align(batch.read_seq(i), batch.read_events(i), batch.model,
+                         float(batch.scale[i]), float(batch.shift[i]))
+        pairs, frm = read_path(res, i)
+        ok = (int(res["path_len"][i]) == r["path_len"] and np.array_equal(pairs, r["path"])
+              and np.array_equal(frm, r["from"]) and int(res["n_pairs"][i]) == r["n_pairs"])
+        s_or = np.float32(r["score"])
+        ok = ok and (res["score"][i] == s_or or (np.isinf(s_or) and np.isinf(res["score"][i])))
+        if r["path_len"]:
+            ok = ok and res["avg"][i] == r["avg_log_emission"]
+        if check_moves and r["n_bands"]:
+            gm = engine.unpack_moves(res["moves"], int(mv_off[i]), r["n_bands"])
+            ok = ok and np.array_equal(gm[2:], r["moves"][2:])
+        if not ok:
+            bad.append((i, int(res["path_len"][i]), r["path_len"], float(res["score"][i]),
+                        r["score"]))
+    return bad
+
+
+def golden_batch():
+    from abea_amd import load_golden_subset
+    return load_golden_subset(GOLDEN)
+
+
+@pytest.mark.parametrize("form", [1, 0], ids=["batched", "plain"])
+def test_golden_pairs_identical(ctx, form):
+    """AlignedPairs equal f5c's own output (FPGA/code/bins/dump_small)."""
+    b, golden, _ = golden_batch()
+    res = run(ctx, b, form)
+    for i in range(b.n_reads):
+        n = int(res["n_pairs"][i])
+        off = int(res["pair_off"][i])
+        assert n == golden[i].shape[0], (i, n, golden[i].shape[0])
+        assert np.array_equal(res["pairs"][off:off + n], golden[i]), i
+
+
+def test_golden_vs_oracle_bitwise(ctx):
+    b, _, _ = golden_batch()
+    res = run(ctx, b, 1)
+    assert compare_to_oracle(b, res) == []
+
+
+def test_aos_events_equal_soa(ctx):
+    b, golden, _ = golden_batch()
+    r1 = run(ctx, b, 1, aos=False)
+    r2 = run(ctx, b, 1, aos=True)
+    for k in ("n_pairs", "path_len", "score", "avg"):
+        assert np.array_equal(r1[k], r2[k]), k
+    assert np.array_equal(r1["pairs"], r2["pairs"])
+
+
+def edge_batch():
+    """Reads the reference treats specially: < 6 bases, no events, more k-mers
+    than events, very few events, non-ACGT bases, a single k-mer."""
+    from abea_amd import synth
+    base = synth.workload("cpu64x2kb", seed=11, n=8)
+    rng = np.random.default_rng(12)
+    seqs, evs = [], []
+    for i in range(8):
+        seqs.append(base.read_seq(i))
+        evs.append(base.read_events(i))
+    specs = [(5, 10), (6, 3), (7, 1), (12, 40), (60, 10), (300, 120), (400, 30), (1000, 60),
+             (50, 0), (2000, 5000)]
+    for L, ne in specs:
+        s = bytearray(rng.choice(np.frombuffer(b"ACGTN", np.uint8), L).tobytes())
+        seqs.append(bytes(s))
+        evs.append(rng.uniform(55, 135, ne).astype(np.float32))
+    rl = np.array([len(s) for s in seqs], np.int32)
+    ne = np.array([len(e) for e in evs], np.int32)
+    sp = np.concatenate([[0], np.cumsum(rl.astype(np.int64))[:-1]])
+    ep = np.concatenate([[0], np.cumsum(ne.astype(np.int64))[:-1]])
+    n = len(seqs)
+    sc = np.concatenate([base.scale, np.ones(n - 8, np.float32)])
+    sh = np.concatenate([base.shift, np.zeros(n - 8, np.float32)])
+    return synth.Batch(base.model, np.frombuffer(b"".join(seqs), np.uint8).copy(), sp, rl,
+                       np.concatenate(evs).astype(np.float32), ep, ne, sc, sh,
+                       np.ones(n, np.float32), np.zeros(n, np.float32))
+
+
+@pytest.mark.parametrize("form", [1, 0], ids=["batched", "plain"])
+def test_synthetic_and_edges_vs_oracle(ctx, form):
+    b = edge_batch()
+    res = run(ctx, b, form)
+    assert compare_to_oracle(b, res) == []
+
+
+def test_tunings_agree(ctx):
+    """Every launch shape the autotuner may pick gives identical results."""
+    b = edge_batch()
+    ref = run(ctx, b, 1)
+    for wpb, wpc, rows in [(1, 8, 32), (2, 16, 64), (4, 32, 128), (4, 16, 32)]:
+        r = run(ctx, b, 1, tune=dict(core_waves_per_block=wpb, core_waves_per_cu=wpc,
+                                     post_rows=rows, use_graph=0))
+        for k in ("n_pairs", "path_len", "score", "avg", "pairs", "path_from", "moves"):
+            assert np.array_equal(ref[k], r[k]), (k, wpb, wpc, rows)
+
+
+def test_graph_replays_match_eager(ctx):
+    """Replay the captured pre->core->post graph several times with eager work
+    and synchronisation in between; every replay equals the eager run."""
+    from abea_amd import engine, synth
+    b = synth.workload("cpu64x2kb", seed=21, n=48)
+    db = engine.DeviceBatch.upload(b, 0)
+    plan = engine.Plan(ctx, db, 1)
+    plan.set_tuning(use_graph=0)
+    plan.forward()
+    ref = plan.results()
+    plan.set_tuning(use_graph=1)
+    for rep in range(4):
+        plan.pairs.zero_()
+        plan.n_pairs.fill_(-1)
+        plan.score.zero_()
+        torch.cuda.synchronize()
+        plan.forward()
+        got = plan.results()
+        for k in ("n_pairs", "path_len", "score", "avg", "pairs", "path_from"):
+            assert np.array_equal(ref[k], got[k]), (rep, k)
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+    plan.close()
+
+
+def test_replay_moves_reproduces(ctx):
+    from abea_amd import engine, synth
+    b = synth.workload("cpu64x2kb", seed=31, n=16)
+    db = engine.DeviceBatch.upload(b, 0)
+    plan = engine.Plan(ctx, db, 1)
+    plan.forward()
+    r1 = plan.results()
+    mv = plan.moves()
+    plan.set_replay(mv)
+    plan.forward()
+    r2 = plan.results()
+    assert np.array_equal(mv, plan.moves())
+    for k in ("n_pairs", "path_len", "score", "pairs"):
+        assert np.array_equal(r1[k], r2[k]), k
+    plan.close()
+
+
+def test_medium_reads_vs_oracle(ctx):
+    """256 reads of the log-normal (median 5 kb) workload, checked read by read."""
+    from abea_amd import synth
+    L = synth.workload_lengths("plain4096_ln5kb", seed=0)[:256]
+    b = synth.generate(L, seed=5)
+    res = run(ctx, b, 1)
+    assert compare_to_oracle(b, res, idx=range(0, 256, 4)) == []
+    # every read: path is a monotone lattice walk ending on the last k-mer
+    import oracle
+    n_pairs, scores, _, _ = oracle.align_batch(b)
+    assert np.array_equal(n_pairs, res["n_pairs"])
+    assert np.array_equal(scores, res["score"])
+
+
+def test_drop_in_align_and_align_db(ctx):
+    """f5c-shaped host entry points (align(), align_cuda(core, db))."""
+    from abea_amd import _lib
+    b, golden, _ = golden_batch()
+    L = _lib.lib()
+    model = np.ascontiguousarray(b.model, np.float32)
+    aos = b.events_aos()
+    n = 6
+    keep = []
+    reads, ets = (C.c_char_p * n)(), (_lib.EventTable * n)()
+    for i in range(n):
+        seq = b.read_seq(i)
+        ev = np.ascontiguousarray(aos[b.ev_ptr[i]:b.ev_ptr[i] + b.n_events[i]])
+        keep += [seq, ev]
+        et = _lib.EventTable(ev.shape[0], 0, ev.shape[0], ev.ctypes.data_as(C.POINTER(_lib.EventT)))
+        sc = _lib.ScalingsT(float(b.scale[i]), float(b.shift[i]), 0.0, 0.0)
+        out = np.zeros((2 * ev.shape[0], 2), np.int32)
+        npairs = L.abea_align(out.ctypes.data_as(C.c_void_p), seq, len(seq), et,
+                              model.ctypes.data_as(C.c_void_p), sc, 4000.0)
+        assert npairs == golden[i].shape[0], (i, npairs, L.abea_last_error())
+        assert np.array_equal(out[:npairs], golden[i])
+        reads[i] = seq
+        ets[i] = et
+    rl = np.ascontiguousarray(b.read_len[:n], np.int32)
+    scal = np.ascontiguousarray(b.scalings()[:n], np.float32)
+    outp = (C.POINTER(_lib.AlignedPair) * n)()
+    nout = np.zeros(n, np.int32)
+    st = L.abea_align_db(n, C.cast(reads, C.c_void_p), rl.ctypes.data_as(C.c_void_p),
+                         C.cast(ets, C.c_void_p), model.ctypes.data_as(C.c_void_p),
+                         scal.ctypes.data_as(C.c_void_p), C.cast(outp, C.c_void_p),
+                         nout.ctypes.data_as(C.c_void_p))
+    assert st == 0, L.abea_last_error()
+    libc = C.CDLL(None)
+    for i in range(n):
+        assert nout[i] == golden[i].shape[0]
+        got = np.ctypeslib.as_array(C.cast(outp[i], C.POINTER(C.c_int32)), (int(nout[i]) * 2,))
+        assert np.array_equal(got.reshape(-1, 2), golden[i])
+        libc.free(C.cast(outp[i], C.c_void_p))
+
+
+def test_adjoint_matches_fp64_path_gradient(ctx):
+    """GPU adjoint vs the oracle's FP64 gradient along the same paths."""
+    import oracle
+    from abea_amd import engine, synth
+    b = synth.workload("cpu64x2kb", seed=41, n=12)
+    db = engine.DeviceBatch.upload(b, 0)
+    plan = engine.Plan(ctx, db, 1)
+    plan.forward()
+    res = plan.results()
+    w = torch.linspace(0.5, 1.5, b.n_reads, device="cuda", dtype=torch.float32)
+    g_scal, g_model = plan.adjoint(w)
+    g_scal = g_scal.cpu().numpy()
+    g_model = g_model.cpu().numpy()
+    ref_read = np.zeros((b.n_reads, 3))
+    ref_mu = np.zeros(4096)
+    ref_sg = np.zeros(4096)
+    wn = w.cpu().numpy().astype(np.float64)
+    for i in range(b.n_reads):
+        pairs, frm = read_path(res, i)
+        gr, gm, gsg = oracle.path_gradient(pairs, frm, b.read_seq(i), b.read_events(i), b.model,
+                                           b.scale[i], b.shift[i], weight=wn[i])
+        ref_read[i] = gr
+        ref_mu += gm
+        ref_sg += gsg
+    np.testing.assert_allclose(g_scal[:, :2], ref_read[:, :2], rtol=1e-5, atol=1e-3)
+    assert np.all(g_scal[:, 2] == 0)
+    np.testing.assert_allclose(g_model[:, 0], ref_mu, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(g_model[:, 1], ref_sg, rtol=1e-9, atol=1e-9)
+    plan.close()
