@@ -57,7 +57,8 @@ class OutputT(C.Structure):
 
 class TuningT(C.Structure):
     _fields_ = [("core_waves_per_block", C.c_int32), ("core_waves_per_cu", C.c_int32),
-                ("post_rows", C.c_int32), ("use_graph", C.c_int32)]
+                ("traceback_overlap", C.c_int32), ("use_graph", C.c_int32),
+                ("fuse_traceback", C.c_int32)]
 
 
 # every symbol include/abea.h declares

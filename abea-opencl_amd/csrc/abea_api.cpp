@@ -101,12 +101,12 @@ struct abea_plan {
   abea_batch_t batch{};
   std::vector<ReadMeta> meta_h;
   std::vector<int32_t> order_h;
-  int64_t total_pairs = 0, trace_bytes = 0, move_words = 0, km_count = 0, ev_count = 0;
-  int64_t n_bands_total = 0;
-  DevBuf meta, queue, counter, km, ev, endcol, trace, moves, replay, rev;
+  int64_t total_pairs = 0, trace_words = 0, move_words = 0, km_count = 0, ev_count = 0;
+  int64_t n_bands_total = 0, seg_bytes = 0, tail_pad = 0;
+  DevBuf meta, queue, counter, km, km_rank, ev, endcol, trace, moves, kck, replay, seg;
   bool replay_on = false;
   double lp_skip = 0, lp_trim = 0;
-  abea_tuning_t tune{1, 24, 64, 1};
+  abea_tuning_t tune{1, 24, 128, 1, 0};
   // graph cache
   hipGraphExec_t gexec = nullptr;
   abea_output_t gout{};
@@ -200,7 +200,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   p->n = n;
   p->form = form;
   p->meta_h.assign(n, ReadMeta{});
-  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0;
+  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0, sg = 0, maxlen = 0;
   for (int32_t i = 0; i < n; ++i) {
     ReadMeta &m = p->meta_h[i];
     const int32_t n_e = b->n_events_host[i];
@@ -215,21 +215,30 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     m.km_off = km;
     m.trace_off = tr;
     m.mv_off = mv;
+    m.seg_off = sg;
     if (n_k > 0 && n_e > 0) {
       const int64_t bands = (int64_t)n_e + n_k + 2;
       km += n_k;
-      tr += bands * abea::kRowBytes;
+      tr += (bands + abea::kTileBands - 1) / abea::kTileBands * abea::kTileWords;
       mv += (bands + 31) / 32;
+      // 64 traceback segments of S = round4(ceil(rows / 64)) rows each
+      sg += (bands + 256 + 3) & ~(int64_t)3;
       nb += bands;
+      maxlen = std::max<int64_t>(maxlen, (int64_t)n_e + n_k);
       transition_constants(n_e, n_k, &m.lp_stay, &m.lp_step);
     }
   }
   p->total_pairs = pr;
-  p->trace_bytes = tr;
+  p->trace_words = tr;
+  p->seg_bytes = sg;
   p->move_words = mv;
   p->km_count = km;
   p->ev_count = evmax;
   p->n_bands_total = nb;
+  // The core prefetches event E+1 and k-mer K+128 of the band walk without
+  // clamping: E <= n_e + n_k + 50 and K + 128 <= n_e + n_k + 77 for any read,
+  // so staged arrays carry this much slack past the last read.
+  p->tail_pad = maxlen + abea::kPad;
   p->lp_skip = log(1e-10);
   p->lp_trim = log(0.01);
 
@@ -247,10 +256,11 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
 
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
-  const size_t need = (size_t)tr + (size_t)km * sizeof(KmerParam) + (size_t)evmax * 8 +
-                      (size_t)mv * 8 + (size_t)pr + (size_t)n * (sizeof(ReadMeta) + 4);
-  if (free_b && need > free_b + p->trace.n + p->km.n + p->ev.n + p->endcol.n + p->moves.n +
-                           p->replay.n + p->rev.n + p->meta.n)
+  const int64_t kmc = km + p->tail_pad, evc = evmax + p->tail_pad;
+  const size_t need = (size_t)tr * 4 + (size_t)kmc * (sizeof(KmerParam) + 2) + (size_t)evc * 8 +
+                      (size_t)mv * 12 + (size_t)sg + (size_t)n * (sizeof(ReadMeta) + 4);
+  if (free_b && need > free_b + p->trace.n + p->km.n + p->km_rank.n + p->ev.n + p->endcol.n +
+                           p->moves.n + p->kck.n + p->replay.n + p->seg.n + p->meta.n)
     return fail(ABEA_ERR_NOMEM, "plan: workspace of %.2f GB exceeds free device memory (%.2f GB)",
                 need / 1e9, free_b / 1e9);
 
@@ -258,14 +268,14 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   if ((st = p->meta.ensure(sizeof(ReadMeta) * (size_t)std::max(n, 1)))) return st;
   if ((st = p->queue.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
   if ((st = p->counter.ensure(256))) return st;
-  if ((st = p->km.ensure(sizeof(KmerParam) * (size_t)std::max<int64_t>(km, 1)))) return st;
-  if (b->events) {
-    if ((st = p->ev.ensure(sizeof(float) * (size_t)std::max<int64_t>(evmax, 1)))) return st;
-  }
+  if ((st = p->km.ensure(sizeof(KmerParam) * (size_t)kmc))) return st;
+  if ((st = p->km_rank.ensure(sizeof(uint16_t) * (size_t)kmc))) return st;
+  if ((st = p->ev.ensure(sizeof(float) * (size_t)evc))) return st;
   if ((st = p->endcol.ensure(sizeof(float) * (size_t)std::max<int64_t>(evmax, 1)))) return st;
-  if ((st = p->trace.ensure((size_t)std::max<int64_t>(tr, 1)))) return st;
+  if ((st = p->trace.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(tr, 1)))) return st;
   if ((st = p->moves.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mv, 1)))) return st;
-  if ((st = p->rev.ensure((size_t)std::max<int64_t>(pr, 1)))) return st;
+  if ((st = p->kck.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(mv, 1)))) return st;
+  if ((st = p->seg.ensure((size_t)std::max<int64_t>(sg, 4)))) return st;
   if (n > 0) {
     HIP_TRY(hipMemcpyAsync(p->meta.p, p->meta_h.data(), sizeof(ReadMeta) * n,
                            hipMemcpyHostToDevice, p->ctx->stream));
@@ -295,12 +305,14 @@ KernelArgs make_args(const abea_plan *p, const abea_output_t *out) {
   a.scalings = p->batch.scalings;
   a.model = p->batch.model;
   a.km = p->km.as<KmerParam>();
-  a.ev = p->batch.events ? p->ev.as<float>() : const_cast<float *>(p->batch.ev_mean);
+  a.km_rank = p->km_rank.as<uint16_t>();
+  a.ev = p->ev.as<float>();
   a.endcol = p->endcol.as<float>();
-  a.trace = p->trace.as<uint8_t>();
+  a.trace = p->trace.as<uint32_t>();
   a.moves = p->moves.as<uint32_t>();
+  a.kck = p->kck.as<int32_t>();
   a.replay = p->replay_on ? p->replay.as<uint32_t>() : nullptr;
-  a.rev_from = p->rev.as<uint8_t>();
+  a.seg = p->seg.as<uint8_t>();
   a.lp_skip = p->lp_skip;
   a.lp_trim = p->lp_trim;
   if (out) a.out = *out;
@@ -323,8 +335,8 @@ abea_status_t enqueue_forward(abea_plan *p, const KernelArgs &a, bool with_event
     // reference-shaped: one launch set per read, reads in input order
     for (int32_t i = 0; i < p->n; ++i) {
       HIP_TRY(abea::launch_pre(a, i, 1, s));
-      HIP_TRY(abea::launch_core(a, i, 1, false, 1, 1, s));
-      HIP_TRY(abea::launch_post(a, i, 1, t.post_rows, s));
+      HIP_TRY(abea::launch_core(a, i, 1, false, 1, 1, 0, t.traceback_overlap, s));
+      HIP_TRY(abea::launch_post(a, i, 1, t.traceback_overlap, s));
     }
     if (with_events) {
       HIP_TRY(hipEventRecord(p->evt[1], s));
@@ -338,9 +350,9 @@ abea_status_t enqueue_forward(abea_plan *p, const KernelArgs &a, bool with_event
   const int wpb = t.core_waves_per_block;
   int grid = std::max(1, (p->ctx->n_cu * t.core_waves_per_cu) / wpb);
   grid = std::min(grid, std::max(1, (p->n + wpb - 1) / wpb));
-  HIP_TRY(abea::launch_core(a, 0, p->n, true, wpb, grid, s));
+  HIP_TRY(abea::launch_core(a, 0, p->n, true, wpb, grid, t.fuse_traceback, t.traceback_overlap, s));
   if (with_events) HIP_TRY(hipEventRecord(p->evt[2], s));
-  HIP_TRY(abea::launch_post(a, 0, p->n, t.post_rows, s));
+  if (!t.fuse_traceback) HIP_TRY(abea::launch_post(a, 0, p->n, t.traceback_overlap, s));
   if (with_events) HIP_TRY(hipEventRecord(p->evt[3], s));
   return ABEA_OK;
 }
@@ -461,8 +473,8 @@ abea_status_t abea_plan_last_times(const abea_plan_t *pc, double times_ms[4]) {
 
 int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
   if (!p) return 0;
-  return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->ev.n + p->endcol.n +
-                   p->trace.n + p->moves.n + p->replay.n + p->rev.n);
+  return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->km_rank.n + p->ev.n +
+                   p->endcol.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n);
 }
 
 int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }
@@ -525,70 +537,68 @@ abea_status_t abea_plan_set_tuning(abea_plan_t *p, const abea_tuning_t *t) {
   if (!p || !t) return fail(ABEA_ERR_INVALID, "set_tuning: null argument");
   const int w = t->core_waves_per_block;
   if ((w != 1 && w != 2 && w != 4) || t->core_waves_per_cu < 1 || t->core_waves_per_cu > 64 ||
-      (t->post_rows != 32 && t->post_rows != 64 && t->post_rows != 128))
-    return fail(ABEA_ERR_INVALID, "set_tuning: unsupported values (%d, %d, %d)", w,
-                t->core_waves_per_cu, t->post_rows);
+      t->traceback_overlap < 32 || t->traceback_overlap > 1024 ||
+      (t->fuse_traceback != 0 && t->fuse_traceback != 1))
+    return fail(ABEA_ERR_INVALID, "set_tuning: unsupported values (%d, %d, %d, %d)", w,
+                t->core_waves_per_cu, t->traceback_overlap, t->fuse_traceback);
   p->tune = *t;
   return ABEA_OK;
 }
 
 abea_status_t abea_plan_autotune(abea_plan_t *p, const abea_output_t *out, int reps,
                                  double *best_ms) {
-  // The project autotuner: times candidate launch shapes of the core
-  // (waves per workgroup x resident waves per CU of the persistent grid) and
-  // the traceback's LDS depth (trace rows per chunk) on this plan's own
-  // batch, each with event timing of that kernel alone, and keeps the best.
+  // The project autotuner: times candidate launch shapes on this plan's own
+  // batch and keeps the fastest.  Axes: waves per workgroup and resident
+  // waves per CU of the persistent core; whether each core wave traces its
+  // own read back or a separate post kernel does; how far above its segment
+  // each traceback lane starts.  Every candidate computes the same result;
+  // only time differs.
   if (!p) return fail(ABEA_ERR_INVALID, "autotune: null plan");
   if (p->form != ABEA_PLAN_BATCHED) return fail(ABEA_ERR_INVALID, "autotune: batched plans only");
   abea_status_t st = check_out(p, out);
   if (st) return st;
   if (reps < 1) reps = 1;
   const abea_tuning_t saved = p->tune;
-  abea_tuning_t t = saved;
-  t.use_graph = 0;
-  struct Cand { int wpb, wpc; };
-  const Cand cands[] = {{1, 8}, {1, 16}, {1, 24}, {1, 32}, {2, 16}, {2, 32}, {4, 16}, {4, 32}};
-  double best_core = 1e30;
-  Cand best_c = {saved.core_waves_per_block, saved.core_waves_per_cu};
+  auto time_forward = [&](const abea_tuning_t &t, double *ms) -> abea_status_t {
+    p->tune = t;
+    double acc = 0;
+    for (int r = 0; r < reps; ++r) {
+      abea_status_t s2 = abea_plan_forward(p, out);
+      if (s2) return s2;
+      double tm[4];
+      if ((s2 = abea_plan_last_times(p, tm))) return s2;
+      acc += tm[0] + tm[1] + tm[2];
+    }
+    *ms = acc / reps;
+    return ABEA_OK;
+  };
+  struct Cand { int wpb, wpc, fuse, overlap; };
+  const Cand cands[] = {{1, 16, 0, 128}, {1, 24, 0, 128}, {1, 32, 0, 128}, {2, 32, 0, 128},
+                        {4, 32, 0, 128}, {1, 16, 1, 128}, {1, 24, 1, 128}, {1, 32, 1, 128},
+                        {2, 32, 1, 128}, {4, 32, 1, 128}, {1, 24, 0, 64},  {1, 24, 0, 256},
+                        {1, 24, 1, 64},  {1, 24, 1, 256}};
+  double best = 1e30;
+  abea_tuning_t best_t = saved;
   for (const Cand &c : cands) {
+    abea_tuning_t t = saved;
+    t.use_graph = 0;
     t.core_waves_per_block = c.wpb;
     t.core_waves_per_cu = c.wpc;
-    p->tune = t;
-    double acc = 0;
-    for (int r = 0; r < reps; ++r) {
-      if ((st = abea_plan_forward(p, out))) { p->tune = saved; return st; }
-      double tm[4];
-      if ((st = abea_plan_last_times(p, tm))) { p->tune = saved; return st; }
-      acc += tm[1];
+    t.fuse_traceback = c.fuse;
+    t.traceback_overlap = c.overlap;
+    double ms = 0;
+    if ((st = time_forward(t, &ms))) {
+      p->tune = saved;
+      return st;
     }
-    if (acc / reps < best_core) {
-      best_core = acc / reps;
-      best_c = c;
-    }
-  }
-  t.core_waves_per_block = best_c.wpb;
-  t.core_waves_per_cu = best_c.wpc;
-  double best_post = 1e30;
-  int best_rows = saved.post_rows;
-  for (int rows : {32, 64, 128}) {
-    t.post_rows = rows;
-    p->tune = t;
-    double acc = 0;
-    for (int r = 0; r < reps; ++r) {
-      if ((st = abea_plan_forward(p, out))) { p->tune = saved; return st; }
-      double tm[4];
-      if ((st = abea_plan_last_times(p, tm))) { p->tune = saved; return st; }
-      acc += tm[2];
-    }
-    if (acc / reps < best_post) {
-      best_post = acc / reps;
-      best_rows = rows;
+    if (ms < best) {
+      best = ms;
+      best_t = t;
     }
   }
-  t.post_rows = best_rows;
-  t.use_graph = saved.use_graph;
-  p->tune = t;
-  if (best_ms) *best_ms = best_core + best_post;
+  best_t.use_graph = saved.use_graph;
+  p->tune = best_t;
+  if (best_ms) *best_ms = best;
   return ABEA_OK;
 }
 

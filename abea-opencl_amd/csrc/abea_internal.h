@@ -9,28 +9,32 @@ namespace abea {
 
 constexpr int kW = ABEA_BANDWIDTH;  // band width, GPU/f5c.h:30
 constexpr int kSlots = 128;          // k-indexed circular band slots per wave (2 per lane)
-constexpr int kRowBytes = 128;       // trace bytes per band row (one byte per slot)
+constexpr int kTileBands = 8;        // bands per trace tile
+constexpr int kTileWords = 64;       // one 32-bit word per lane per tile: 8 bands x 2 slots x 2 bits
+constexpr int kPad = 256;            // elements of slack after staged arrays (prefetch reads past a read)
 
-// Per-read metadata, one 64-byte record per read (read-order index).
+// Per-read metadata, one 80-byte record per read (read-order index).
 struct alignas(16) ReadMeta {
-  int64_t trace_off;  // byte offset of the read's trace rows (n_bands * 128 B)
+  int64_t trace_off;  // word offset of the read's 2-bit trace tiles (ceil(n_bands/8) * 64 words)
   int64_t km_off;     // offset (in k-mer parameter records) of the read's k-mers
   int64_t ev_off;     // offset of the read's events (= batch ev_ptr)
   int64_t pair_off;   // offset of the read's pair slot
-  int64_t mv_off;     // offset (uint32 words) of the read's band-move bits
+  int64_t mv_off;     // offset (uint32 words) of the read's band-move bits and K checkpoints
+  int64_t seg_off;    // 4-byte aligned byte offset of the read's traceback segment scratch
   int32_t n_e;        // events
   int32_t n_k;        // k-mers = read_len - 5 (<= 0 for reads that are skipped)
   double lp_stay;     // per-read transition log-probabilities (host glibc log/exp)
   double lp_step;
+  int64_t reserved;
 };
-static_assert(sizeof(ReadMeta) == 64, "ReadMeta must stay 64 bytes");
+static_assert(sizeof(ReadMeta) == 80, "ReadMeta must stay 80 bytes");
 
 // Staged per-k-mer emission parameters written by the pre kernel.
 struct alignas(16) KmerParam {
   float mean;   // scale * level_mean + shift (f32 multiply then add)
   float stdv;   // level_stdv
+  float rstdv;  // 1 / level_stdv, correctly rounded
   float c;      // -0.918938f - level_log_stdv
-  int32_t rank; // 6-mer rank (for the model-gradient scatter)
 };
 
 struct KernelArgs {
@@ -45,12 +49,14 @@ struct KernelArgs {
   const abea_scalings_t *scalings;
   const abea_model_t *model;
   KmerParam *km;             // staged k-mer parameters
+  uint16_t *km_rank;         // 6-mer rank per staged k-mer (model-gradient scatter)
   float *ev;                 // staged event means (SoA)
   float *endcol;             // band cell (e, n_k - 1) per event
-  uint8_t *trace;            // uint8 trace rows
+  uint32_t *trace;           // 2-bit from-codes in 8-band tiles
   uint32_t *moves;           // 1 bit per band (1 = right)
+  int32_t *kck;              // K after every 32nd band (band 32w+31), per move word
   const uint32_t *replay;    // band moves to replay, or null
-  uint8_t *rev_from;         // traceback from-codes in traceback order (scratch)
+  uint8_t *seg;              // traceback segment scratch (from-codes, one byte each)
   double lp_skip;
   double lp_trim;
   // outputs
@@ -59,9 +65,11 @@ struct KernelArgs {
 
 // Kernel launchers (abea_kernels.hip).
 hipError_t launch_pre(const KernelArgs &a, int q_begin, int q_count, hipStream_t s);
+// fuse != 0: the core wave also does its read's traceback; otherwise the
+// post kernel does.  overlap: rows each traceback segment starts above itself.
 hipError_t launch_core(const KernelArgs &a, int q_begin, int q_count, bool persistent,
-                       int waves_per_block, int grid_blocks, hipStream_t s);
-hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int rows, hipStream_t s);
+                       int waves_per_block, int grid_blocks, int fuse, int overlap, hipStream_t s);
+hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overlap, hipStream_t s);
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
                           double *grad_model, hipStream_t s);
 hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t s);

@@ -2,9 +2,10 @@
 //
 //   pre      stage per-k-mer emission parameters and SoA event means
 //            (replaces align_kernel_pre_2d, GPU/align_cuda.cl:162-276)
-//   core     adaptive band walk, one wavefront per read
+//   core     adaptive band walk, one wavefront per read; optionally followed
+//            by that read's traceback in the same wave
 //            (replaces align_kernel_core_2d_shm, GPU/align_cuda.cl:297-609)
-//   post     best end cell, traceback over the uint8 trace, QC
+//   post     best end cell, traceback over the trace, QC
 //            (replaces align_kernel_post, GPU/align_cuda.cl:618-899)
 //   adjoint  gradient of the best-path score along the traced path (new)
 //
@@ -18,14 +19,20 @@
 //   diag (e-1, k-1) = band b-2, slot - 1 = the "left" vector of band b-1
 // so one DPP rotate per band supplies them, each lane's k-mer parameters stay
 // put (a right move refills one slot), and the event under every slot shifts
-// by one slot per band (a down move feeds one new event in).  Slots outside
-// the band window hold -inf, so out-of-band neighbours read -inf exactly as
-// the reference's is_offset_valid() tests do.
+// by one slot per band.  Slots outside the band window hold -inf, so
+// out-of-band neighbours read -inf exactly as the reference's
+// is_offset_valid() tests do.
 //
-// Numerics follow f5c's CPU align(): float cells, emission in float with an
-// IEEE divide and no contraction (built with -ffp-contract=off), transition
-// sums in double (the reference's constants are double, GPU/align_cuda.cl:
-// 355-359), argmax with the reference's tie order (GPU/align_cuda.cl:536-542).
+// Trace layout: 2-bit from-codes (0 = D, 1 = U, 2 = L) in tiles of 8 bands;
+// tile t holds one 32-bit word per lane, band 8t+i in bits [4i, 4i+4) as
+// (code of slot 2l) | (code of slot 2l+1) << 2.  Cell (b, slot j) is bits
+// 4(b&7) + 2(j&1) of word 64(b>>3) + (j>>1).
+//
+// Numerics follow f5c's CPU align(): float cells, emission in float with
+// correctly rounded operations and no contraction (built with
+// -ffp-contract=off), transition sums in double (the reference's constants are
+// double, GPU/align_cuda.cl:355-359), argmax with the reference's tie order
+// (GPU/align_cuda.cl:536-542).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,13 +48,14 @@ typedef __attribute__((address_space(4))) const int32_t c_i32;
 typedef __attribute__((address_space(4))) const uint32_t c_u32;
 
 constexpr uint32_t FROM_D = 0, FROM_U = 1, FROM_L = 2;  // GPU/align_cuda.cl:130-132
+constexpr int kHalfW = ABEA_BANDWIDTH / 2;
 
 // Uniform (scalar-cache) load of one read's metadata record.
 __device__ __forceinline__ ReadMeta load_meta(const ReadMeta *meta, int r) {
   const c_i32 *p = (const c_i32 *)(meta + r);
-  int32_t w[16];
+  int32_t w[sizeof(ReadMeta) / 4];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) w[i] = p[i];
+  for (int i = 0; i < (int)(sizeof(ReadMeta) / 4); ++i) w[i] = p[i];
   ReadMeta m;
   __builtin_memcpy(&m, w, sizeof(m));
   return m;
@@ -55,17 +63,50 @@ __device__ __forceinline__ ReadMeta load_meta(const ReadMeta *meta, int r) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Ordering of this wave's own LDS and global accesses across its lanes (all
+// lanes of a wave belong to one workgroup and share the CU's caches).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // lane l receives lane (l - 1) mod 64: DPP wave_ror:1.
 __device__ __forceinline__ float rotate_up(float v) {
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, true));
 }
 
-// Value of slot j (0..127) of a band held as (slot 2l, slot 2l+1) per lane.
-__device__ __forceinline__ float slot_value(float s0, float s1, int j) {
-  const int a = __builtin_amdgcn_readlane(__builtin_bit_cast(int, s0), j >> 1);
-  const int b = __builtin_amdgcn_readlane(__builtin_bit_cast(int, s1), j >> 1);
-  return __builtin_bit_cast(float, (j & 1) ? b : a);
+__device__ __forceinline__ uint32_t lane_bits(float v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane);
+}
+
+// Signed-integer order key of a float's bits: for non-NaN values without -0,
+// key(x) < key(y) exactly when x < y.  Band cells are never NaN or -0 (they
+// are -inf or rounded sums), so the band-move comparison runs on the scalar
+// unit.
+__device__ __forceinline__ int32_t order_key(uint32_t u) {
+  const int32_t i = (int32_t)u;
+  return i ^ ((i >> 31) & 0x7fffffff);
+}
+
+constexpr uint32_t kNegInfBits = 0xff800000u;
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int s) {
+  return s ? ((x << s) | (x >> (64 - s))) : x;
+}
+
+// Lanes l (0..63) whose slot 2l+p lies in the circular slot range
+// [first, first+span) (mod 128); span <= 100.
+__device__ __forceinline__ uint64_t slot_lanes(int first, int span, int p) {
+  if (span <= 0) return 0;
+  const int t = (first - p) & 127;
+  const int l0 = (t + 1) >> 1;
+  const int cnt = ((t + span + 1) >> 1) - l0;
+  return rotl64((1ull << cnt) - 1, l0 & 63);  // cnt <= 50
+}
+
+__device__ __forceinline__ bool lane_in(uint64_t mask) {
+  return __builtin_amdgcn_inverse_ballot_w64(mask);
 }
 
 __device__ __forceinline__ uint32_t base_code(char c) {
@@ -73,14 +114,25 @@ __device__ __forceinline__ uint32_t base_code(char c) {
   return c == 'C' ? 1u : c == 'G' ? 2u : c == 'T' ? 3u : 0u;
 }
 
-// f5c emission, GPU/align_cuda.cl:511-528: a = (x - m) / s; em = c + (-0.5a)a
-// with c = -0.918938f - level_log_stdv precomputed by pre.
-__device__ __forceinline__ float emission(float x, float m, float s, float c) {
-  const float a = (x - m) / s;
+// f5c emission, GPU/align_cuda.cl:511-528: a = (x - m) / s;
+// em = (-0.918938f - log_s) + (-0.5a)a, with c = -0.918938f - log_s staged by
+// pre.  The quotient is the correctly rounded (x - m) / s, obtained from the
+// correctly rounded reciprocal rs = 1/s by one fused residual correction
+// (Markstein): q0 = n*rs, r = fma(-q0, s, n) (exact), a = fma(r, rs, q0).
+// This equals IEEE division for the finite, non-underflowing quotients that
+// occur here and costs 4 operations instead of the ~10 of a divide.
+__device__ __forceinline__ float emission(float x, float m, float s, float rs, float c) {
+  const float n = x - m;
+  const float q0 = n * rs;
+  const float r = __builtin_fmaf(-q0, s, n);
+  const float a = __builtin_fmaf(r, rs, q0);
   return c + ((-0.5f * a) * a);
 }
 
 // One DP cell, GPU/align_cuda.cl:532-542 with double transition constants.
+// The reference keeps D unless U is strictly larger, then takes U when it
+// equals the running maximum, and likewise for L: on the values that occur
+// (no NaN, no -0) that is "later candidate wins ties", i.e. >=.
 __device__ __forceinline__ void dp_cell(float diag, float up, float left, float em,
                                         double lp_step, double lp_stay, double lp_skip,
                                         float &score, uint32_t &from) {
@@ -88,14 +140,11 @@ __device__ __forceinline__ void dp_cell(float diag, float up, float left, float 
   const float sd = (float)(((double)diag + lp_step) + emd);
   const float su = (float)(((double)up + lp_stay) + emd);
   const float sl = (float)((double)left + lp_skip);
-  float mx = sd;
-  uint32_t f = FROM_D;
-  mx = su > mx ? su : mx;
-  f = mx == su ? FROM_U : f;
-  mx = sl > mx ? sl : mx;
-  f = mx == sl ? FROM_L : f;
-  score = mx;
-  from = f;
+  const bool u = su >= sd;
+  const float m1 = u ? su : sd;
+  const bool l = sl >= m1;
+  score = l ? sl : m1;
+  from = l ? FROM_L : (u ? FROM_U : FROM_D);
 }
 
 // ---------------------------------------------------------------------------
@@ -109,6 +158,7 @@ __global__ __launch_bounds__(256) void abea_pre_kernel(KernelArgs a, int q_begin
     const char *s = a.seq + a.seq_ptr[r];
     const abea_scalings_t sc = a.scalings[r];
     KmerParam *km = a.km + m.km_off;
+    uint16_t *kr = a.km_rank + m.km_off;
     for (int k = threadIdx.x; k < m.n_k; k += blockDim.x) {
       uint32_t rank = 0;
 #pragma unroll
@@ -117,14 +167,15 @@ __global__ __launch_bounds__(256) void abea_pre_kernel(KernelArgs a, int q_begin
       KmerParam p;
       p.mean = sc.scale * md.level_mean + sc.shift;  // two roundings, as the reference
       p.stdv = md.level_stdv;                        // scaling.var unused, GPU/align_cuda.cl:515
+      p.rstdv = 1.0f / md.level_stdv;                // IEEE (correctly rounded) divide
       p.c = -0.918938f - md.level_log_stdv;          // cached log, GPU/align_cuda.cl:517-518
-      p.rank = (int32_t)rank;
       km[k] = p;
+      kr[k] = (uint16_t)rank;
     }
     float *ev = a.ev + m.ev_off;
     float *endcol = a.endcol + m.ev_off;
     for (int e = threadIdx.x; e < m.n_e; e += blockDim.x) {
-      if (a.events) ev[e] = a.events[m.ev_off + e].mean;
+      ev[e] = a.events ? a.events[m.ev_off + e].mean : a.ev_mean_in[m.ev_off + e];
       endcol[e] = ABEA_NEG_INF;
     }
   }
@@ -132,10 +183,8 @@ __global__ __launch_bounds__(256) void abea_pre_kernel(KernelArgs a, int q_begin
 
 // ---------------------------------------------------------------------------
 // core: the band walk of one read by one wavefront.
-__device__ __forceinline__ void core_read(const KernelArgs &a, int r, int lane) {
-  const ReadMeta m = load_meta(a.meta, r);
+__device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int lane) {
   const int n_e = m.n_e, n_k = m.n_k;
-  if (n_k <= 0 || n_e <= 0) return;
   const int n_bands = n_e + n_k + 2;
   const double lp_stay = m.lp_stay, lp_step = m.lp_step;
   const double lp_skip = a.lp_skip, lp_trim = a.lp_trim;
@@ -144,128 +193,160 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, int r, int lane) 
   c_f32 *evs = (c_f32 *)evp;
   c_f32 *kms = (c_f32 *)kmp;
   float *endcol = a.endcol + m.ev_off;
-  uint8_t *trace = a.trace + m.trace_off;
+  uint32_t *tw = a.trace + m.trace_off + lane;
   uint32_t *moves = a.moves + m.mv_off;
+  int32_t *kck = a.kck + m.mv_off;
   c_u32 *replay = a.replay ? (c_u32 *)(a.replay + m.mv_off) : nullptr;
 
-  const int j0 = 2 * lane, j1 = 2 * lane + 1;
   // band 1 lower-left (E, K) = (50, -51), GPU/align_cuda.cl:251-255
-  int E = ABEA_BANDWIDTH / 2, K = -1 - ABEA_BANDWIDTH / 2;
+  int E = kHalfW, K = -1 - kHalfW;
 
   // Event and k-mer parameters under each slot for band 1's window.
-  float ev0, ev1, km0m, km0s, km0c, km1m, km1s, km1c;
+  float ev0, ev1;
+  KmerParam p0{0.f, 1.f, 1.f, 0.f}, p1{0.f, 1.f, 1.f, 0.f};
   {
-    const int o0 = (j0 - K) & 127, o1 = (j1 - K) & 127;
+    const int o0 = (2 * lane - K) & 127, o1 = (2 * lane + 1 - K) & 127;
     const int e0 = E - o0, e1 = E - o1;
     ev0 = (e0 >= 0 && e0 < n_e) ? evp[e0] : 0.f;
     ev1 = (e1 >= 0 && e1 < n_e) ? evp[e1] : 0.f;
     const int k0 = K + o0, k1 = K + o1;
-    KmerParam p0{0.f, 1.f, 0.f, 0}, p1{0.f, 1.f, 0.f, 0};
     if (k0 >= 0 && k0 < n_k) p0 = kmp[k0];
     if (k1 >= 0 && k1 < n_k) p1 = kmp[k1];
-    km0m = p0.mean; km0s = p0.stdv; km0c = p0.c;
-    km1m = p1.mean; km1s = p1.stdv; km1c = p1.c;
   }
+  float km0m = p0.mean, km0s = p0.stdv, km0r = p0.rstdv, km0c = p0.c;
+  float km1m = p1.mean, km1s = p1.stdv, km1r = p1.rstdv, km1c = p1.c;
   // band 1: only the first trim cell (e=0, k=-1 -> slot 127) = lp_trim
   // (GPU/align_cuda.cl:263-267); band 0: start cell (e=-1,k=-1) = 0, whose
   // left-shifted copy is the diag source of band 2.
   float b1_0 = ABEA_NEG_INF, b1_1 = (lane == 63) ? (float)lp_trim : ABEA_NEG_INF;
   float lv0 = (lane == 0) ? 0.0f : ABEA_NEG_INF, lv1 = ABEA_NEG_INF;
 
+  // lane masks of the full 100-slot window [K, K+99] (parity 0 / 1)
+  uint64_t win0 = slot_lanes(K & 127, ABEA_BANDWIDTH, 0);
+  uint64_t win1 = slot_lanes(K & 127, ABEA_BANDWIDTH, 1);
+
   // scalar prefetch of what band 2 may need: next event, k-mer K+128
-  float nx_ev = evs[min(E + 1, n_e - 1)];
-  int kn = min(K + 128, n_k - 1);
-  float nk_m = kms[4 * kn], nk_s = kms[4 * kn + 1], nk_c = kms[4 * kn + 2];
+  // (staged arrays are padded, so these reads stay inside the buffers)
+  float nx_ev = evs[E + 1];
+  float nk_m = kms[4 * (K + 128)], nk_s = kms[4 * (K + 128) + 1],
+        nk_r = kms[4 * (K + 128) + 2], nk_c = kms[4 * (K + 128) + 3];
 
   const int js = (n_k - 1) & 127;  // slot of the last k-mer (end-cell column)
-  uint32_t mv = 0;
-  uint16_t *row = (uint16_t *)(trace + 2 * kRowBytes) + lane;
+  uint32_t mv = 0, acc = 0;
 
-  for (int b = 2; b < n_bands; ++b, row += kRowBytes / 2) {
-    // Suzuki's rule on band b-1's lower-left and upper-right cells,
-    // GPU/align_cuda.cl:398-410.
-    const float ll = slot_value(b1_0, b1_1, K & 127);
-    const float ur = slot_value(b1_0, b1_1, (K + ABEA_BANDWIDTH - 1) & 127);
-    bool right;
-    if (replay)
-      right = (replay[b >> 5] >> (b & 31)) & 1u;
-    else if (ll == ABEA_NEG_INF && ur == ABEA_NEG_INF)
-      right = (b & 1) != 0;
-    else
-      right = ll < ur;
-    mv |= (uint32_t)right << (b & 31);
-    if ((b & 31) == 31) {
-      if (lane == 0) moves[b >> 5] = mv;
-      mv = 0;
+  for (int b = 2; b < n_bands; ++b) {
+    // Suzuki's rule on band b-1's lower-left (slot K) and upper-right (slot
+    // K+99, the other parity) cells, GPU/align_cuda.cl:398-410.
+    const int jl = K & 127, ju = (K + ABEA_BANDWIDTH - 1) & 127;
+    uint32_t llb, urb;
+    if (K & 1) {
+      llb = lane_bits(b1_1, jl >> 1);
+      urb = lane_bits(b1_0, ju >> 1);
+    } else {
+      llb = lane_bits(b1_0, jl >> 1);
+      urb = lane_bits(b1_1, ju >> 1);
     }
+    const bool both_out = ((llb ^ kNegInfBits) | (urb ^ kNegInfBits)) == 0;
+    uint32_t right = both_out ? ((uint32_t)b & 1u)
+                              : (uint32_t)(order_key(llb) < order_key(urb));
+    if (replay) right = (replay[b >> 5] >> (b & 31)) & 1u;
+    right = (uint32_t)uniform((int)right);
+    mv |= right << (b & 31);
 
+    // Events move up one slot.  Slot K_{b-1} takes event E_{b-1}+1 whatever
+    // the move: on a down move it is the new bottom cell's event; on a right
+    // move that slot leaves the window and the value is exactly the event the
+    // next slot needs one band later.
     const float e_rot = rotate_up(ev1);
     const float l_rot = rotate_up(b1_1);
-    float nev0 = e_rot, nev1 = ev0;
-    if (right) {
-      K += 1;
-      const int jn = (K + 127) & 127;  // slot that now holds k-mer K+127
-      const bool mine = lane == (jn >> 1);
-      if (jn & 1) {
-        km1m = mine ? nk_m : km1m;
-        km1s = mine ? nk_s : km1s;
-        km1c = mine ? nk_c : km1c;
-      } else {
-        km0m = mine ? nk_m : km0m;
-        km0s = mine ? nk_s : km0s;
-        km0c = mine ? nk_c : km0c;
-      }
+    const int jb = K & 127;
+    const uint64_t jb_lane = 1ull << (jb >> 1);
+    if (jb & 1) {
+      ev1 = lane_in(jb_lane) ? nx_ev : ev0;
+      ev0 = e_rot;
     } else {
-      E += 1;
-      const int jb = K & 127;  // bottom slot takes the new event E
-      const bool mine = lane == (jb >> 1);
-      if (jb & 1)
-        nev1 = mine ? nx_ev : nev1;
-      else
-        nev0 = mine ? nx_ev : nev0;
+      ev1 = ev0;
+      ev0 = lane_in(jb_lane) ? nx_ev : e_rot;
     }
-    ev0 = nev0;
-    ev1 = nev1;
-    nx_ev = evs[min(E + 1, n_e - 1)];
-    kn = min(K + 128, n_k - 1);
-    nk_m = kms[4 * kn];
-    nk_s = kms[4 * kn + 1];
-    nk_c = kms[4 * kn + 2];
-
-    // fill range of offsets, GPU/align_cuda.cl:460-469
-    const int lo = max(max(-K, E - (n_e - 1)), 0);
-    const int hi = min(min(n_k - K, E + 1), ABEA_BANDWIDTH);
-    const uint32_t span = hi > lo ? (uint32_t)(hi - lo) : 0u;
-    const int o0 = (j0 - K) & 127, o1 = (j1 - K) & 127;
-    const bool v0 = (uint32_t)(o0 - lo) < span;
-    const bool v1 = (uint32_t)(o1 - lo) < span;
+    if (right) {
+      // slot jb becomes the new window top, k-mer K+128 (K before the move)
+      const uint64_t upd0 = (jb & 1) ? 0ull : jb_lane, upd1 = (jb & 1) ? jb_lane : 0ull;
+      km0m = lane_in(upd0) ? nk_m : km0m;
+      km0s = lane_in(upd0) ? nk_s : km0s;
+      km0r = lane_in(upd0) ? nk_r : km0r;
+      km0c = lane_in(upd0) ? nk_c : km0c;
+      km1m = lane_in(upd1) ? nk_m : km1m;
+      km1s = lane_in(upd1) ? nk_s : km1s;
+      km1r = lane_in(upd1) ? nk_r : km1r;
+      km1c = lane_in(upd1) ? nk_c : km1c;
+      // the window moves up one slot
+      const uint64_t w0 = rotl64(win1, 1);
+      win1 = win0;
+      win0 = w0;
+    }
+    K += (int)right;
+    E += 1 - (int)right;
+    if ((b & 31) == 31) {
+      if (lane == 0) {
+        moves[b >> 5] = mv;
+        kck[b >> 5] = K;
+      }
+      mv = 0;
+    }
+    nx_ev = evs[E + 1];
+    nk_m = kms[4 * (K + 128)];
+    nk_s = kms[4 * (K + 128) + 1];
+    nk_r = kms[4 * (K + 128) + 2];
+    nk_c = kms[4 * (K + 128) + 3];
 
     float M0, M1;
     uint32_t f0, f1;
-    dp_cell(lv0, b1_0, l_rot, emission(ev0, km0m, km0s, km0c), lp_step, lp_stay, lp_skip, M0, f0);
-    dp_cell(lv1, b1_1, b1_0, emission(ev1, km1m, km1s, km1c), lp_step, lp_stay, lp_skip, M1, f1);
-    M0 = v0 ? M0 : ABEA_NEG_INF;
-    M1 = v1 ? M1 : ABEA_NEG_INF;
-    f0 = v0 ? f0 : 0u;
-    f1 = v1 ? f1 : 0u;
-    if (K < 0) {
-      // trim cell (e, k=-1) sits in slot 127 while K <= -1,
-      // GPU/align_cuda.cl:432-443
-      const int et = E - (-1 - K);
-      const bool in = et >= 0 && et < n_e;
-      const float tv = in ? (float)(lp_trim * (double)(et + 1)) : ABEA_NEG_INF;
-      if (lane == 63) {
-        M1 = tv;
-        f1 = in ? FROM_U : 0u;
+    dp_cell(lv0, b1_0, l_rot, emission(ev0, km0m, km0s, km0r, km0c), lp_step, lp_stay, lp_skip,
+            M0, f0);
+    dp_cell(lv1, b1_1, b1_0, emission(ev1, km1m, km1s, km1r, km1c), lp_step, lp_stay, lp_skip,
+            M1, f1);
+
+    // fill range of offsets, GPU/align_cuda.cl:460-469, as lane masks; in
+    // the interior of the matrix it is the whole window
+    const bool full = K >= 0 && E <= n_e - 1 && K <= n_k - ABEA_BANDWIDTH && E >= ABEA_BANDWIDTH - 1;
+    if (full) {
+      M0 = lane_in(win0) ? M0 : ABEA_NEG_INF;
+      M1 = lane_in(win1) ? M1 : ABEA_NEG_INF;
+      if (K == n_k - ABEA_BANDWIDTH) {  // last k-mer at offset 99
+        const float v = (js & 1) ? M1 : M0;
+        if (lane == (js >> 1)) endcol[E - (ABEA_BANDWIDTH - 1)] = v;
+      }
+    } else {
+      const int lo = max(max(-K, E - (n_e - 1)), 0);
+      const int hi = min(min(n_k - K, E + 1), ABEA_BANDWIDTH);
+      const int span = hi - lo;
+      const int first = (K + lo) & 127;
+      M0 = lane_in(slot_lanes(first, span, 0)) ? M0 : ABEA_NEG_INF;
+      M1 = lane_in(slot_lanes(first, span, 1)) ? M1 : ABEA_NEG_INF;
+      if (K < 0) {
+        // trim cell (e, k=-1) sits in slot 127 while K <= -1,
+        // GPU/align_cuda.cl:432-443
+        const int et = E - (-1 - K);
+        const bool in = et >= 0 && et < n_e;
+        const float tv = in ? (float)(lp_trim * (double)(et + 1)) : ABEA_NEG_INF;
+        if (lane == 63) {
+          M1 = tv;
+          f1 = FROM_U;
+        }
+      }
+      // end-cell column (k = n_k - 1) for the end search
+      const int os = (n_k - 1) - K;
+      if (span > 0 && (uint32_t)(os - lo) < (uint32_t)span) {
+        const float v = (js & 1) ? M1 : M0;
+        if (lane == (js >> 1)) endcol[E - os] = v;
       }
     }
-    *row = (uint16_t)(f0 | (f1 << 8));
-
-    // end-cell column (k = n_k - 1) for the post kernel's end search
-    const int os = (n_k - 1) - K;  // offset of the last k-mer in this band
-    if ((uint32_t)(os - lo) < span) {
-      const float v = (js & 1) ? M1 : M0;
-      if (lane == (js >> 1)) endcol[E - os] = v;
+    // codes of cells outside the fill range are never read: the traceback
+    // only follows computed cells
+    acc |= (f0 | (f1 << 2)) << (4 * (b & 7));
+    if ((b & 7) == 7) {
+      tw[(b >> 3) * kTileWords] = acc;
+      acc = 0;
     }
 
     lv0 = l_rot;
@@ -273,146 +354,191 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, int r, int lane) 
     b1_0 = M0;
     b1_1 = M1;
   }
-  if (((n_bands - 1) & 31) != 31 && lane == 0) moves[(n_bands - 1) >> 5] = mv;
-}
-
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q_begin, int q_count,
-                                                             int persistent) {
-  const int lane = threadIdx.x & 63;
-  int qi;
-  if (persistent) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(a.counter, 1);
-    qi = uniform(t);
-  } else {
-    qi = uniform(blockIdx.x * WPB + (threadIdx.x >> 6));
-  }
-  while (qi < q_count) {
-    const int r = ((c_i32 *)a.queue)[q_begin + qi];
-    core_read(a, r, lane);
-    if (!persistent) break;
-    int t = 0;
-    if (lane == 0) t = atomicAdd(a.counter, 1);
-    qi = uniform(t);
-  }
+  const int last = n_bands - 1;
+  if ((last & 7) != 7) tw[(last >> 3) * kTileWords] = acc;
+  if ((last & 31) != 31 && lane == 0) moves[last >> 5] = mv;
 }
 
 // ---------------------------------------------------------------------------
-// post: one wavefront per read.
-//  1. best end cell on the last k-mer (GPU/align_cuda.cl:722-746), wave-wide
-//     with the reference's first-maximum rule
-//  2. traceback (GPU/align_cuda.cl:755-828): the trace rows the path visits
-//     are staged through LDS in chunks of ROWS rows; lane 0 follows the
-//     from-codes and records them in traceback order
-//  3. pairs in ascending order (no reversal pass): the path is rebuilt from
-//     its end cell with a wave prefix sum over the recorded moves; the QC
-//     emission sum runs in the reference's (traceback) order, in double
-//  4. QC (GPU/align_cuda.cl:856-882)
-template <int ROWS>
-__global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin, int q_count) {
-  __shared__ uint4 tile[ROWS * (kRowBytes / 16)];
-  __shared__ float em_buf[64];
-  const int lane = threadIdx.x;
-  const uint8_t *tile_b = (const uint8_t *)tile;
-  for (int qi = blockIdx.x; qi < q_count; qi += gridDim.x) {
-    const int r = ((c_i32 *)a.queue)[q_begin + qi];
-    const ReadMeta m = load_meta(a.meta, r);
-    const int n_e = m.n_e, n_k = m.n_k;
-    const abea_output_t &o = a.out;
-    float best = ABEA_NEG_INF;
-    int best_e = 0x7fffffff;
-    if (n_k > 0 && n_e > 0) {
-      const float *endcol = a.endcol + m.ev_off;
-      for (int e = lane; e < n_e; e += 64) {
-        const float s = (float)((double)endcol[e] + (double)(n_e - e) * a.lp_trim);
-        if (s > best) {
-          best = s;
-          best_e = e;
-        }
+// Traceback.
+
+__device__ __forceinline__ uint32_t trace_code(const uint32_t *tw, int b, int j) {
+  const uint32_t w = tw[(b >> 3) * kTileWords + (j >> 1)];
+  return (w >> (4 * (b & 7) + 2 * (j & 1))) & 3u;
+}
+
+// K (lower-left k-mer) of band b from the move bits and the checkpoints.
+__device__ __forceinline__ int band_k(const uint32_t *mv, const int32_t *kck, int b) {
+  if (b <= 1) return -1 - kHalfW;
+  const int w = b >> 5, bit = b & 31;
+  const int base = w ? kck[w - 1] : -1 - kHalfW;
+  const uint32_t mask = bit == 31 ? 0xffffffffu : ((2u << bit) - 1u);
+  return base + __builtin_popcount(mv[w] & mask);
+}
+
+// Follow from-codes from cell (k, e) down to the first cell below row lo.
+// Records the first cell at or below row hi (the segment's entry), the codes
+// of the cells with lo <= row <= hi (one byte each, traceback order), and the
+// first cell below row lo (the exit; (-1, -1) when the path ends first).
+__device__ __forceinline__ void chase_segment(const uint32_t *tw, int k, int e, int lo, int hi,
+                                              uint32_t *codes, int &ent_k, int &ent_e, int &cnt,
+                                              int &ex_k, int &ex_e) {
+  ent_k = -1;
+  ent_e = -1;
+  ex_k = -1;
+  ex_e = -1;
+  cnt = 0;
+  bool have_ent = false;
+  uint32_t word = 0;
+  while (k >= 0 && e >= 0) {
+    const int b = e + k + 2;
+    if (!have_ent && b <= hi) {
+      have_ent = true;
+      ent_k = k;
+      ent_e = e;
+    }
+    if (b < lo) {
+      ex_k = k;
+      ex_e = e;
+      break;
+    }
+    const uint32_t f = trace_code(tw, b, k & 127);
+    if (b <= hi) {
+      word |= f << (8 * (cnt & 3));
+      if ((cnt & 3) == 3) {
+        codes[cnt >> 2] = word;
+        word = 0;
       }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const float ob = __shfl_xor(best, off);
-        const int oe = __shfl_xor(best_e, off);
-        if (ob > best || (ob == best && oe < best_e)) {
-          best = ob;
-          best_e = oe;
-        }
+      ++cnt;
+    }
+    k -= (f != FROM_U) ? 1 : 0;
+    e -= (f != FROM_L) ? 1 : 0;
+  }
+  if (cnt & 3) codes[cnt >> 2] = word;
+}
+
+// Traceback of one read by one wavefront (the body of the post kernel, and
+// the tail of the fused core).
+//  1. best end cell on the last k-mer (GPU/align_cuda.cl:722-746), wave-wide
+//     with the reference's first-maximum rule.
+//  2. traceback (GPU/align_cuda.cl:755-828), segment-parallel: the rows the
+//     path spans are cut into 64 segments, one per lane.  Each lane follows
+//     the from-codes through its segment starting OVERLAP rows above it (from
+//     the band centre there, or from the true end cell when that is within
+//     reach).  Tracebacks from different cells merge, so the lane is on the
+//     true path by the time it enters its segment.  This is then checked
+//     exactly, top down: a segment is accepted only if the cell where the lane
+//     entered it is the cell where the true path (the accepted segment above)
+//     leaves for it; otherwise that segment is followed again from the true
+//     cell.  The result is the reference's path, whatever the speculation.
+//  3. pairs in ascending order (no reversal pass), rebuilt from the end cell
+//     with a wave prefix sum over the moves; the QC emission sum runs in the
+//     reference's (traceback) order, in double; max gap from ballots.
+//  4. QC (GPU/align_cuda.cl:856-882).
+__device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, int lane,
+                               int overlap, float *em_buf) {
+  const int n_e = m.n_e, n_k = m.n_k;
+  const abea_output_t &o = a.out;
+  float best = ABEA_NEG_INF;
+  int best_e = 0x7fffffff;
+  if (n_k > 0 && n_e > 0) {
+    const float *endcol = a.endcol + m.ev_off;
+    for (int e = lane; e < n_e; e += 64) {
+      const float s = (float)((double)endcol[e] + (double)(n_e - e) * a.lp_trim);
+      if (s > best) {
+        best = s;
+        best_e = e;
       }
     }
-    if (n_k <= 0 || n_e <= 0 || best == ABEA_NEG_INF) {
-      // empty read, or the band never reached the last k-mer (the reference
-      // would trace cells outside the band): reported as a failed read
-      if (lane == 0) {
-        o.n_pairs[r] = 0;
-        if (o.path_len) o.path_len[r] = 0;
-        if (o.score) o.score[r] = best;
-        if (o.avg_log_emission) o.avg_log_emission[r] = 0.0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float ob = __shfl_xor(best, off);
+      const int oe = __shfl_xor(best_e, off);
+      if (ob > best || (ob == best && oe < best_e)) {
+        best = ob;
+        best_e = oe;
       }
+    }
+  }
+  if (n_k <= 0 || n_e <= 0 || best == ABEA_NEG_INF) {
+    // empty read, or the band never reached the last k-mer (the reference
+    // would trace cells outside the band): reported as a failed read
+    if (lane == 0) {
+      o.n_pairs[r] = 0;
+      if (o.path_len) o.path_len[r] = 0;
+      if (o.score) o.score[r] = best;
+      if (o.avg_log_emission) o.avg_log_emission[r] = 0.0;
+    }
+    return;
+  }
+
+  const uint32_t *tw = a.trace + m.trace_off;
+  const uint32_t *mvw = a.moves + m.mv_off;
+  const int32_t *kck = a.kck + m.mv_off;
+  uint8_t *seg = a.seg + m.seg_off;
+  const int e_end = uniform(best_e), k_end = n_k - 1;
+  const int b_end = e_end + k_end + 2;
+  // rows 2..b_end in 64 segments of S rows (S a multiple of 4: word-aligned
+  // code storage per segment)
+  const int rows = b_end - 1;
+  const int S = (((rows + 63) >> 6) + 3) & ~3;
+  const int nseg = (rows + S - 1) / S;
+
+  // ---- phase A: every lane follows its own segment
+  int ent_k = -1, ent_e = -1, cnt = 0, ex_k = -1, ex_e = -1;
+  const int lo = 2 + lane * S;
+  const int hi = min(lo + S - 1, b_end);
+  uint32_t *codes = (uint32_t *)(seg + lane * S);
+  if (lane < nseg) {
+    int k = k_end, e = e_end;  // exact start: the end cell itself
+    const int s = hi + overlap;
+    if (s < b_end) {
+      const int Ks = band_k(mvw, kck, s), Es = s - 2 - Ks;
+      const int slo = max(max(-Ks, Es - (n_e - 1)), 0);
+      const int shi = min(min(n_k - Ks, Es + 1), ABEA_BANDWIDTH);
+      if (shi > slo) {
+        const int off = min(max(kHalfW, slo), shi - 1);
+        k = Ks + off;
+        e = Es - off;
+      }
+    }
+    chase_segment(tw, k, e, lo, hi, codes, ent_k, ent_e, cnt, ex_k, ex_e);
+  }
+
+  // ---- phase B: accept segments top down, following again where needed
+  int tk = k_end, te = e_end;
+  int P = 0;
+  for (int i = nseg - 1; i >= 0; --i) {
+    if (tk < 0) {  // the path ended above this segment
+      if (lane == i) cnt = 0;
       continue;
     }
-
-    // ---- traceback
-    uint8_t *rev = a.rev_from + m.pair_off;
-    const uint8_t *trace = a.trace + m.trace_off;
-    int k = n_k - 1, e = uniform(best_e);
-    int b = e + k + 2;
-    int n = 0, gap = 0, max_gap = 0;
-    int lo = b - ROWS + 1;
-    for (;;) {
-      const int r0 = max(lo, 0);
-      const uint4 *src = (const uint4 *)(trace + (int64_t)r0 * kRowBytes);
-      const int nvec = (b - r0 + 1) * (kRowBytes / 16);
-      const int base = (r0 - lo) * (kRowBytes / 16);
-      for (int i = lane; i < nvec; i += 64) tile[base + i] = src[i];
-      __syncthreads();
-      if (lane == 0) {
-        while (b >= lo) {
-          const uint32_t f = tile_b[(b - lo) * kRowBytes + (k & 127)];
-          rev[n++] = (uint8_t)f;
-          if (f == FROM_D) {
-            --k;
-            --e;
-            b -= 2;
-            gap = 0;
-          } else if (f == FROM_U) {
-            --e;
-            --b;
-            gap = 0;
-          } else {
-            --k;
-            --b;
-            ++gap;
-            max_gap = max(max_gap, gap);
-          }
-          if (k < 0 || e < 0) break;
-        }
-      }
-      __syncthreads();
-      k = __shfl(k, 0);
-      e = __shfl(e, 0);
-      b = __shfl(b, 0);
-      n = __shfl(n, 0);
-      if (k < 0 || e < 0) break;
-      lo -= ROWS;
+    const int ik = __builtin_amdgcn_readlane(ent_k, i), ie = __builtin_amdgcn_readlane(ent_e, i);
+    if (ik != tk || ie != te) {
+      if (lane == i) chase_segment(tw, tk, te, lo, hi, codes, ent_k, ent_e, cnt, ex_k, ex_e);
     }
-    max_gap = __shfl(max_gap, 0);
-    const int P = n;
-    __threadfence();  // lane 0's from-codes become visible to the whole wave
+    P += __builtin_amdgcn_readlane(cnt, i);
+    tk = __builtin_amdgcn_readlane(ex_k, i);
+    te = __builtin_amdgcn_readlane(ex_e, i);
+  }
+  wave_sync();  // segment codes are visible to the whole wave
 
-    // ---- ascending pairs, from the end cell down, 64 pairs per step
-    const float *evp = a.ev + m.ev_off;
-    const KmerParam *kmp = a.km + m.km_off;
-    abea_aligned_pair_t *pairs = o.pairs + m.pair_off;
-    uint8_t *from_out = o.path_from ? o.path_from + m.pair_off : nullptr;
-    int ck = n_k - 1, ce = uniform(best_e);  // position of the chunk's top pair
-    double sum_em = 0.0;
-    int first_k = 0;
-    for (int t0 = 0; t0 < P; t0 += 64) {
-      const int t = t0 + lane;  // t-th pair in traceback order = ascending index P-1-t
-      const bool act = t < P;
-      const uint32_t f = act ? rev[t] : FROM_D;
+  // ---- phase C: ascending pairs, QC statistics
+  const float *evp = a.ev + m.ev_off;
+  const KmerParam *kmp = a.km + m.km_off;
+  abea_aligned_pair_t *pairs = o.pairs + m.pair_off;
+  uint8_t *from_out = o.path_from ? o.path_from + m.pair_off : nullptr;
+  int ck = k_end, ce = e_end;  // position of the next pair in traceback order
+  double sum_em = 0.0;
+  int first_k = k_end, run = 0, max_gap = 0;
+  int t_base = 0;  // traceback index of the current segment's first code
+  for (int i = nseg - 1; i >= 0; --i) {
+    const int ci = __builtin_amdgcn_readlane(cnt, i);
+    const uint8_t *sc = seg + i * S;
+    for (int u0 = 0; u0 < ci; u0 += 64) {
+      const int u = u0 + lane;
+      const bool act = u < ci;
+      const uint32_t f = act ? sc[u] : FROM_D;
       // move from ascending pair i-1 to pair i (= inverse of the from code)
       const int dk = act && f != FROM_U ? 1 : 0;
       const int de = act && f != FROM_L ? 1 : 0;
@@ -428,34 +554,96 @@ __global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin
       }
       const int kk = ck - (pk - dk);
       const int ee = ce - (pe - de);
+      const int nact = min(64, ci - u0);
       if (act) {
-        const int i = P - 1 - t;
-        pairs[i].ref_pos = kk;
-        pairs[i].read_pos = ee;
-        if (from_out) from_out[i] = (uint8_t)f;
+        const int idx = P - 1 - (t_base + u);
+        pairs[idx].ref_pos = kk;
+        pairs[idx].read_pos = ee;
+        if (from_out) from_out[idx] = (uint8_t)f;
         // indices stay inside the read even if the path were malformed
         const KmerParam p = kmp[min(max(kk, 0), n_k - 1)];
-        em_buf[lane] = emission(evp[min(max(ee, 0), n_e - 1)], p.mean, p.stdv, p.c);
+        em_buf[lane] = emission(evp[min(max(ee, 0), n_e - 1)], p.mean, p.stdv, p.rstdv, p.c);
       }
-      if (t0 + 64 >= P) first_k = __shfl(kk, (P - 1 - t0) & 63);
+      first_k = __shfl(kk, nact - 1);
       ck -= __shfl(pk, 63);
       ce -= __shfl(pe, 63);
-      __syncthreads();
-      if (lane == 0) {
-        const int cnt = min(64, P - t0);
-        for (int i = 0; i < cnt; ++i) sum_em += (double)em_buf[i];
+      // max run of consecutive L codes (the reference's gap counter)
+      const uint64_t full = nact == 64 ? ~0ull : ((1ull << nact) - 1);
+      const uint64_t lm = __ballot(act && f == FROM_L) & full;
+      if (lm == full) {
+        run += nact;
+        max_gap = max(max_gap, run);
+      } else {
+        const int lead = __builtin_ctzll(~lm);
+        max_gap = max(max_gap, run + lead);
+        uint64_t x = lm;
+        int rmax = 0;
+        while (x) {
+          x &= x >> 1;
+          ++rmax;
+        }
+        max_gap = max(max_gap, rmax);
+        const uint64_t zeros = ~lm & full;
+        run = nact - 1 - (63 - __builtin_clzll(zeros));
       }
-      __syncthreads();
+      wave_sync();
+      if (lane == 0) {
+        for (int q = 0; q < nact; ++q) sum_em += (double)em_buf[q];
+      }
+      wave_sync();
     }
-    if (lane == 0) {
-      const double avg = sum_em / (double)P;
-      const bool spanned = first_k == 0;  // last pair is always on k-mer n_k-1
-      const bool fail = avg < -5.0 || !spanned || max_gap > 50;  // GPU/align_cuda.cl:134,139,873
-      o.n_pairs[r] = fail ? 0 : P;
-      if (o.path_len) o.path_len[r] = P;
-      if (o.score) o.score[r] = best;
-      if (o.avg_log_emission) o.avg_log_emission[r] = avg;
+    t_base += ci;
+  }
+  if (lane == 0) {
+    const double avg = sum_em / (double)P;
+    const bool spanned = first_k == 0;  // the last pair is always on k-mer n_k-1
+    const bool fail = avg < -5.0 || !spanned || max_gap > 50;  // GPU/align_cuda.cl:134,139,873
+    o.n_pairs[r] = fail ? 0 : P;
+    if (o.path_len) o.path_len[r] = P;
+    if (o.score) o.score[r] = best;
+    if (o.avg_log_emission) o.avg_log_emission[r] = avg;
+  }
+}
+
+// Persistent (or one-read-per-wave) core; FUSE appends the traceback of each
+// read to its band walk in the same wave.
+template <int WPB, bool FUSE>
+__global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q_begin, int q_count,
+                                                             int persistent, int overlap) {
+  __shared__ float em_bufs[WPB][64];
+  const int lane = threadIdx.x & 63;
+  const int wib = uniform(threadIdx.x >> 6);
+  int qi;
+  if (persistent) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(a.counter, 1);
+    qi = uniform(t);
+  } else {
+    qi = uniform(blockIdx.x * WPB + wib);
+  }
+  while (qi < q_count) {
+    const int r = ((c_i32 *)a.queue)[q_begin + qi];
+    const ReadMeta m = load_meta(a.meta, r);
+    if (m.n_k > 0 && m.n_e > 0) core_read(a, m, lane);
+    if constexpr (FUSE) {
+      wave_sync();  // this wave's trace, moves and end column are visible to it
+      traceback_read(a, m, r, lane, overlap, em_bufs[wib]);
     }
+    if (!persistent) break;
+    int t = 0;
+    if (lane == 0) t = atomicAdd(a.counter, 1);
+    qi = uniform(t);
+  }
+}
+
+__global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin, int q_count,
+                                                      int overlap) {
+  __shared__ float em_buf[64];
+  const int lane = threadIdx.x;
+  for (int qi = blockIdx.x; qi < q_count; qi += gridDim.x) {
+    const int r = ((c_i32 *)a.queue)[q_begin + qi];
+    const ReadMeta m = load_meta(a.meta, r);
+    traceback_read(a, m, r, lane, overlap, em_buf);
   }
 }
 
@@ -478,14 +666,14 @@ __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const f
     const abea_aligned_pair_t *pairs = a.out.pairs + m.pair_off;
     const uint8_t *from = a.out.path_from + m.pair_off;
     const float *evp = a.ev + m.ev_off;
-    const KmerParam *kmp = a.km + m.km_off;
+    const uint16_t *kr = a.km_rank + m.km_off;
     double gs = 0.0, gc = 0.0;
     for (int i = lane; i < P; i += 64) {
       if (from[i] == FROM_L) continue;
       const abea_aligned_pair_t pr = pairs[i];
       const int kk = min(max(pr.ref_pos, 0), m.n_k - 1);
       const int ee = min(max(pr.read_pos, 0), m.n_e - 1);
-      const int rank = kmp[kk].rank & (ABEA_NUM_KMER - 1);
+      const int rank = kr[kk] & (ABEA_NUM_KMER - 1);
       const abea_model_t md = a.model[rank];
       const double mu = (double)md.level_mean, sg = (double)md.level_stdv;
       const double inv = 1.0 / sg;
@@ -522,46 +710,40 @@ hipError_t launch_pre(const KernelArgs &a, int q_begin, int q_count, hipStream_t
   return hipGetLastError();
 }
 
+template <int WPB, bool FUSE>
+static void launch_core_t(const KernelArgs &a, int q_begin, int q_count, int p, int grid,
+                          int overlap, hipStream_t s) {
+  hipLaunchKernelGGL((abea_core_kernel<WPB, FUSE>), dim3(grid), dim3(64 * WPB), 0, s, a, q_begin,
+                     q_count, p, overlap);
+}
+
 hipError_t launch_core(const KernelArgs &a, int q_begin, int q_count, bool persistent,
-                       int waves_per_block, int grid_blocks, hipStream_t s) {
+                       int waves_per_block, int grid_blocks, int fuse, int overlap, hipStream_t s) {
   if (q_count <= 0) return hipSuccess;
   const int p = persistent ? 1 : 0;
   int grid = persistent ? grid_blocks : (q_count + waves_per_block - 1) / waves_per_block;
   if (grid < 1) grid = 1;
-  switch (waves_per_block) {
-    case 1:
-      hipLaunchKernelGGL(abea_core_kernel<1>, dim3(grid), dim3(64), 0, s, a, q_begin, q_count, p);
-      break;
-    case 2:
-      hipLaunchKernelGGL(abea_core_kernel<2>, dim3(grid), dim3(128), 0, s, a, q_begin, q_count, p);
-      break;
-    case 4:
-      hipLaunchKernelGGL(abea_core_kernel<4>, dim3(grid), dim3(256), 0, s, a, q_begin, q_count, p);
-      break;
-    default:
-      return hipErrorInvalidValue;
+#define ABEA_CORE_CASE(W_, F_)                                        \
+  if (waves_per_block == W_ && (fuse != 0) == F_) {                  \
+    launch_core_t<W_, F_>(a, q_begin, q_count, p, grid, overlap, s); \
+    return hipGetLastError();                                        \
   }
-  return hipGetLastError();
+  ABEA_CORE_CASE(1, false)
+  ABEA_CORE_CASE(2, false)
+  ABEA_CORE_CASE(4, false)
+  ABEA_CORE_CASE(1, true)
+  ABEA_CORE_CASE(2, true)
+  ABEA_CORE_CASE(4, true)
+#undef ABEA_CORE_CASE
+  return hipErrorInvalidValue;
 }
 
 int core_max_waves_per_block() { return 4; }
 
-hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int rows, hipStream_t s) {
+hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overlap, hipStream_t s) {
   if (q_count <= 0) return hipSuccess;
   const int grid = q_count < 65536 ? q_count : 65536;
-  switch (rows) {
-    case 32:
-      hipLaunchKernelGGL(abea_post_kernel<32>, dim3(grid), dim3(64), 0, s, a, q_begin, q_count);
-      break;
-    case 64:
-      hipLaunchKernelGGL(abea_post_kernel<64>, dim3(grid), dim3(64), 0, s, a, q_begin, q_count);
-      break;
-    case 128:
-      hipLaunchKernelGGL(abea_post_kernel<128>, dim3(grid), dim3(64), 0, s, a, q_begin, q_count);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+  hipLaunchKernelGGL(abea_post_kernel, dim3(grid), dim3(64), 0, s, a, q_begin, q_count, overlap);
   return hipGetLastError();
 }
 

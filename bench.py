@@ -1,0 +1,223 @@
+"""ABEA benchmark on one MI355X.
+
+Headline: batched persistent ABEA (length-sorted queue, persistent core,
+pre->core->post replayed as one hipGraph, autotuned launch shapes) over the
+16,384-read long-tailed workload (log-normal median 8 kb, clipped 500 b-200 kb,
+~150 Mbases, ~40 G nominal cells), inputs resident in HBM as f5c event_t
+records.  A "step" is one full forward over that batch.
+
+Also reported (same JSON line): the plain per-read-launch form and the batched
+form on a sample of the 4,096-read log-normal (median 5 kb) workload; the
+adjoint/forward time ratio on 1,024 x 10 kb reads; the CPU baseline (the
+oracle, OpenMP over reads on this host) on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "abea-opencl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+W = 100
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def core_alg_bytes(b) -> int:
+    """Algorithmic HBM bytes of the core kernel for a batch: the uint8 trace
+    (1 byte per nominal cell), event means (4 B/event), staged k-mer
+    parameters (12 B/k-mer), end column (4 B/event) and band-move bits."""
+    nb = b.n_bands.astype("int64")
+    ok = (b.n_kmers > 0) & (b.n_events > 0)
+    return int((W * nb[ok]).sum() + 8 * b.n_events[ok].sum() + 12 * b.n_kmers[ok].sum()
+               + (nb[ok] // 8).sum())
+
+
+def timed_steps(plan, steps, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.forward()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true", help="headline only")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--plain-reads", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: every read is independent and this build "
+              "measures one GPU's share (see the sharding note in DESIGN.md); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from abea_amd import engine, synth
+    from abea_amd._lib import PLAN_BATCHED, PLAN_PLAIN
+
+    t = time.perf_counter()
+    wl = "batched16384_ln8kb"
+    b = synth.workload(wl, seed=args.seed, with_aos=True)
+    cells = b.nominal_cells()
+    log(f"[bench] generated {wl}: {b.n_reads} reads, {int(b.read_len.sum()) / 1e6:.1f} Mb, "
+        f"{int(b.n_events.sum()) / 1e6:.1f} M events, {cells / 1e9:.2f} G cells "
+        f"in {time.perf_counter() - t:.1f}s")
+    ctx = engine.Context(0)
+    db = engine.DeviceBatch.upload(b, 0, aos_events=True)
+    plan = engine.Plan(ctx, db, PLAN_BATCHED)
+    log(f"[bench] plan workspace {plan.workspace_bytes() / 1e9:.1f} GB")
+    plan.forward()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    plan.autotune(reps=1)
+    tune = plan.tuning()
+    log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s: {tune}")
+
+    # headline: graph replay, whole steps
+    plan.set_tuning(use_graph=1)
+    for _ in range(args.warmup):
+        plan.forward()
+    sec = timed_steps(plan, args.steps, torch)
+    gcups = cells / sec / 1e9
+    log(f"[bench] batched graph: {sec * 1e3:.2f} ms/step, {gcups:.1f} GCUPS")
+
+    # per-kernel device times over the same number of eager steps
+    plan.set_tuning(use_graph=0)
+    plan.forward()
+    kt = np.zeros(3)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        plan.forward()
+        kt += np.array(plan.times()[:3])
+    kt /= args.steps
+    log(f"[bench] kernel ms pre/core/post: {kt.round(3).tolist()}")
+    res = plan.results()
+    qc_pass = int((res["n_pairs"] > 0).sum())
+    core_bytes = core_alg_bytes(b)
+    achieved = core_bytes / (kt[1] * 1e-3) / 1e9
+    out = {
+        "metric": "ABEA GCUPS (batched persistent form, 1 GPU)",
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 cells, f64 transition sums",
+        "data": "synthetic (seeded; f5c event_t records in HBM)",
+        "config": {"workload": "batched16384_ln8kb", "reads": b.n_reads,
+                   "bases": int(b.read_len.sum()), "events": int(b.n_events.sum()),
+                   "nominal_cells": cells, "bandwidth": W, "tuning": tune},
+        "reads_per_s": round(b.n_reads / sec, 1),
+        "kernel_ms": {"pre": round(kt[0], 3), "core": round(kt[1], 3), "post": round(kt[2], 3)},
+        "qc_pass_reads": qc_pass,
+        "roofline": {"bound": "hbm", "kernel": "abea_core_kernel", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": core_bytes},
+    }
+    del plan, db
+    torch.cuda.empty_cache()
+
+    if not args.no_extras:
+        # plain per-read launches vs batched on the same sample of the
+        # 4,096-read log-normal (median 5 kb) workload
+        L1 = synth.workload_lengths("plain4096_ln5kb", seed=args.seed)
+        b1 = synth.generate(L1, seed=args.seed + 1)
+        sub = b1.subset(np.arange(min(args.plain_reads, b1.n_reads)))
+        dsub = engine.DeviceBatch.upload(sub, 0)
+        p_plain = engine.Plan(ctx, dsub, PLAN_PLAIN)
+        p_plain.forward()
+        s_plain = timed_steps(p_plain, 1, torch)
+        p_b = engine.Plan(ctx, dsub, PLAN_BATCHED)
+        p_b.forward()
+        s_b = timed_steps(p_b, 3, torch)
+        d1 = engine.DeviceBatch.upload(b1, 0)
+        p_full = engine.Plan(ctx, d1, PLAN_BATCHED)
+        p_full.set_tuning(**{k: v for k, v in tune.items()})
+        p_full.forward()
+        s_full = timed_steps(p_full, 3, torch)
+        out["plain_vs_batched"] = {
+            "workload": "plain4096_ln5kb", "sample_reads": sub.n_reads,
+            "plain_gcups": round(sub.nominal_cells() / s_plain / 1e9, 3),
+            "batched_gcups_same_sample": round(sub.nominal_cells() / s_b / 1e9, 3),
+            "batched_gcups_all_4096": round(b1.nominal_cells() / s_full / 1e9, 3),
+            "plain_reads_per_s": round(sub.n_reads / s_plain, 2),
+            "batched_reads_per_s_all_4096": round(b1.n_reads / s_full, 1),
+        }
+        log(f"[bench] plain/batched: {out['plain_vs_batched']}")
+        del p_plain, p_b, p_full, d1, dsub
+
+        # adjoint / forward on 1,024 x 10 kb
+        b3 = synth.workload("adjoint1024x10kb", seed=args.seed + 3)
+        d3 = engine.DeviceBatch.upload(b3, 0)
+        p3 = engine.Plan(ctx, d3, PLAN_BATCHED)
+        p3.set_tuning(**{k: v for k, v in tune.items()})
+        p3.forward()
+        s_fwd = timed_steps(p3, 3, torch)
+        gsc = torch.ones(b3.n_reads, device="cuda")
+        p3.adjoint(gsc)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            p3.adjoint(gsc)
+        torch.cuda.synchronize()
+        s_adj = (time.perf_counter() - t) / 3
+        out["adjoint"] = {"workload": "adjoint1024x10kb", "forward_ms": round(s_fwd * 1e3, 3),
+                          "adjoint_ms": round(s_adj * 1e3, 3),
+                          "adjoint_over_forward": round(s_adj / s_fwd, 4)}
+        log(f"[bench] adjoint: {out['adjoint']}")
+        del p3, d3
+
+        # CPU baseline: the oracle (f5c semantics), OpenMP over reads, on a
+        # bounded random sample of the headline workload
+        if args.cpu_seconds > 0:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            rng = np.random.default_rng(args.seed)
+            order = rng.permutation(b.n_reads)
+            probe = b.subset(order[:32])
+            t = time.perf_counter()
+            _, _, used, _ = oracle.align_batch(probe)
+            rate = probe.nominal_cells() / (time.perf_counter() - t)
+            want = rate * args.cpu_seconds
+            cum = np.cumsum(b.n_bands[order] * W)
+            n_s = int(min(b.n_reads, max(32, np.searchsorted(cum, want))))
+            samp = b.subset(order[:n_s])
+            t = time.perf_counter()
+            _, _, used, _ = oracle.align_batch(samp)
+            dt = time.perf_counter() - t
+            cpu_gcups = samp.nominal_cells() / dt / 1e9
+            out["cpu_baseline"] = {
+                "value": round(cpu_gcups, 4), "unit": "GCUPS", "cores": used, "kind": "port",
+                "sample": f"{n_s} random reads of batched16384_ln8kb "
+                          f"({samp.nominal_cells() / 1e9:.2f} G cells, {dt:.1f} s)",
+                "reads_per_s": round(n_s / dt, 2)}
+            out["gpu_over_cpu"] = round(gcups / cpu_gcups, 1)
+            log(f"[bench] cpu: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -186,8 +186,11 @@ abea_status_t abea_plan_adjoint(abea_plan_t *plan, const abea_output_t *fwd,
 typedef struct {
   int32_t core_waves_per_block; /* 1, 2 or 4 waves per workgroup */
   int32_t core_waves_per_cu;    /* resident waves per CU for the persistent grid */
-  int32_t post_rows;            /* trace rows per LDS chunk in the traceback */
+  int32_t traceback_overlap;    /* rows above its segment where each traceback lane starts
+                                   following the trace (32..1024); affects time, not results */
   int32_t use_graph;            /* replay pre->core->post as one hipGraph */
+  int32_t fuse_traceback;       /* 0: separate post kernel; 1: each core wave traces its
+                                   own read back */
 } abea_tuning_t;
 
 abea_status_t abea_plan_get_tuning(const abea_plan_t *plan, abea_tuning_t *t);

@@ -1,0 +1,21 @@
+"""The core kernel's emission quotient (correctly rounded reciprocal + one
+fused residual correction) must equal IEEE single-precision division, which
+is what the reference computes (GPU/align_cuda.cl:527).  Checked on the CPU
+with glibc's correctly rounded fmaf."""
+import os
+import subprocess
+
+import numpy as np
+
+from conftest import GOLDEN, ROOT
+
+
+def test_reciprocal_fma_quotient_equals_ieee_division(tmp_path):
+    src = os.path.join(ROOT, "tests", "native", "division_check.c")
+    exe = str(tmp_path / "division_check")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, src, "-lm"], check=True)
+    model = tmp_path / "model.bin"
+    np.load(GOLDEN)["model"].astype(np.float32).tofile(model)
+    r = subprocess.run([exe, str(model)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 mismatches" in r.stdout

@@ -142,11 +142,16 @@ def test_tunings_agree(ctx):
     """Every launch shape the autotuner may pick gives identical results."""
     b = edge_batch()
     ref = run(ctx, b, 1)
-    for wpb, wpc, rows in [(1, 8, 32), (2, 16, 64), (4, 32, 128), (4, 16, 32)]:
+    # overlap 32 makes most traceback segments miss the path and be followed
+    # again; 1024 starts nearly every lane at the true end cell
+    for wpb, wpc, overlap, fuse in [(1, 8, 32, 0), (2, 16, 128, 0), (4, 32, 1024, 0),
+                                    (4, 16, 64, 0), (1, 16, 32, 1), (2, 24, 128, 1),
+                                    (4, 32, 256, 1), (1, 8, 1024, 1)]:
         r = run(ctx, b, 1, tune=dict(core_waves_per_block=wpb, core_waves_per_cu=wpc,
-                                     post_rows=rows, use_graph=0))
+                                     traceback_overlap=overlap, use_graph=0,
+                                     fuse_traceback=fuse))
         for k in ("n_pairs", "path_len", "score", "avg", "pairs", "path_from", "moves"):
-            assert np.array_equal(ref[k], r[k]), (k, wpb, wpc, rows)
+            assert np.array_equal(ref[k], r[k]), (k, wpb, wpc, overlap, fuse)
 
 
 def test_graph_replays_match_eager(ctx):
