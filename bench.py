@@ -26,20 +26,36 @@ sys.path.insert(0, os.path.join(ROOT, "abea-opencl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 W = 100
+# HBM bytes per launch of the core kernel on this workload, from rocprofv3
+# FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings (the
+# profiles/ summary for the batched 16,384-read workload gives the runs).
+TRAFFIC_BYTES = None
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def core_alg_bytes(b) -> int:
-    """Algorithmic HBM bytes of the core kernel for a batch: the uint8 trace
-    (1 byte per nominal cell), event means (4 B/event), staged k-mer
-    parameters (12 B/k-mer), end column (4 B/event) and band-move bits."""
+def alg_bytes(b, path_len) -> tuple[int, int]:
+    """Algorithmic HBM bytes of one forward, split (band walk, traceback).
+
+    Per read: W*n_bands (a one-byte from-code per band cell) + 4*n_e (event
+    means) + 2*n_k (k-mer ranks) + n_bands/8 (band-move bits) + 16
+    (scalings) for the band walk; path_len (from-codes read back) +
+    8*path_len (aligned pairs) for the traceback.  The library stores 2-bit
+    codes, so its actual traffic is below this figure; the pmc summaries
+    under profiles/ give the measured bytes."""
     nb = b.n_bands.astype("int64")
     ok = (b.n_kmers > 0) & (b.n_events > 0)
-    return int((W * nb[ok]).sum() + 8 * b.n_events[ok].sum() + 12 * b.n_kmers[ok].sum()
-               + (nb[ok] // 8).sum())
+    walk = int((W * nb[ok]).sum() + 4 * b.n_events[ok].sum() + 2 * b.n_kmers[ok].sum()
+               + (nb[ok] // 8).sum() + 16 * ok.sum())
+    tb = int(9 * path_len[ok].astype("int64").sum())
+    return walk, tb
+
+
+def kernel_name(tune) -> str:
+    return (f"abea_core_kernel<{tune['core_waves_per_block']}, "
+            f"{'true' if tune['fuse_traceback'] else 'false'}>")
 
 
 def timed_steps(plan, steps, torch):
@@ -60,10 +76,16 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="headline only")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-reads", type=int, default=256)
+    ap.add_argument("--profile", action="store_true",
+                    help="headline workload only, default tuning, no autotune; also one "
+                         "1 GiB device copy to calibrate HBM byte counters")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of the captured graph")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: every read is independent and this build "
-              "measures one GPU's share (see the sharding note in DESIGN.md); use --gpus 1",
+        print("bench.py runs on one GPU only: reads are independent, so N GPUs would each "
+              "run this same batched plan on a length-balanced share of the reads with no "
+              "exchange; the figure measured here is one GPU's. Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -86,18 +108,30 @@ def main():
     log(f"[bench] plan workspace {plan.workspace_bytes() / 1e9:.1f} GB")
     plan.forward()
     torch.cuda.synchronize()
-    t = time.perf_counter()
-    plan.autotune(reps=1)
+    if args.profile:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE
+        # readings scale the forward's counters to bytes
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
+    else:
+        t = time.perf_counter()
+        plan.autotune(reps=1)
+        log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s")
     tune = plan.tuning()
-    log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s: {tune}")
+    log(f"[bench] tuning {tune}")
 
     # headline: graph replay, whole steps
-    plan.set_tuning(use_graph=1)
+    plan.set_tuning(use_graph=0 if args.eager else 1)
     for _ in range(args.warmup):
         plan.forward()
     sec = timed_steps(plan, args.steps, torch)
     gcups = cells / sec / 1e9
-    log(f"[bench] batched graph: {sec * 1e3:.2f} ms/step, {gcups:.1f} GCUPS")
+    log(f"[bench] batched {'eager' if args.eager else 'graph'}: {sec * 1e3:.2f} ms/step, "
+        f"{gcups:.1f} GCUPS")
 
     # per-kernel device times over the same number of eager steps
     plan.set_tuning(use_graph=0)
@@ -111,7 +145,9 @@ def main():
     log(f"[bench] kernel ms pre/core/post: {kt.round(3).tolist()}")
     res = plan.results()
     qc_pass = int((res["n_pairs"] > 0).sum())
-    core_bytes = core_alg_bytes(b)
+    walk_b, tb_b = alg_bytes(b, res["path_len"])
+    # the dominant kernel is the core; fused, it also does the traceback
+    core_bytes = walk_b + (tb_b if tune["fuse_traceback"] else 0)
     achieved = core_bytes / (kt[1] * 1e-3) / 1e9
     out = {
         "metric": "ABEA GCUPS (batched persistent form, 1 GPU)",
@@ -132,15 +168,15 @@ def main():
         "reads_per_s": round(b.n_reads / sec, 1),
         "kernel_ms": {"pre": round(kt[0], 3), "core": round(kt[1], 3), "post": round(kt[2], 3)},
         "qc_pass_reads": qc_pass,
-        "roofline": {"bound": "hbm", "kernel": "abea_core_kernel", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernel_name(tune), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES,
                      "alg_bytes_per_launch": core_bytes},
     }
     del plan, db
     torch.cuda.empty_cache()
 
-    if not args.no_extras:
+    if not args.no_extras and not args.profile:
         # plain per-read launches vs batched on the same sample of the
         # 4,096-read log-normal (median 5 kb) workload
         L1 = synth.workload_lengths("plain4096_ln5kb", seed=args.seed)
