@@ -121,30 +121,66 @@ __device__ __forceinline__ uint32_t base_code(char c) {
 // (Markstein): q0 = n*rs, r = fma(-q0, s, n) (exact), a = fma(r, rs, q0).
 // This equals IEEE division for the finite, non-underflowing quotients that
 // occur here and costs 4 operations instead of the ~10 of a divide.
+// The last step uses that halving is exact: RN(RN(-0.5a)*a) = -0.5*RN(a*a)
+// whenever 0.5a^2 is a normal float, so c + (-0.5a)a = fma(RN(a*a), -0.5, c)
+// with the same rounding.  For 0 < |a| < 2^-62 both sides round to c unless
+// c is exactly 0 (level_log_stdv == -0.918938f).
 __device__ __forceinline__ float emission(float x, float m, float s, float rs, float c) {
   const float n = x - m;
   const float q0 = n * rs;
   const float r = __builtin_fmaf(-q0, s, n);
   const float a = __builtin_fmaf(r, rs, q0);
-  return c + ((-0.5f * a) * a);
+  return __builtin_fmaf(a * a, -0.5f, c);
 }
 
-// One DP cell, GPU/align_cuda.cl:532-542 with double transition constants.
-// The reference keeps D unless U is strictly larger, then takes U when it
-// equals the running maximum, and likewise for L: on the values that occur
-// (no NaN, no -0) that is "later candidate wins ties", i.e. >=.
-__device__ __forceinline__ void dp_cell(float diag, float up, float left, float em,
-                                        double lp_step, double lp_stay, double lp_skip,
-                                        float &score, uint32_t &from) {
+// One DP cell, GPU/align_cuda.cl:532-542 with double transition constants;
+// the neighbour scores arrive widened to double (exact).  The reference keeps
+// D unless U is strictly larger, then takes U when it equals the running
+// maximum, and likewise for L: on the values that occur (no NaN, no -0) that
+// is "later candidate wins ties", i.e. >=.  u / l report the two decisions.
+__device__ __forceinline__ float dp_cell(double diag, double up, double left, float em,
+                                         double lp_step, double lp_stay, double lp_skip,
+                                         bool &u, bool &l) {
   const double emd = (double)em;
-  const float sd = (float)(((double)diag + lp_step) + emd);
-  const float su = (float)(((double)up + lp_stay) + emd);
-  const float sl = (float)((double)left + lp_skip);
-  const bool u = su >= sd;
+  const float sd = (float)((diag + lp_step) + emd);
+  const float su = (float)((up + lp_stay) + emd);
+  const float sl = (float)(left + lp_skip);
+  u = su >= sd;
   const float m1 = u ? su : sd;
-  const bool l = sl >= m1;
-  score = l ? sl : m1;
-  from = l ? FROM_L : (u ? FROM_U : FROM_D);
+  l = sl >= m1;
+  return l ? sl : m1;
+}
+
+// lane `lane` of v takes the wave-uniform value x (v_writelane_b32)
+extern "C" __device__ int abea_llvm_writelane(int x, int lane, int v) __asm(
+    "llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ float put_lane(float v, float x, int lane) {
+  return __builtin_bit_cast(float, abea_llvm_writelane(__builtin_bit_cast(int, x), lane,
+                                                       __builtin_bit_cast(int, v)));
+}
+
+__device__ __forceinline__ double rotate_up_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x13C, 0xF, 0xF, true);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x13C, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Shift the four decision bits of one band into each lane's trace word:
+// acc = acc * 16 + (u0 << 3 | l0 << 2 | u1 << 1 | l1), one add-with-carry per
+// bit with the decision mask as carry-in.
+__device__ __forceinline__ uint32_t shift_in4(uint32_t acc, uint64_t u0, uint64_t l0, uint64_t u1,
+                                              uint64_t l1) {
+  uint32_t out;
+  uint64_t carry;
+  asm("v_addc_co_u32 %0, %1, %2, %2, %3\n\t"
+      "v_addc_co_u32 %0, %1, %0, %0, %4\n\t"
+      "v_addc_co_u32 %0, %1, %0, %0, %5\n\t"
+      "v_addc_co_u32 %0, %1, %0, %0, %6"
+      : "=&v"(out), "=&s"(carry)
+      : "v"(acc), "s"(u0), "s"(l0), "s"(u1), "s"(l1));
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -183,188 +219,417 @@ __global__ __launch_bounds__(256) void abea_pre_kernel(KernelArgs a, int q_begin
 
 // ---------------------------------------------------------------------------
 // core: the band walk of one read by one wavefront.
+//
+// Per-lane state: the event and k-mer parameters under the lane's two slots,
+// the scores of the last band (float, and widened to double for the next
+// transition sums) and the diagonal sources one band further back.
+struct Slots {
+  float ev0, ev1;
+  float km0m, km0s, km0r, km0c, km1m, km1s, km1r, km1c;
+  float M0, M1;   // band b-1, parity 0 / 1
+  double D0, D1;  // the same, widened
+  double Lq;      // left neighbour of parity 0 at band b-1 = its diagonal at band b
+  double Dq;      // D0 of band b-2 = diagonal of parity 1 at band b
+  uint32_t acc;   // trace bits of the current 8-band tile
+};
+
+// Wave-uniform walk state.  K and E (lower-left k-mer and event of the last
+// band) are kept as byte offsets of the next prefetch: koff = 16 (K + 128)
+// into the staged k-mer records, eoff = 4 (E + 1) into the staged events.
+struct Walk {
+  uint32_t eoff, koff;
+  float nx_ev, nk_m, nk_s, nk_r, nk_c;  // event E+1 and k-mer K+128
+  uint32_t mv;                          // move bits of the current 32-band word
+};
+
+typedef __attribute__((address_space(4))) const uint8_t c_u8;
+
+__device__ __forceinline__ void prefetch(Walk &w, c_u8 *evb, c_u8 *kmb) {
+  w.nx_ev = *(c_f32 *)(evb + w.eoff);
+  c_f32 *kp = (c_f32 *)(kmb + w.koff);
+  w.nk_m = kp[0];
+  w.nk_s = kp[1];
+  w.nk_r = kp[2];
+  w.nk_c = kp[3];
+}
+
+__device__ __forceinline__ int walk_k(const Walk &w) { return (int)(w.koff >> 4) - 128; }
+__device__ __forceinline__ int walk_e(const Walk &w) { return (int)(w.eoff >> 2) - 1; }
+
+// Events move up one slot per band; slot K_{b-1} (lane h, parity P) takes
+// event E_{b-1}+1 whatever the move: on a down move it is the new bottom
+// cell's event; on a right move that slot leaves the window and the value is
+// exactly the event the next slot needs one band later.  On a right move the
+// same slot becomes the window top, k-mer K_{b-1}+128.
+template <int P>
+__device__ __forceinline__ void shift_slots(Slots &s, const Walk &w, int h, bool right) {
+  const float e_rot = rotate_up(s.ev1);
+  if (P == 0) {
+    s.ev1 = s.ev0;
+    s.ev0 = put_lane(e_rot, w.nx_ev, h);
+  } else {
+    s.ev1 = put_lane(s.ev0, w.nx_ev, h);
+    s.ev0 = e_rot;
+  }
+  if (right) {
+    if (P == 0) {
+      s.km0m = put_lane(s.km0m, w.nk_m, h);
+      s.km0s = put_lane(s.km0s, w.nk_s, h);
+      s.km0r = put_lane(s.km0r, w.nk_r, h);
+      s.km0c = put_lane(s.km0c, w.nk_c, h);
+    } else {
+      s.km1m = put_lane(s.km1m, w.nk_m, h);
+      s.km1s = put_lane(s.km1s, w.nk_s, h);
+      s.km1r = put_lane(s.km1r, w.nk_r, h);
+      s.km1c = put_lane(s.km1c, w.nk_c, h);
+    }
+  }
+}
+
+// The two cells of every lane for band b.
+//   parity 0 (slot 2l):   up = D0, left = slot 2l-1 of b-1 = D1 of lane l-1,
+//                         diag = that left one band earlier (Lq)
+//   parity 1 (slot 2l+1): up = D1, left = D0, diag = D0 of b-2 (Dq)
+struct Cells {
+  float M0, M1;
+  double left0;
+  uint64_t u0, l0, u1, l1;
+};
+__device__ __forceinline__ Cells compute_cells(const Slots &s, double lp_step, double lp_stay,
+                                               double lp_skip) {
+  Cells c;
+  c.left0 = rotate_up_f64(s.D1);
+  const float em0 = emission(s.ev0, s.km0m, s.km0s, s.km0r, s.km0c);
+  const float em1 = emission(s.ev1, s.km1m, s.km1s, s.km1r, s.km1c);
+  bool u0, l0, u1, l1;
+  c.M0 = dp_cell(s.Lq, s.D0, c.left0, em0, lp_step, lp_stay, lp_skip, u0, l0);
+  c.M1 = dp_cell(s.Dq, s.D1, s.D0, em1, lp_step, lp_stay, lp_skip, u1, l1);
+  c.u0 = __builtin_amdgcn_ballot_w64(u0);
+  c.l0 = __builtin_amdgcn_ballot_w64(l0);
+  c.u1 = __builtin_amdgcn_ballot_w64(u1);
+  c.l1 = __builtin_amdgcn_ballot_w64(l1);
+  return c;
+}
+
+__device__ __forceinline__ void commit_cells(Slots &s, const Cells &c, float M0, float M1,
+                                             uint64_t u1, uint64_t l1) {
+  s.acc = shift_in4(s.acc, c.u0, c.l0, u1, l1);
+  s.Lq = c.left0;
+  s.Dq = s.D0;
+  s.M0 = M0;
+  s.M1 = M1;
+  s.D0 = (double)M0;
+  s.D1 = (double)M1;
+}
+
+// Suzuki's rule on band b-1's lower-left (slot K) and upper-right (slot
+// K+99) cells, GPU/align_cuda.cl:398-410: right iff ll < ur; both outside
+// the matrix (-inf) -> right iff b is odd.  With K = 2h + P, slot K is lane h
+// of parity P and slot K+99 is lane h+49+P of the other parity.
+template <int P>
+__device__ __forceinline__ void band_ends(const Slots &s, int h, uint32_t &llb, uint32_t &urb) {
+  llb = lane_bits(P ? s.M1 : s.M0, h);
+  urb = lane_bits(P ? s.M0 : s.M1, (h + 49 + P) & 63);
+}
+
+// One interior band: every window cell is inside the matrix, no trim cell,
+// the last k-mer is not in the window.  The window masks are kept
+// incrementally: with h = K >> 1 (mod 64), A = lanes h..h+49 and
+// B = lanes h+1..h+50; the window is (A, A) for even K and (B, A) for odd K
+// (parity-0 / parity-1 registers).  Returns the move.
+template <int P, bool ODD>
+__device__ __forceinline__ bool band_fast(Slots &s, Walk &w, int &h, uint64_t &A, uint64_t &B,
+                                          uint32_t &gm, uint32_t bit, c_u8 *evb, c_u8 *kmb,
+                                          double lp_step, double lp_stay, double lp_skip) {
+  uint32_t llb, urb;
+  band_ends<P>(s, h, llb, urb);
+  bool right = order_key(llb) < order_key(urb);
+  if (ODD) right = right || (llb == kNegInfBits && urb == kNegInfBits);
+  shift_slots<P>(s, w, h, right);
+  uint64_t mk0;
+  if (right) {
+    w.koff += 16;
+    gm |= bit;
+    if (P == 1) {
+      h = (h + 1) & 63;
+      A = B;
+      B = rotl64(B, 1);
+      mk0 = A;
+    } else {
+      mk0 = B;
+    }
+  } else {
+    w.eoff += 4;
+    mk0 = P ? B : A;
+  }
+  prefetch(w, evb, kmb);
+  const Cells c = compute_cells(s, lp_step, lp_stay, lp_skip);
+  const float M0 = lane_in(mk0) ? c.M0 : ABEA_NEG_INF;
+  const float M1 = lane_in(A) ? c.M1 : ABEA_NEG_INF;
+  commit_cells(s, c, M0, M1, c.u1, c.l1);
+  return right;
+}
+
+struct ReadConst {
+  int n_e, n_k, js;
+  double lp_stay, lp_step, lp_skip, lp_trim;
+  c_u8 *evb;
+  c_u8 *kmb;
+  float *endcol;
+  uint32_t *trace;  // the read's tiles, offset by the lane
+  uint32_t *moves;
+  int32_t *kck;
+  c_u32 *replay;
+};
+
+// Any band: fill range from the matrix bounds (GPU/align_cuda.cl:457-470),
+// the trim cell (e, -1) while K < 0 (GPU/align_cuda.cl:432-443), the end-cell
+// column (k = n_k - 1) for the end search, and replayed moves.
+template <int P>
+__device__ __forceinline__ void band_general(Slots &s, Walk &w, const ReadConst &rc, int b,
+                                             int lane) {
+  int K = walk_k(w), E = walk_e(w);
+  const int h = (K >> 1) & 63;
+  uint32_t llb, urb;
+  band_ends<P>(s, h, llb, urb);
+  const bool both_out = llb == kNegInfBits && urb == kNegInfBits;
+  bool right = both_out ? (b & 1) != 0 : order_key(llb) < order_key(urb);
+  if (rc.replay) right = ((rc.replay[b >> 5] >> (b & 31)) & 1u) != 0;
+  shift_slots<P>(s, w, h, right);
+  if (right) {
+    w.koff += 16;
+    ++K;
+  } else {
+    w.eoff += 4;
+    ++E;
+  }
+  w.mv |= (uint32_t)right << (b & 31);
+  prefetch(w, rc.evb, rc.kmb);
+
+  const Cells c = compute_cells(s, rc.lp_step, rc.lp_stay, rc.lp_skip);
+  const int lo = max(max(-K, E - (rc.n_e - 1)), 0);
+  const int hi = min(min(rc.n_k - K, E + 1), ABEA_BANDWIDTH);
+  const int span = hi - lo;
+  const int first = (K + lo) & 127;
+  float M0 = lane_in(slot_lanes(first, span, 0)) ? c.M0 : ABEA_NEG_INF;
+  float M1 = lane_in(slot_lanes(first, span, 1)) ? c.M1 : ABEA_NEG_INF;
+  const bool trim = K < 0;
+  if (trim) {
+    // the trim cell (e, k=-1) sits in slot 127 (lane 63, parity 1)
+    const int et = E - (-1 - K);
+    const bool in = et >= 0 && et < rc.n_e;
+    const float tv = in ? (float)(rc.lp_trim * (double)(et + 1)) : ABEA_NEG_INF;
+    if (lane == 63) M1 = tv;
+  }
+  const int os = (rc.n_k - 1) - K;
+  if (span > 0 && (uint32_t)(os - lo) < (uint32_t)span) {
+    // every lane stores the same value: no lane-dependent branch in the walk
+    rc.endcol[E - os] = __builtin_bit_cast(float, lane_bits((rc.js & 1) ? M1 : M0, rc.js >> 1));
+  }
+  commit_cells(s, c, M0, M1, c.u1, c.l1);
+  // the trim cell's from-code is U: bits (u1, l1) = (1, 0) in lane 63
+  if (trim && lane == 63) s.acc = (s.acc & ~3u) | 2u;
+  if ((b & 7) == 7) {
+    rc.trace[(b >> 3) * kTileWords] = s.acc;
+    s.acc = 0;
+  }
+  if ((b & 31) == 31) {
+    rc.moves[b >> 5] = w.mv;  // same value from every lane
+    rc.kck[b >> 5] = K;
+    w.mv = 0;
+  }
+}
+
+__device__ __forceinline__ void band_any(Slots &s, Walk &w, const ReadConst &rc, int b, int lane) {
+  if (walk_k(w) & 1)
+    band_general<1>(s, w, rc, b, lane);
+  else
+    band_general<0>(s, w, rc, b, lane);
+}
+
+// Bands b0 .. b0+7 (b0 a multiple of 8) when all of them are interior: a
+// state machine over (band in tile, parity of K) so that every register
+// choice that depends on the parity is fixed in the code.
+__device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc, int b0,
+                                          int lane) {
+  const int K0 = walk_k(w);
+  int h = (K0 >> 1) & 63;
+  uint64_t A = rotl64((1ull << 50) - 1, h);
+  uint64_t B = rotl64(A, 1);
+  uint32_t gm = 0;
+  const double st = rc.lp_step, sy = rc.lp_stay, sk = rc.lp_skip;
+#define ABEA_BAND(I, P, ODD)                                                        \
+  band_##I##_##P:                                                                   \
+  if (band_fast<P, ODD>(s, w, h, A, B, gm, 1u << I, rc.evb, rc.kmb, st, sy, sk))    \
+    goto band_##I##_##P##_r;                                                        \
+  else                                                                              \
+    goto band_##I##_##P##_d;
+  if (K0 & 1)
+    goto band_0_1;
+  else
+    goto band_0_0;
+  ABEA_BAND(0, 0, false)
+  ABEA_BAND(0, 1, false)
+band_0_0_r:
+band_0_1_d:
+  goto band_1_1;
+band_0_0_d:
+band_0_1_r:
+  goto band_1_0;
+  ABEA_BAND(1, 0, true)
+  ABEA_BAND(1, 1, true)
+band_1_0_r:
+band_1_1_d:
+  goto band_2_1;
+band_1_0_d:
+band_1_1_r:
+  goto band_2_0;
+  ABEA_BAND(2, 0, false)
+  ABEA_BAND(2, 1, false)
+band_2_0_r:
+band_2_1_d:
+  goto band_3_1;
+band_2_0_d:
+band_2_1_r:
+  goto band_3_0;
+  ABEA_BAND(3, 0, true)
+  ABEA_BAND(3, 1, true)
+band_3_0_r:
+band_3_1_d:
+  goto band_4_1;
+band_3_0_d:
+band_3_1_r:
+  goto band_4_0;
+  ABEA_BAND(4, 0, false)
+  ABEA_BAND(4, 1, false)
+band_4_0_r:
+band_4_1_d:
+  goto band_5_1;
+band_4_0_d:
+band_4_1_r:
+  goto band_5_0;
+  ABEA_BAND(5, 0, true)
+  ABEA_BAND(5, 1, true)
+band_5_0_r:
+band_5_1_d:
+  goto band_6_1;
+band_5_0_d:
+band_5_1_r:
+  goto band_6_0;
+  ABEA_BAND(6, 0, false)
+  ABEA_BAND(6, 1, false)
+band_6_0_r:
+band_6_1_d:
+  goto band_7_1;
+band_6_0_d:
+band_6_1_r:
+  goto band_7_0;
+  ABEA_BAND(7, 0, true)
+  ABEA_BAND(7, 1, true)
+band_7_0_r:
+band_7_0_d:
+band_7_1_r:
+band_7_1_d:
+#undef ABEA_BAND
+  w.mv |= gm << (b0 & 31);
+  rc.trace[(b0 >> 3) * kTileWords] = s.acc;
+  s.acc = 0;
+  if ((b0 & 31) == 24) {
+    rc.moves[b0 >> 5] = w.mv;  // same value from every lane
+    rc.kck[b0 >> 5] = walk_k(w);
+    w.mv = 0;
+  }
+}
+
 __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int lane) {
+  ReadConst rc;
+  rc.n_e = m.n_e;
+  rc.n_k = m.n_k;
+  rc.js = (m.n_k - 1) & 127;  // slot of the last k-mer (end-cell column)
+  rc.lp_stay = m.lp_stay;
+  rc.lp_step = m.lp_step;
+  rc.lp_skip = a.lp_skip;
+  rc.lp_trim = a.lp_trim;
+  rc.evb = (c_u8 *)(a.ev + m.ev_off);
+  rc.kmb = (c_u8 *)(a.km + m.km_off);
+  rc.endcol = a.endcol + m.ev_off;
+  rc.trace = a.trace + m.trace_off + lane;
+  rc.moves = a.moves + m.mv_off;
+  rc.kck = a.kck + m.mv_off;
+  rc.replay = a.replay ? (c_u32 *)(a.replay + m.mv_off) : nullptr;
   const int n_e = m.n_e, n_k = m.n_k;
   const int n_bands = n_e + n_k + 2;
-  const double lp_stay = m.lp_stay, lp_step = m.lp_step;
-  const double lp_skip = a.lp_skip, lp_trim = a.lp_trim;
   const float *evp = a.ev + m.ev_off;
   const KmerParam *kmp = a.km + m.km_off;
-  c_f32 *evs = (c_f32 *)evp;
-  c_f32 *kms = (c_f32 *)kmp;
-  float *endcol = a.endcol + m.ev_off;
-  uint32_t *tw = a.trace + m.trace_off + lane;
-  uint32_t *moves = a.moves + m.mv_off;
-  int32_t *kck = a.kck + m.mv_off;
-  c_u32 *replay = a.replay ? (c_u32 *)(a.replay + m.mv_off) : nullptr;
 
-  // band 1 lower-left (E, K) = (50, -51), GPU/align_cuda.cl:251-255
-  int E = kHalfW, K = -1 - kHalfW;
-
-  // Event and k-mer parameters under each slot for band 1's window.
-  float ev0, ev1;
-  KmerParam p0{0.f, 1.f, 1.f, 0.f}, p1{0.f, 1.f, 1.f, 0.f};
+  // band 1: lower-left (E, K) = (50, -51), GPU/align_cuda.cl:251-255; slots
+  // hold k-mers K..K+127 and the events on band 1's anti-diagonal
+  const int E = kHalfW, K = -1 - kHalfW;
+  Slots s;
   {
     const int o0 = (2 * lane - K) & 127, o1 = (2 * lane + 1 - K) & 127;
     const int e0 = E - o0, e1 = E - o1;
-    ev0 = (e0 >= 0 && e0 < n_e) ? evp[e0] : 0.f;
-    ev1 = (e1 >= 0 && e1 < n_e) ? evp[e1] : 0.f;
+    s.ev0 = (e0 >= 0 && e0 < n_e) ? evp[e0] : 0.f;
+    s.ev1 = (e1 >= 0 && e1 < n_e) ? evp[e1] : 0.f;
     const int k0 = K + o0, k1 = K + o1;
+    KmerParam p0{0.f, 1.f, 1.f, 0.f}, p1{0.f, 1.f, 1.f, 0.f};
     if (k0 >= 0 && k0 < n_k) p0 = kmp[k0];
     if (k1 >= 0 && k1 < n_k) p1 = kmp[k1];
+    s.km0m = p0.mean;
+    s.km0s = p0.stdv;
+    s.km0r = p0.rstdv;
+    s.km0c = p0.c;
+    s.km1m = p1.mean;
+    s.km1s = p1.stdv;
+    s.km1r = p1.rstdv;
+    s.km1c = p1.c;
   }
-  float km0m = p0.mean, km0s = p0.stdv, km0r = p0.rstdv, km0c = p0.c;
-  float km1m = p1.mean, km1s = p1.stdv, km1r = p1.rstdv, km1c = p1.c;
   // band 1: only the first trim cell (e=0, k=-1 -> slot 127) = lp_trim
-  // (GPU/align_cuda.cl:263-267); band 0: start cell (e=-1,k=-1) = 0, whose
-  // left-shifted copy is the diag source of band 2.
-  float b1_0 = ABEA_NEG_INF, b1_1 = (lane == 63) ? (float)lp_trim : ABEA_NEG_INF;
-  float lv0 = (lane == 0) ? 0.0f : ABEA_NEG_INF, lv1 = ABEA_NEG_INF;
+  // (GPU/align_cuda.cl:263-267); band 0: the start cell (e=-1, k=-1) = 0 in
+  // slot 127, whose left-shifted copy is parity 0's diagonal at band 2
+  s.M0 = ABEA_NEG_INF;
+  s.M1 = (lane == 63) ? (float)rc.lp_trim : ABEA_NEG_INF;
+  s.D0 = (double)s.M0;
+  s.D1 = (double)s.M1;
+  s.Lq = (lane == 0) ? 0.0 : (double)ABEA_NEG_INF;
+  s.Dq = (double)ABEA_NEG_INF;
+  s.acc = 0;
+  Walk w;
+  w.eoff = 4u * (uint32_t)(E + 1);
+  w.koff = 16u * (uint32_t)(K + 128);
+  w.mv = 0;
+  prefetch(w, rc.evb, rc.kmb);
 
-  // lane masks of the full 100-slot window [K, K+99] (parity 0 / 1)
-  uint64_t win0 = slot_lanes(K & 127, ABEA_BANDWIDTH, 0);
-  uint64_t win1 = slot_lanes(K & 127, ABEA_BANDWIDTH, 1);
-
-  // scalar prefetch of what band 2 may need: next event, k-mer K+128
-  // (staged arrays are padded, so these reads stay inside the buffers)
-  float nx_ev = evs[E + 1];
-  float nk_m = kms[4 * (K + 128)], nk_s = kms[4 * (K + 128) + 1],
-        nk_r = kms[4 * (K + 128) + 2], nk_c = kms[4 * (K + 128) + 3];
-
-  const int js = (n_k - 1) & 127;  // slot of the last k-mer (end-cell column)
-  uint32_t mv = 0, acc = 0;
-
-  for (int b = 2; b < n_bands; ++b) {
-    // Suzuki's rule on band b-1's lower-left (slot K) and upper-right (slot
-    // K+99, the other parity) cells, GPU/align_cuda.cl:398-410.
-    const int jl = K & 127, ju = (K + ABEA_BANDWIDTH - 1) & 127;
-    uint32_t llb, urb;
-    if (K & 1) {
-      llb = lane_bits(b1_1, jl >> 1);
-      urb = lane_bits(b1_0, ju >> 1);
+  for (int b = 2; b < n_bands;) {
+    // a whole tile of interior bands takes the fast path: over 8 bands K and
+    // E grow by at most 8
+    const int K0 = walk_k(w), E0 = walk_e(w);
+    const bool fast = (b & 7) == 0 && b + 8 <= n_bands && !rc.replay && K0 >= 0 &&
+                      E0 >= ABEA_BANDWIDTH - 1 && K0 + 8 < n_k - ABEA_BANDWIDTH &&
+                      E0 + 8 <= n_e - 1;
+    if (fast) {
+      tile_fast(s, w, rc, b, lane);
+      b += 8;
     } else {
-      llb = lane_bits(b1_0, jl >> 1);
-      urb = lane_bits(b1_1, ju >> 1);
+      band_any(s, w, rc, b, lane);
+      ++b;
     }
-    const bool both_out = ((llb ^ kNegInfBits) | (urb ^ kNegInfBits)) == 0;
-    uint32_t right = both_out ? ((uint32_t)b & 1u)
-                              : (uint32_t)(order_key(llb) < order_key(urb));
-    if (replay) right = (replay[b >> 5] >> (b & 31)) & 1u;
-    right = (uint32_t)uniform((int)right);
-    mv |= right << (b & 31);
-
-    // Events move up one slot.  Slot K_{b-1} takes event E_{b-1}+1 whatever
-    // the move: on a down move it is the new bottom cell's event; on a right
-    // move that slot leaves the window and the value is exactly the event the
-    // next slot needs one band later.
-    const float e_rot = rotate_up(ev1);
-    const float l_rot = rotate_up(b1_1);
-    const int jb = K & 127;
-    const uint64_t jb_lane = 1ull << (jb >> 1);
-    if (jb & 1) {
-      ev1 = lane_in(jb_lane) ? nx_ev : ev0;
-      ev0 = e_rot;
-    } else {
-      ev1 = ev0;
-      ev0 = lane_in(jb_lane) ? nx_ev : e_rot;
-    }
-    if (right) {
-      // slot jb becomes the new window top, k-mer K+128 (K before the move)
-      const uint64_t upd0 = (jb & 1) ? 0ull : jb_lane, upd1 = (jb & 1) ? jb_lane : 0ull;
-      km0m = lane_in(upd0) ? nk_m : km0m;
-      km0s = lane_in(upd0) ? nk_s : km0s;
-      km0r = lane_in(upd0) ? nk_r : km0r;
-      km0c = lane_in(upd0) ? nk_c : km0c;
-      km1m = lane_in(upd1) ? nk_m : km1m;
-      km1s = lane_in(upd1) ? nk_s : km1s;
-      km1r = lane_in(upd1) ? nk_r : km1r;
-      km1c = lane_in(upd1) ? nk_c : km1c;
-      // the window moves up one slot
-      const uint64_t w0 = rotl64(win1, 1);
-      win1 = win0;
-      win0 = w0;
-    }
-    K += (int)right;
-    E += 1 - (int)right;
-    if ((b & 31) == 31) {
-      if (lane == 0) {
-        moves[b >> 5] = mv;
-        kck[b >> 5] = K;
-      }
-      mv = 0;
-    }
-    nx_ev = evs[E + 1];
-    nk_m = kms[4 * (K + 128)];
-    nk_s = kms[4 * (K + 128) + 1];
-    nk_r = kms[4 * (K + 128) + 2];
-    nk_c = kms[4 * (K + 128) + 3];
-
-    float M0, M1;
-    uint32_t f0, f1;
-    dp_cell(lv0, b1_0, l_rot, emission(ev0, km0m, km0s, km0r, km0c), lp_step, lp_stay, lp_skip,
-            M0, f0);
-    dp_cell(lv1, b1_1, b1_0, emission(ev1, km1m, km1s, km1r, km1c), lp_step, lp_stay, lp_skip,
-            M1, f1);
-
-    // fill range of offsets, GPU/align_cuda.cl:460-469, as lane masks; in
-    // the interior of the matrix it is the whole window
-    const bool full = K >= 0 && E <= n_e - 1 && K <= n_k - ABEA_BANDWIDTH && E >= ABEA_BANDWIDTH - 1;
-    if (full) {
-      M0 = lane_in(win0) ? M0 : ABEA_NEG_INF;
-      M1 = lane_in(win1) ? M1 : ABEA_NEG_INF;
-      if (K == n_k - ABEA_BANDWIDTH) {  // last k-mer at offset 99
-        const float v = (js & 1) ? M1 : M0;
-        if (lane == (js >> 1)) endcol[E - (ABEA_BANDWIDTH - 1)] = v;
-      }
-    } else {
-      const int lo = max(max(-K, E - (n_e - 1)), 0);
-      const int hi = min(min(n_k - K, E + 1), ABEA_BANDWIDTH);
-      const int span = hi - lo;
-      const int first = (K + lo) & 127;
-      M0 = lane_in(slot_lanes(first, span, 0)) ? M0 : ABEA_NEG_INF;
-      M1 = lane_in(slot_lanes(first, span, 1)) ? M1 : ABEA_NEG_INF;
-      if (K < 0) {
-        // trim cell (e, k=-1) sits in slot 127 while K <= -1,
-        // GPU/align_cuda.cl:432-443
-        const int et = E - (-1 - K);
-        const bool in = et >= 0 && et < n_e;
-        const float tv = in ? (float)(lp_trim * (double)(et + 1)) : ABEA_NEG_INF;
-        if (lane == 63) {
-          M1 = tv;
-          f1 = FROM_U;
-        }
-      }
-      // end-cell column (k = n_k - 1) for the end search
-      const int os = (n_k - 1) - K;
-      if (span > 0 && (uint32_t)(os - lo) < (uint32_t)span) {
-        const float v = (js & 1) ? M1 : M0;
-        if (lane == (js >> 1)) endcol[E - os] = v;
-      }
-    }
-    // codes of cells outside the fill range are never read: the traceback
-    // only follows computed cells
-    acc |= (f0 | (f1 << 2)) << (4 * (b & 7));
-    if ((b & 7) == 7) {
-      tw[(b >> 3) * kTileWords] = acc;
-      acc = 0;
-    }
-
-    lv0 = l_rot;
-    lv1 = b1_0;
-    b1_0 = M0;
-    b1_1 = M1;
   }
   const int last = n_bands - 1;
-  if ((last & 7) != 7) tw[(last >> 3) * kTileWords] = acc;
-  if ((last & 31) != 31 && lane == 0) moves[last >> 5] = mv;
+  if ((last & 7) != 7) rc.trace[(last >> 3) * kTileWords] = s.acc << (4 * (7 - (last & 7)));
+  if ((last & 31) != 31) rc.moves[last >> 5] = w.mv;
 }
 
 // ---------------------------------------------------------------------------
 // Traceback.
 
+// From-code of cell (band b, slot j): the band's nibble is bits
+// [4(7 - b%8), +4) of the lane's tile word, (u0 l0 u1 l1) from the top; a set
+// l bit means L, else a set u bit means U, else D.
 __device__ __forceinline__ uint32_t trace_code(const uint32_t *tw, int b, int j) {
   const uint32_t w = tw[(b >> 3) * kTileWords + (j >> 1)];
-  return (w >> (4 * (b & 7) + 2 * (j & 1))) & 3u;
+  const uint32_t bits = (w >> (4 * (7 - (b & 7)) + 2 * (1 - (j & 1)))) & 3u;
+  return (bits & 1u) ? FROM_L : (bits >> 1);
 }
 
 // K (lower-left k-mer) of band b from the move bits and the checkpoints.
