@@ -332,42 +332,55 @@ __device__ __forceinline__ void band_ends(const Slots &s, int h, uint32_t &llb, 
   urb = lane_bits(P ? s.M0 : s.M1, (h + 49 + P) & 63);
 }
 
+// Lane of slot k (k mod 128, two slots per lane).
+__device__ __forceinline__ int slot_lane(int k) { return (k >> 1) & 63; }
+
 // One interior band: every window cell is inside the matrix, no trim cell,
-// the last k-mer is not in the window.  The window masks are kept
-// incrementally: with h = K >> 1 (mod 64), A = lanes h..h+49 and
-// B = lanes h+1..h+50; the window is (A, A) for even K and (B, A) for odd K
-// (parity-0 / parity-1 registers).  Returns the move.
-template <int P, bool ODD>
-__device__ __forceinline__ bool band_fast(Slots &s, Walk &w, int &h, uint64_t &A, uint64_t &B,
-                                          uint32_t &gm, uint32_t bit, c_u8 *evb, c_u8 *kmb,
-                                          double lp_step, double lp_stay, double lp_skip) {
-  uint32_t llb, urb;
-  band_ends<P>(s, h, llb, urb);
-  bool right = order_key(llb) < order_key(urb);
-  if (ODD) right = right || (llb == kNegInfBits && urb == kNegInfBits);
-  shift_slots<P>(s, w, h, right);
-  uint64_t mk0;
-  if (right) {
-    w.koff += 16;
-    gm |= bit;
-    if (P == 1) {
-      h = (h + 1) & 63;
-      A = B;
-      B = rotl64(B, 1);
-      mk0 = A;
-    } else {
-      mk0 = B;
-    }
-  } else {
-    w.eoff += 4;
-    mk0 = P ? B : A;
+// the last k-mer is not in the window.  Branch-free; the scalar state is the
+// lower-left k-mer K, the parity of K only selecting registers and lanes.
+// Masking: an in-window cell reads only in-window cells of the previous two
+// bands or the two slots just outside the window (K-1 and K+100), so those
+// two are set to -inf and the other out-of-window slots are left as they are
+// (never read by an in-window cell).  The k-mer refill of a right move is
+// deferred to the end of the tile (tile_fast).
+template <bool ODD>
+__device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, float &nx_ev, int i,
+                                          uint32_t ebase, c_u8 *evb, double lp_step,
+                                          double lp_stay, double lp_skip) {
+  const int p = K & 1, h = slot_lane(K);
+  // Suzuki's rule (GPU/align_cuda.cl:398-410) as the reference's float
+  // comparison: ll (slot K) against every lane of the other register,
+  // keeping the bit of slot K+99's lane.
+  const uint32_t ll0 = lane_bits(s.M0, h), ll1 = lane_bits(s.M1, h);
+  const uint32_t llb = p ? ll1 : ll0;
+  const float ll = __builtin_bit_cast(float, llb);
+  const float q = p ? s.M0 : s.M1;
+  const int lu = slot_lane(K + 99);
+  uint32_t r = (uint32_t)(__builtin_amdgcn_ballot_w64(ll < q) >> lu) & 1u;
+  if (ODD) {
+    // both ends outside the matrix: right on odd bands
+    const uint32_t ur_out = (uint32_t)(__builtin_amdgcn_ballot_w64(q == ABEA_NEG_INF) >> lu) & 1u;
+    r |= ur_out & (llb == kNegInfBits ? 1u : 0u);
   }
-  prefetch(w, evb, kmb);
+  // Events move up one slot; slot K takes event E+1.  The other register
+  // takes the same write in slot K+101, above the window, whose event value is
+  // never read again (values move up one slot per band, the window at most
+  // one).
+  const float e_rot = rotate_up(s.ev1);
+  const int d = slot_lane(K + 101);
+  s.ev1 = put_lane(s.ev0, nx_ev, p ? h : d);
+  s.ev0 = put_lane(e_rot, nx_ev, p ? d : h);
+  // the move; E + 1 = b - 1 - K
+  K += (int)r;
+  gm |= r << i;
+  nx_ev = *(c_f32 *)(evb + (ebase - 4u * (uint32_t)K));
   const Cells c = compute_cells(s, lp_step, lp_stay, lp_skip);
-  const float M0 = lane_in(mk0) ? c.M0 : ABEA_NEG_INF;
-  const float M1 = lane_in(A) ? c.M1 : ABEA_NEG_INF;
+  const int p2 = K & 1;
+  const int lb = slot_lane(K - 1), lt = slot_lane(K + 100);
+  const float ninf = ABEA_NEG_INF;
+  const float M0 = put_lane(c.M0, ninf, p2 ? lb : lt);
+  const float M1 = put_lane(c.M1, ninf, p2 ? lt : lb);
   commit_cells(s, c, M0, M1, c.u1, c.l1);
-  return right;
 }
 
 struct ReadConst {
@@ -375,6 +388,7 @@ struct ReadConst {
   double lp_stay, lp_step, lp_skip, lp_trim;
   c_u8 *evb;
   c_u8 *kmb;
+  const KmerParam *kmp;  // the same k-mer records, for vector loads
   float *endcol;
   uint32_t *trace;  // the read's tiles, offset by the lane
   uint32_t *moves;
@@ -447,98 +461,50 @@ __device__ __forceinline__ void band_any(Slots &s, Walk &w, const ReadConst &rc,
     band_general<0>(s, w, rc, b, lane);
 }
 
-// Bands b0 .. b0+7 (b0 a multiple of 8) when all of them are interior: a
-// state machine over (band in tile, parity of K) so that every register
-// choice that depends on the parity is fixed in the code.
+// Bands b0 .. b0+7 (b0 a multiple of 8) when all of them are interior.
+// The slots a right move vacates (slot K takes k-mer K+128) are among
+// K0 .. K0+7; their k-mers are loaded when the tile starts and merged when it
+// ends, for the slots actually vacated.  A vacated slot re-enters the window
+// only after 29 more right moves, so the merge is in time.
 __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc, int b0,
                                           int lane) {
   const int K0 = walk_k(w);
-  int h = (K0 >> 1) & 63;
-  uint64_t A = rotl64((1ull << 50) - 1, h);
-  uint64_t B = rotl64(A, 1);
+  int K = K0;
   uint32_t gm = 0;
-  const double st = rc.lp_step, sy = rc.lp_stay, sk = rc.lp_skip;
-#define ABEA_BAND(I, P, ODD)                                                        \
-  band_##I##_##P:                                                                   \
-  if (band_fast<P, ODD>(s, w, h, A, B, gm, 1u << I, rc.evb, rc.kmb, st, sy, sk))    \
-    goto band_##I##_##P##_r;                                                        \
-  else                                                                              \
-    goto band_##I##_##P##_d;
-  if (K0 & 1)
-    goto band_0_1;
-  else
-    goto band_0_0;
-  ABEA_BAND(0, 0, false)
-  ABEA_BAND(0, 1, false)
-band_0_0_r:
-band_0_1_d:
-  goto band_1_1;
-band_0_0_d:
-band_0_1_r:
-  goto band_1_0;
-  ABEA_BAND(1, 0, true)
-  ABEA_BAND(1, 1, true)
-band_1_0_r:
-band_1_1_d:
-  goto band_2_1;
-band_1_0_d:
-band_1_1_r:
-  goto band_2_0;
-  ABEA_BAND(2, 0, false)
-  ABEA_BAND(2, 1, false)
-band_2_0_r:
-band_2_1_d:
-  goto band_3_1;
-band_2_0_d:
-band_2_1_r:
-  goto band_3_0;
-  ABEA_BAND(3, 0, true)
-  ABEA_BAND(3, 1, true)
-band_3_0_r:
-band_3_1_d:
-  goto band_4_1;
-band_3_0_d:
-band_3_1_r:
-  goto band_4_0;
-  ABEA_BAND(4, 0, false)
-  ABEA_BAND(4, 1, false)
-band_4_0_r:
-band_4_1_d:
-  goto band_5_1;
-band_4_0_d:
-band_4_1_r:
-  goto band_5_0;
-  ABEA_BAND(5, 0, true)
-  ABEA_BAND(5, 1, true)
-band_5_0_r:
-band_5_1_d:
-  goto band_6_1;
-band_5_0_d:
-band_5_1_r:
-  goto band_6_0;
-  ABEA_BAND(6, 0, false)
-  ABEA_BAND(6, 1, false)
-band_6_0_r:
-band_6_1_d:
-  goto band_7_1;
-band_6_0_d:
-band_6_1_r:
-  goto band_7_0;
-  ABEA_BAND(7, 0, true)
-  ABEA_BAND(7, 1, true)
-band_7_0_r:
-band_7_0_d:
-band_7_1_r:
-band_7_1_d:
-#undef ABEA_BAND
+  const int off0 = (2 * lane - K0) & 127, off1 = (2 * lane + 1 - K0) & 127;
+  const KmerParam st0 = rc.kmp[K0 + 128 + (off0 < 8 ? off0 : 0)];
+  const KmerParam st1 = rc.kmp[K0 + 128 + (off1 < 8 ? off1 : 0)];
+  float nx_ev = w.nx_ev;
+#pragma unroll
+  for (int i = 0; i < 8; i += 2) {
+    // byte offset of event E+1 after band b0+i is 4 (b0 + i - 1 - K)
+    band_fast<false>(s, K, gm, nx_ev, i, 4u * (uint32_t)(b0 + i - 1), rc.evb, rc.lp_step,
+                     rc.lp_stay, rc.lp_skip);
+    band_fast<true>(s, K, gm, nx_ev, i + 1, 4u * (uint32_t)(b0 + i), rc.evb, rc.lp_step,
+                    rc.lp_stay, rc.lp_skip);
+  }
+  const int n = K - K0;  // right moves in this tile
+  const bool v0 = off0 < n, v1 = off1 < n;
+  s.km0m = v0 ? st0.mean : s.km0m;
+  s.km0s = v0 ? st0.stdv : s.km0s;
+  s.km0r = v0 ? st0.rstdv : s.km0r;
+  s.km0c = v0 ? st0.c : s.km0c;
+  s.km1m = v1 ? st1.mean : s.km1m;
+  s.km1s = v1 ? st1.stdv : s.km1s;
+  s.km1r = v1 ? st1.rstdv : s.km1r;
+  s.km1c = v1 ? st1.c : s.km1c;
+  w.koff = 16u * (uint32_t)(K + 128);
+  w.eoff = 4u * (uint32_t)(b0 + 6 - K);  // E + 1 = b - 1 - K after band b = b0+7
   w.mv |= gm << (b0 & 31);
   rc.trace[(b0 >> 3) * kTileWords] = s.acc;
   s.acc = 0;
   if ((b0 & 31) == 24) {
     rc.moves[b0 >> 5] = w.mv;  // same value from every lane
-    rc.kck[b0 >> 5] = walk_k(w);
+    rc.kck[b0 >> 5] = K;
     w.mv = 0;
   }
+  // the general path expects both prefetches current
+  prefetch(w, rc.evb, rc.kmb);
 }
 
 __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int lane) {
@@ -552,6 +518,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.lp_trim = a.lp_trim;
   rc.evb = (c_u8 *)(a.ev + m.ev_off);
   rc.kmb = (c_u8 *)(a.km + m.km_off);
+  rc.kmp = a.km + m.km_off;
   rc.endcol = a.endcol + m.ev_off;
   rc.trace = a.trace + m.trace_off + lane;
   rc.moves = a.moves + m.mv_off;
