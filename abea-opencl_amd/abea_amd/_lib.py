@@ -68,6 +68,7 @@ EXPORTS = [
     "abea_plan_destroy", "abea_plan_forward", "abea_plan_move_words", "abea_plan_get_moves",
     "abea_plan_set_replay", "abea_plan_adjoint", "abea_plan_get_tuning", "abea_plan_set_tuning",
     "abea_plan_autotune", "abea_plan_last_times", "abea_plan_workspace_bytes",
+    "abea_plan_refine_step", "abea_plan_refine",
 ]
 
 _lib = None
@@ -125,6 +126,11 @@ def lib():
     L.abea_plan_last_times.argtypes = [P, C.POINTER(C.c_double)]
     L.abea_plan_workspace_bytes.restype = C.c_int64
     L.abea_plan_workspace_bytes.argtypes = [P]
+    L.abea_plan_refine_step.restype = C.c_int
+    L.abea_plan_refine_step.argtypes = [P, C.POINTER(OutputT), P, C.c_float]
+    L.abea_plan_refine.restype = C.c_int
+    L.abea_plan_refine.argtypes = [P, C.POINTER(OutputT), C.c_int, C.c_float,
+                                   C.POINTER(C.c_double)]
     _lib = L
     return L
 

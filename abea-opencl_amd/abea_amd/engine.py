@@ -174,10 +174,27 @@ class Plan:
             pass
 
     # ---- runs
+    generation = 0  # forwards run so far (the adjoint needs the paths of the latest)
+
+    def set_params(self, scalings: torch.Tensor | None = None, model: torch.Tensor | None = None):
+        """Overwrite the batch's per-read scalings ([n, 4] f32: scale, shift,
+        var, log_var) and/or pore model ([4096, 3] f32: level_mean,
+        level_stdv, level_log_stdv) in place; the next forward uses them."""
+        n = self.batch.n_reads
+        if scalings is not None:
+            if tuple(scalings.shape) != (n, 4):
+                raise ValueError(f"scalings must be [{n}, 4], got {tuple(scalings.shape)}")
+            self.batch.scalings[:n].copy_(scalings.detach().to(torch.float32))
+        if model is not None:
+            if tuple(model.shape) != (4096, 3):
+                raise ValueError(f"model must be [4096, 3], got {tuple(model.shape)}")
+            self.batch.model.copy_(model.detach().to(torch.float32))
+
     def forward(self):
         self.ctx.enter()
         _lib.check(_lib.lib().abea_plan_forward(self.handle, C.byref(self.out)), "abea_plan_forward")
         self.ctx.leave()
+        self.generation += 1
 
     def adjoint(self, grad_score: torch.Tensor):
         n = self.batch.n_reads
@@ -191,6 +208,28 @@ class Plan:
         self.ctx.leave()
         gs.record_stream(self.ctx.stream)
         return g_scal[:n], g_model
+
+    def refine_step(self, damping: float = 1.0, scalings_out: torch.Tensor | None = None):
+        """One Gauss-Newton step on every read's (scale, shift) along the
+        paths of the last forward; writes the batch's scalings in place
+        unless scalings_out ([n, 4] f32, device) is given."""
+        dst = self.batch.scalings if scalings_out is None else scalings_out
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_refine_step(self.handle, C.byref(self.out), _ptr(dst),
+                                                    float(damping)), "abea_plan_refine_step")
+        self.ctx.leave()
+        return dst
+
+    def refine(self, steps: int, damping: float = 1.0) -> float:
+        """steps x (forward + refine step), graph-replayed, scalings updated
+        in place; returns ms per step."""
+        ms = C.c_double()
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_refine(self.handle, C.byref(self.out), int(steps),
+                                               float(damping), C.byref(ms)), "abea_plan_refine")
+        self.ctx.leave()
+        self.generation += steps
+        return ms.value
 
     def times(self) -> list[float]:
         t = (C.c_double * 4)()

@@ -215,3 +215,14 @@ def workload_lengths(name: str, seed: int = 0, n: int | None = None) -> np.ndarr
 
 def workload(name: str, seed: int = 0, n: int | None = None, with_aos: bool = False) -> Batch:
     return generate(workload_lengths(name, seed, n), seed=seed, with_aos=with_aos)
+
+
+def perturb_scalings(b: Batch, seed: int = 0, shift_pm: float = 5.0, scale_pm: float = 0.10):
+    """The refinement workload's start: each read's shift moved by
+    U(-shift_pm, shift_pm) and its scale by a factor 1 + U(-scale_pm,
+    scale_pm).  Modifies b in place; returns the generating (scale, shift)."""
+    rng = np.random.default_rng(seed + 0x5CA1E)
+    true_scale, true_shift = b.scale.copy(), b.shift.copy()
+    b.scale = (b.scale * (1.0 + rng.uniform(-scale_pm, scale_pm, b.n_reads))).astype(np.float32)
+    b.shift = (b.shift + rng.uniform(-shift_pm, shift_pm, b.n_reads)).astype(np.float32)
+    return true_scale, true_shift

@@ -106,7 +106,7 @@ struct abea_plan {
   DevBuf meta, queue, counter, km, km_rank, ev, endcol, trace, moves, kck, replay, seg;
   bool replay_on = false;
   double lp_skip = 0, lp_trim = 0;
-  abea_tuning_t tune{1, 24, 128, 1, 0};
+  abea_tuning_t tune{1, 24, 128, 1, 1};
   // graph cache
   hipGraphExec_t gexec = nullptr;
   abea_output_t gout{};
@@ -289,6 +289,12 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     p->gexec = nullptr;
   }
   p->tune.use_graph = form == ABEA_PLAN_BATCHED ? 1 : 0;
+  // default residency: every read in flight at once when the batch fits in
+  // 16..32 waves per CU (equal-length batches then finish together)
+  {
+    const int per_cu = (int)((n + p->ctx->n_cu - 1) / std::max(p->ctx->n_cu, 1));
+    p->tune.core_waves_per_cu = std::min(32, std::max(16, per_cu));
+  }
   return ABEA_OK;
 }
 
@@ -523,6 +529,62 @@ abea_status_t abea_plan_adjoint(abea_plan_t *p, const abea_output_t *fwd, const 
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, p->evt[4], p->evt[3]));
   p->times[3] = ms;
+  p->have_times = false;
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_refine_step(abea_plan_t *p, const abea_output_t *fwd,
+                                    abea_scalings_t *scalings_out, float damping) {
+  if (!p || !fwd || !scalings_out) return fail(ABEA_ERR_INVALID, "refine_step: null argument");
+  if (!fwd->pairs || !fwd->path_from || !fwd->path_len)
+    return fail(ABEA_ERR_INVALID, "refine_step: the forward must record pairs, path_from and path_len");
+  if (p->n == 0) return ABEA_OK;
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const KernelArgs a = make_args(p, fwd);
+  HIP_TRY(abea::launch_refine(a, scalings_out, damping, p->ctx->stream));
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_refine(abea_plan_t *p, const abea_output_t *out, int steps, float damping,
+                               double *ms_per_step) {
+  if (!p || !out) return fail(ABEA_ERR_INVALID, "refine: null argument");
+  if (!out->pairs || !out->path_from || !out->path_len || !out->n_pairs)
+    return fail(ABEA_ERR_INVALID, "refine: out must record pairs, path_from, path_len, n_pairs");
+  if (steps < 0) return fail(ABEA_ERR_INVALID, "refine: negative step count");
+  if (ms_per_step) *ms_per_step = 0.0;
+  if (p->n == 0 || steps == 0) return ABEA_OK;
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  abea_status_t st = ensure_events(p);
+  if (st) return st;
+  hipStream_t s = p->ctx->stream;
+  // the batch's own scalings are updated in place; the next forward reads them
+  abea_scalings_t *scal = const_cast<abea_scalings_t *>(p->batch.scalings);
+  const KernelArgs a = make_args(p, out);
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  abea_status_t cst = enqueue_forward(p, a, false);
+  hipError_t le = cst ? hipSuccess : abea::launch_refine(a, scal, damping, s);
+  hipError_t ce = hipStreamEndCapture(s, &g);
+  if (cst || le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return cst ? cst
+               : fail(ABEA_ERR_HIP, "refine: graph capture failed: %s",
+                      hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) return fail(ABEA_ERR_HIP, "refine: hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+  hipError_t e = hipEventRecord(p->evt[4], s);
+  for (int i = 0; i < steps && e == hipSuccess; ++i) e = hipGraphLaunch(ge, s);
+  if (e == hipSuccess) e = hipEventRecord(p->evt[3], s);
+  if (e == hipSuccess) e = hipEventSynchronize(p->evt[3]);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, p->evt[4], p->evt[3]);
+  (void)hipGraphExecDestroy(ge);
+  if (e != hipSuccess) return fail(ABEA_ERR_HIP, "refine: %s", hipGetErrorString(e));
+  if (ms_per_step) *ms_per_step = ms / steps;
   p->have_times = false;
   return ABEA_OK;
 }

@@ -73,6 +73,8 @@ hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overla
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
                           double *grad_model, hipStream_t s);
 hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t s);
+hipError_t launch_refine(const KernelArgs &a, abea_scalings_t *scal_out, float damping,
+                         hipStream_t s);
 int core_max_waves_per_block();
 
 }  // namespace abea

@@ -928,6 +928,65 @@ __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const f
   }
 }
 
+// refine: one Gauss-Newton step on each read's (scale, shift) along the path
+// of the last forward.  With m = scale*mu + shift and a = (x - m)/sigma,
+// every D or U cell contributes -a^2/2 to the score, so on a fixed path the
+// score is a quadratic in (scale, shift) and one Newton step reaches its
+// maximum: solve [Sum mu^2/s^2, Sum mu/s^2; Sum mu/s^2, Sum 1/s^2] d = g with
+// g the path gradient (the adjoint's per-read terms).  Sums in double.
+__global__ __launch_bounds__(256) void abea_refine_kernel(KernelArgs a, abea_scalings_t *scal_out,
+                                                         float damping, int n_reads) {
+  const int lane = threadIdx.x & 63;
+  const int wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int n_waves = (gridDim.x * blockDim.x) >> 6;
+  for (int r = wave; r < n_reads; r += n_waves) {
+    const ReadMeta m = load_meta(a.meta, r);
+    const int P = (m.n_k > 0 && m.n_e > 0) ? a.out.path_len[r] : 0;
+    const abea_scalings_t sc = a.scalings[r];
+    const double scale = (double)sc.scale, shift = (double)sc.shift;
+    const abea_aligned_pair_t *pairs = a.out.pairs + m.pair_off;
+    const uint8_t *from = a.out.path_from + m.pair_off;
+    const float *evp = a.ev + m.ev_off;
+    const uint16_t *kr = a.km_rank + m.km_off;
+    double gs = 0.0, gc = 0.0, hss = 0.0, hsc = 0.0, hcc = 0.0;
+    for (int i = lane; i < P; i += 64) {
+      if (from[i] == FROM_L) continue;
+      const abea_aligned_pair_t pr = pairs[i];
+      const int kk = min(max(pr.ref_pos, 0), m.n_k - 1);
+      const int ee = min(max(pr.read_pos, 0), m.n_e - 1);
+      const abea_model_t md = a.model[kr[kk] & (ABEA_NUM_KMER - 1)];
+      const double mu = (double)md.level_mean;
+      const double inv = 1.0 / (double)md.level_stdv;
+      const double z = ((double)evp[ee] - (scale * mu + shift)) * inv;
+      const double i2 = inv * inv;
+      gc += z * inv;
+      gs += z * mu * inv;
+      hcc += i2;
+      hsc += mu * i2;
+      hss += mu * mu * i2;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      gs += __shfl_xor(gs, off);
+      gc += __shfl_xor(gc, off);
+      hss += __shfl_xor(hss, off);
+      hsc += __shfl_xor(hsc, off);
+      hcc += __shfl_xor(hcc, off);
+    }
+    if (lane == 0) {
+      abea_scalings_t o = sc;
+      const double det = hss * hcc - hsc * hsc;
+      if (P > 0 && det > 0.0) {
+        const double d_scale = (hcc * gs - hsc * gc) / det;
+        const double d_shift = (hss * gc - hsc * gs) / det;
+        o.scale = (float)(scale + (double)damping * d_scale);
+        o.shift = (float)(shift + (double)damping * d_shift);
+      }
+      scal_out[r] = o;
+    }
+  }
+}
+
 __global__ void abea_zero_f64_kernel(double *p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -986,6 +1045,15 @@ hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *g
   const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
   hipLaunchKernelGGL(abea_adjoint_kernel, dim3(grid), dim3(256), 0, s, a, grad_score,
                      grad_scalings, grad_model, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine(const KernelArgs &a, abea_scalings_t *scal_out, float damping,
+                         hipStream_t s) {
+  const int n = a.n_queue;
+  if (n <= 0) return hipSuccess;
+  const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
+  hipLaunchKernelGGL(abea_refine_kernel, dim3(grid), dim3(256), 0, s, a, scal_out, damping, n);
   return hipGetLastError();
 }
 

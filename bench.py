@@ -8,7 +8,9 @@ records.  A "step" is one full forward over that batch.
 
 Also reported (same JSON line): the plain per-read-launch form and the batched
 form on a sample of the 4,096-read log-normal (median 5 kb) workload; the
-adjoint/forward time ratio on 1,024 x 10 kb reads; the CPU baseline (the
+adjoint/forward time ratio on 1,024 x 10 kb reads; the scaling-refinement loop
+on 8,192 x 20 kb reads (five graph-replayed forward + Gauss-Newton steps from
+perturbed scalings, with the recovered-scaling error); the CPU baseline (the
 oracle, OpenMP over reads on this host) on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
@@ -26,10 +28,11 @@ sys.path.insert(0, os.path.join(ROOT, "abea-opencl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 W = 100
-# HBM bytes per launch of the core kernel on this workload, from rocprofv3
-# FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings (the
-# profiles/ summary for the batched 16,384-read workload gives the runs).
-TRAFFIC_BYTES = None
+# HBM bytes per launch of the core kernel on the headline workload, from
+# rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings
+# (profiles/abea_kernels_batched16384_ln8kb.md gives the runs).  Known for
+# the fused single-wave-workgroup variant the autotuner picks here.
+TRAFFIC_BYTES = {"abea_core_kernel<1, true>": 34140534728}
 
 
 def log(*a):
@@ -170,7 +173,8 @@ def main():
         "qc_pass_reads": qc_pass,
         "roofline": {"bound": "hbm", "kernel": kernel_name(tune), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_BYTES.get(kernel_name(tune)),
                      "alg_bytes_per_launch": core_bytes},
     }
     del plan, db
@@ -225,6 +229,39 @@ def main():
                           "adjoint_over_forward": round(s_adj / s_fwd, 4)}
         log(f"[bench] adjoint: {out['adjoint']}")
         del p3, d3
+
+        # scaling refinement: 8,192 x 20 kb reads, scalings perturbed (shift
+        # +-5, scale +-10%), five graph-replayed forward + Gauss-Newton steps
+        b4 = synth.workload("refine8192x20kb", seed=args.seed + 4)
+        true_scale, true_shift = synth.perturb_scalings(b4, seed=args.seed + 4)
+        d4 = engine.DeviceBatch.upload(b4, 0)
+        p4 = engine.Plan(ctx, d4, PLAN_BATCHED)
+        p4.forward()
+        p4.autotune(reps=1)
+        p4.set_tuning(use_graph=1)
+        torch.cuda.synchronize()
+        s0 = d4.scalings.cpu().numpy()
+        steps = 5
+        t = time.perf_counter()
+        ms = p4.refine(steps, 1.0)
+        wall = (time.perf_counter() - t) / steps
+        s1 = d4.scalings.cpu().numpy()
+        out["refine"] = {
+            "workload": "refine8192x20kb", "steps": steps, "tuning": p4.tuning(),
+            "ms_per_step": round(ms, 3),
+            "wall_ms_per_step": round(wall * 1e3, 3),
+            "gcups_per_step": round(b4.nominal_cells() / (ms * 1e-3) / 1e9, 2),
+            "shift_abs_err_before": {"mean": round(float(np.abs(s0[:, 1] - true_shift).mean()), 4),
+                                     "median": round(float(np.median(np.abs(s0[:, 1] - true_shift))), 4)},
+            "shift_abs_err_after": {"mean": round(float(np.abs(s1[:, 1] - true_shift).mean()), 4),
+                                    "median": round(float(np.median(np.abs(s1[:, 1] - true_shift))), 4)},
+            "scale_abs_err_before": {"mean": round(float(np.abs(s0[:, 0] - true_scale).mean()), 5),
+                                     "median": round(float(np.median(np.abs(s0[:, 0] - true_scale))), 5)},
+            "scale_abs_err_after": {"mean": round(float(np.abs(s1[:, 0] - true_scale).mean()), 5),
+                                    "median": round(float(np.median(np.abs(s1[:, 0] - true_scale))), 5)}}
+        log(f"[bench] refine: {out['refine']}")
+        del p4, d4, b4
+        torch.cuda.empty_cache()
 
         # CPU baseline: the oracle (f5c semantics), OpenMP over reads, on a
         # bounded random sample of the headline workload

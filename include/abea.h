@@ -161,8 +161,9 @@ void abea_plan_destroy(abea_plan_t *plan);
 /* Run pre -> core -> post on the context's stream (asynchronous). */
 abea_status_t abea_plan_forward(abea_plan_t *plan, const abea_output_t *out);
 
-/* Record the band moves (1 bit per band) of the last forward in
- * moves[n_words], words laid out per read at abea_plan_move_offset(). */
+/* Record the band moves of the last forward in moves[n_words]: per read, in
+ * read order, ceil(n_bands / 32) words (reads with no k-mers or no events
+ * take none); bit b % 32 of word b / 32 is band b's move (1 = right). */
 int64_t abea_plan_move_words(const abea_plan_t *plan);
 abea_status_t abea_plan_get_moves(abea_plan_t *plan, uint32_t *moves_host);
 /* Hold the band fixed: later forwards replay these moves instead of
@@ -199,6 +200,23 @@ abea_status_t abea_plan_set_tuning(abea_plan_t *plan, const abea_tuning_t *t);
  * the fastest; best_ms receives its time. */
 abea_status_t abea_plan_autotune(abea_plan_t *plan, const abea_output_t *out, int reps,
                                  double *best_ms);
+
+/* Scaling refinement (new; no reference counterpart: f5c re-estimates
+ * scalings by least squares after alignment, recalibrate_model(),
+ * GPU/f5cmisc.h:53-54).  One Gauss-Newton step on each read's (scale,
+ * shift) along the path of the last forward on this plan: on a fixed path
+ * the score is quadratic in (scale, shift), so the step moves to that
+ * path's optimum (times damping).  scalings_out (device, n_reads records)
+ * receives the new scalings (var, log_var copied); it may be the batch's
+ * own scalings array, updated in place.  Needs path_from and path_len in
+ * fwd.  Asynchronous on the context stream. */
+abea_status_t abea_plan_refine_step(abea_plan_t *plan, const abea_output_t *fwd,
+                                    abea_scalings_t *scalings_out, float damping);
+/* steps x (forward + refine step) with the batch's scalings updated in
+ * place, replayed as one captured graph per step; returns when done.
+ * ms_per_step (optional) receives the average device time of one step. */
+abea_status_t abea_plan_refine(abea_plan_t *plan, const abea_output_t *out, int steps,
+                               float damping, double *ms_per_step);
 
 /* Per-kernel device time (ms) of the last forward/adjoint, hipEvent-timed
  * on the plan's stream: [0]=pre [1]=core [2]=post [3]=adjoint. */

@@ -54,6 +54,43 @@ int main(int argc, char **argv) {
     fclose(f);
     for (size_t i = 0; i < k / 3; ++i) bad += check(model[3 * i + 1], exps, 0, 2000, &tested);
   }
+  /* The emission tail: the kernel computes fma(RN(a*a), -0.5, c) for the
+   * reference's c + RN(RN(-0.5a) * a) (GPU/align_cuda.cl:528).  Equal
+   * whenever 0.5a^2 is a normal float (halving commutes with rounding), and
+   * for smaller |a| unless c is exactly 0. */
+  long tail_tested = 0, tail_bad = 0;
+  for (long i = 0; i < 20000000; ++i) {
+    uint64_t r = next();
+    /* |a| in [2^-60, 2^60), either sign */
+    float a = bits_f((uint32_t)((r >> 63) << 31) | ((67u + (uint32_t)((r >> 40) % 120)) << 23) |
+                     (uint32_t)(r & 0x7fffff));
+    uint64_t r2 = next();
+    /* c in [-2^5, 2^5) excluding the exact zero, plus values near 0 */
+    float c = bits_f((uint32_t)((r2 >> 63) << 31) | ((100u + (uint32_t)((r2 >> 40) % 32)) << 23) |
+                     (uint32_t)(r2 & 0x7fffff));
+    volatile float h = -0.5f * a;
+    volatile float t = h * a;
+    volatile float ref = c + t;
+    float got = fmaf(a * a, -0.5f, c);
+    if (f_bits(got) != f_bits(ref)) {
+      if (tail_bad < 3) printf("tail mismatch a=%a c=%a\n", a, c);
+      ++tail_bad;
+    }
+    ++tail_tested;
+  }
+  const float specials[] = {0.0f, -0.0f, INFINITY, -INFINITY, 1e-30f, 3.5f};
+  for (unsigned i = 0; i < 6; ++i)
+    for (unsigned j = 0; j < 6; ++j) {
+      float a = specials[i], c = specials[j] == 0.0f ? -1.75f : -specials[j];
+      volatile float h = -0.5f * a;
+      volatile float t = h * a;
+      volatile float ref = c + t;
+      float got = fmaf(a * a, -0.5f, c);
+      if (f_bits(got) != f_bits(ref) && !(isnan(got) && isnan(ref))) ++tail_bad;
+      ++tail_tested;
+    }
+  printf("tested %ld emission tails, %ld tail mismatches\n", tail_tested, tail_bad);
+  bad += tail_bad;
   printf("tested %ld quotients, %ld mismatches\n", tested, bad);
   return bad != 0;
 }

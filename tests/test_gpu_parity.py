@@ -278,3 +278,105 @@ def test_adjoint_matches_fp64_path_gradient(ctx):
     np.testing.assert_allclose(g_model[:, 0], ref_mu, rtol=1e-9, atol=1e-9)
     np.testing.assert_allclose(g_model[:, 1], ref_sg, rtol=1e-9, atol=1e-9)
     plan.close()
+
+
+def test_autograd_score_and_gradient(ctx):
+    """torch.autograd over abea_score: the forward equals the plain forward
+    and the oracle's score; the backward equals the FP64 path gradient
+    mapped onto the f5c struct columns."""
+    import oracle
+    from abea_amd import engine, synth
+    from abea_amd.autograd import abea_score
+    b = synth.workload("cpu64x2kb", seed=43, n=10)
+    db = engine.DeviceBatch.upload(b, 0)
+    plan = engine.Plan(ctx, db, 1)
+    scal = torch.tensor(b.scalings(), device="cuda", requires_grad=True)
+    model = torch.tensor(b.model.astype(np.float32), device="cuda", requires_grad=True)
+    w = torch.linspace(0.5, 1.5, b.n_reads, device="cuda")
+    score = abea_score(plan, scal, model)
+    (score * w).sum().backward()
+    res = plan.results()
+    wn = w.cpu().numpy().astype(np.float64)
+    gs = scal.grad.cpu().numpy()
+    gm = model.grad.cpu().numpy()
+    ref_mu = np.zeros(4096)
+    ref_sg = np.zeros(4096)
+    for i in range(b.n_reads):
+        r = oracle.align(b.read_seq(i), b.read_events(i), b.model, float(b.scale[i]),
+                         float(b.shift[i]))
+        assert np.float32(r["score"]) == score[i].item()
+        pairs, frm = read_path(res, i)
+        gr, g_mu, g_sg = oracle.path_gradient(pairs, frm, b.read_seq(i), b.read_events(i),
+                                              b.model, b.scale[i], b.shift[i], weight=wn[i])
+        np.testing.assert_allclose(gs[i, 0], gr[1], rtol=1e-5, atol=1e-3)  # scale
+        np.testing.assert_allclose(gs[i, 1], gr[0], rtol=1e-5, atol=1e-3)  # shift
+        assert gs[i, 2] == 0 and gs[i, 3] == 0
+        ref_mu += g_mu
+        ref_sg += g_sg
+    np.testing.assert_allclose(gm[:, 0], ref_mu, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(gm[:, 1], ref_sg, rtol=1e-6, atol=1e-6)
+    assert np.all(gm[:, 2] == 0)
+    # a second forward on the plan invalidates the first one's backward
+    s1 = abea_score(plan, scal, model)
+    plan.forward()
+    with pytest.raises(RuntimeError):
+        s1.sum().backward()
+    plan.close()
+
+
+def _newton_reference(b, res, i):
+    """FP64 Gauss-Newton step on (scale, shift) along read i's traced path."""
+    pairs, frm = read_path(res, i)
+    code = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+    seq = b.read_seq(i)
+    ev = b.read_events(i).astype(np.float64)
+    scale, shift = float(b.scale[i]), float(b.shift[i])
+    g = np.zeros(2)
+    H = np.zeros((2, 2))
+    for (k, e), f in zip(pairs, frm):
+        if f == 2:
+            continue
+        r = 0
+        for c in seq[k:k + 6]:
+            r = (r << 2) | code.get(c, 0)
+        mu, sd = float(b.model[r, 0]), float(b.model[r, 1])
+        z = (ev[e] - (scale * mu + shift)) / sd
+        g += np.array([z * mu / sd, z / sd])
+        H += np.array([[mu * mu, mu], [mu, 1.0]]) / (sd * sd)
+    d = np.linalg.solve(H, g)
+    return scale + d[0], shift + d[1]
+
+
+def test_refine_step_and_loop(ctx):
+    """One refine step equals an FP64 Gauss-Newton step along each traced
+    path; a few graph-replayed steps bring perturbed scalings back toward
+    the generating ones."""
+    from abea_amd import engine, synth
+    b = synth.workload("cpu64x2kb", seed=47, n=16)
+    true_scale, true_shift = b.scale.copy(), b.shift.copy()
+    rng = np.random.default_rng(47)
+    b.scale = (b.scale * (1 + rng.uniform(-0.1, 0.1, b.n_reads))).astype(np.float32)
+    b.shift = (b.shift + rng.uniform(-5, 5, b.n_reads)).astype(np.float32)
+    db = engine.DeviceBatch.upload(b, 0)
+    plan = engine.Plan(ctx, db, 1)
+    plan.forward()
+    res = plan.results()
+    out = torch.empty((b.n_reads, 4), dtype=torch.float32, device="cuda")
+    plan.refine_step(1.0, out)
+    got = out.cpu().numpy()
+    for i in range(b.n_reads):
+        s_ref, h_ref = _newton_reference(b, res, i)
+        np.testing.assert_allclose(got[i, 0], s_ref, rtol=1e-6)
+        np.testing.assert_allclose(got[i, 1], h_ref, rtol=1e-5, atol=1e-5)
+        assert got[i, 2] == b.var[i] and got[i, 3] == b.log_var[i]
+    # most reads converge within a few steps; a read whose first path is
+    # misaligned can drift, so the medians are compared
+    err0 = np.median(np.abs(b.shift - true_shift))
+    serr0 = np.median(np.abs(b.scale - true_scale))
+    ms = plan.refine(4, 1.0)
+    final = plan.batch.scalings.cpu().numpy()
+    err1 = np.median(np.abs(final[:, 1] - true_shift))
+    serr1 = np.median(np.abs(final[:, 0] - true_scale))
+    assert ms > 0 and err1 < 0.35 * err0, (err0, err1)
+    assert serr1 < 0.5 * serr0, (serr0, serr1)
+    plan.close()

@@ -362,3 +362,75 @@ def test_gradient_vs_fd_band_fixed(h):
         assert smooth == len(params)
     else:
         assert smooth >= 1
+
+
+# --------------------------------------------------------------------- QC
+def _emission_f32(x, mu, sd, lsd, scale, shift):
+    """f5c's emission in float32 steps (GPU/align_cuda.cl:511-528)."""
+    f = np.float32
+    m = f(f(scale) * f(mu)) + f(shift)
+    a = (f(x) - m) / f(sd)
+    return (f(-0.918938) - f(lsd)) + (f(-0.5) * a) * a
+
+
+def _rank(s):
+    code = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+    r = 0
+    for c in s:
+        r = (r << 2) | code.get(c, 0)
+    return r
+
+
+def _qc_stats(seq, ev, model, scale, shift, path, frm):
+    """QC statistics of a traced path, restated from the reference
+    (GPU/align_cuda.cl:755-882): double sum of float emissions over every
+    traced cell in traceback order, longest run of L moves, span."""
+    total = 0.0
+    for k, e in path[::-1]:
+        mu, sd, lsd = model[_rank(seq[k:k + 6])]
+        total += float(_emission_f32(ev[e], mu, sd, lsd, scale, shift))
+    run = gap = 0
+    for f in frm[::-1]:
+        run = run + 1 if f == 2 else 0
+        gap = max(gap, run)
+    n_k = len(seq) - 5
+    spanned = path[0][0] == 0 and path[-1][0] == n_k - 1
+    return total / len(path), gap, spanned
+
+
+def test_qc_rules_against_path_statistics():
+    """The oracle's QC decision (avg log-emission >= -5, spanned, max gap <=
+    50; GPU/align_cuda.cl:134,139,873) equals the rule applied to statistics
+    recomputed from its path, and the average and gap rules both pass and
+    fail somewhere in the set.  (A traced path is always spanned: it starts
+    at the last k-mer, and the only finite predecessors of a cell on event 0
+    lead to k-mer 0 or to the trim column, so the span test only guards
+    against malformed paths.)"""
+    b, _, _ = golden()
+    rng = np.random.default_rng(23)
+    cases = [(b.read_seq(i), b.read_events(i), b.model, float(b.scale[i]), float(b.shift[i]))
+             for i in range(4)]
+    from abea_amd import synth
+    s = synth.workload("cpu64x2kb", seed=5, n=4)
+    cases += [(s.read_seq(i), s.read_events(i), s.model, float(s.scale[i]), float(s.shift[i]))
+              for i in range(4)]
+    # few events for many k-mers: long runs of L moves (gaps > 50)
+    for n_k, n_e in [(40, 6), (300, 5), (60, 12), (250, 3)]:
+        sq = rng.choice(np.frombuffer(b"ACGT", np.uint8), n_k + 5).tobytes()
+        cases.append((sq, rng.uniform(60, 130, n_e).astype(np.float32), b.model, 1.0, 0.0))
+    fired = {"avg": set(), "gap": set(), "span": set()}
+    for seq, ev, model, scale, shift in cases:
+        r = oracle.align(seq, ev, model, scale, shift)
+        path, frm = r["path"], r["from"]
+        assert len(path) == r["path_len"] > 0
+        avg, gap, spanned = _qc_stats(seq, ev, model, scale, shift, path, frm)
+        assert avg == r["avg_log_emission"]
+        assert gap == r["max_gap"] and bool(spanned) == bool(r["spanned"])
+        ok = avg >= -5.0 and spanned and gap <= 50
+        assert bool(r["qc_pass"]) == ok
+        assert r["n_pairs"] == (len(path) if ok else 0)
+        fired["avg"].add(avg < -5.0)
+        fired["gap"].add(gap > 50)
+        fired["span"].add(not spanned)
+    assert fired["avg"] == {True, False} and fired["gap"] == {True, False}, fired
+    assert fired["span"] == {False}
