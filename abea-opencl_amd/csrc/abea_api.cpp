@@ -103,7 +103,7 @@ struct abea_plan {
   std::vector<int32_t> order_h;
   int64_t total_pairs = 0, trace_words = 0, move_words = 0, km_count = 0, ev_count = 0;
   int64_t n_bands_total = 0, seg_bytes = 0, tail_pad = 0;
-  DevBuf meta, queue, counter, km, km_rank, ev, endcol, trace, moves, kck, replay, seg;
+  DevBuf meta, queue, counter, km, km_rank, ev, ends, trace, moves, kck, replay, seg;
   bool replay_on = false;
   double lp_skip = 0, lp_trim = 0;
   abea_tuning_t tune{1, 24, 128, 1, 1};
@@ -221,8 +221,10 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
       km += n_k;
       tr += (bands + abea::kTileBands - 1) / abea::kTileBands * abea::kTileWords;
       mv += (bands + 31) / 32;
-      // 64 traceback segments of S = round4(ceil(rows / 64)) rows each
-      sg += (bands + 256 + 3) & ~(int64_t)3;
+      // traceback segments: one code byte per row they cover; the last
+      // segment ends at the path's end row, so the codes stay below
+      // rows + 4 bytes (slack kept)
+      sg += (bands + 1024 + 3) & ~(int64_t)3;
       nb += bands;
       maxlen = std::max<int64_t>(maxlen, (int64_t)n_e + n_k);
       transition_constants(n_e, n_k, &m.lp_stay, &m.lp_step);
@@ -257,9 +259,9 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
   const int64_t kmc = km + p->tail_pad, evc = evmax + p->tail_pad;
-  const size_t need = (size_t)tr * 4 + (size_t)kmc * (sizeof(KmerParam) + 2) + (size_t)evc * 8 +
-                      (size_t)mv * 12 + (size_t)sg + (size_t)n * (sizeof(ReadMeta) + 4);
-  if (free_b && need > free_b + p->trace.n + p->km.n + p->km_rank.n + p->ev.n + p->endcol.n +
+  const size_t need = (size_t)tr * 4 + (size_t)kmc * (sizeof(KmerParam) + 2) + (size_t)evc * 4 +
+                      (size_t)mv * 12 + (size_t)sg + (size_t)n * (sizeof(ReadMeta) + 12);
+  if (free_b && need > free_b + p->trace.n + p->km.n + p->km_rank.n + p->ev.n + p->ends.n +
                            p->moves.n + p->kck.n + p->replay.n + p->seg.n + p->meta.n)
     return fail(ABEA_ERR_NOMEM, "plan: workspace of %.2f GB exceeds free device memory (%.2f GB)",
                 need / 1e9, free_b / 1e9);
@@ -271,7 +273,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   if ((st = p->km.ensure(sizeof(KmerParam) * (size_t)kmc))) return st;
   if ((st = p->km_rank.ensure(sizeof(uint16_t) * (size_t)kmc))) return st;
   if ((st = p->ev.ensure(sizeof(float) * (size_t)evc))) return st;
-  if ((st = p->endcol.ensure(sizeof(float) * (size_t)std::max<int64_t>(evmax, 1)))) return st;
+  if ((st = p->ends.ensure(sizeof(abea::EndCell) * (size_t)std::max(n, 1)))) return st;
   if ((st = p->trace.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(tr, 1)))) return st;
   if ((st = p->moves.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mv, 1)))) return st;
   if ((st = p->kck.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(mv, 1)))) return st;
@@ -313,7 +315,7 @@ KernelArgs make_args(const abea_plan *p, const abea_output_t *out) {
   a.km = p->km.as<KmerParam>();
   a.km_rank = p->km_rank.as<uint16_t>();
   a.ev = p->ev.as<float>();
-  a.endcol = p->endcol.as<float>();
+  a.ends = p->ends.as<abea::EndCell>();
   a.trace = p->trace.as<uint32_t>();
   a.moves = p->moves.as<uint32_t>();
   a.kck = p->kck.as<int32_t>();
@@ -480,7 +482,7 @@ abea_status_t abea_plan_last_times(const abea_plan_t *pc, double times_ms[4]) {
 int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
   if (!p) return 0;
   return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->km_rank.n + p->ev.n +
-                   p->endcol.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n);
+                   p->ends.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n);
 }
 
 int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }

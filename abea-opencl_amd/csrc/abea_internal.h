@@ -37,6 +37,12 @@ struct alignas(16) KmerParam {
   float c;      // -0.918938f - level_log_stdv
 };
 
+// Result of the end search over the last k-mer's column (per read).
+struct EndCell {
+  float score;  // best (float) end score, -inf when no end cell is in the band
+  int32_t e;    // its event (the first maximum)
+};
+
 struct KernelArgs {
   const ReadMeta *meta;
   const int32_t *queue;      // read indices in processing order
@@ -51,7 +57,7 @@ struct KernelArgs {
   KmerParam *km;             // staged k-mer parameters
   uint16_t *km_rank;         // 6-mer rank per staged k-mer (model-gradient scatter)
   float *ev;                 // staged event means (SoA)
-  float *endcol;             // band cell (e, n_k - 1) per event
+  EndCell *ends;             // per read: best end score and its event (the end search)
   uint32_t *trace;           // 2-bit from-codes in 8-band tiles
   uint32_t *moves;           // 1 bit per band (1 = right)
   int32_t *kck;              // K after every 32nd band (band 32w+31), per move word

@@ -209,11 +209,8 @@ __global__ __launch_bounds__(256) void abea_pre_kernel(KernelArgs a, int q_begin
       kr[k] = (uint16_t)rank;
     }
     float *ev = a.ev + m.ev_off;
-    float *endcol = a.endcol + m.ev_off;
-    for (int e = threadIdx.x; e < m.n_e; e += blockDim.x) {
+    for (int e = threadIdx.x; e < m.n_e; e += blockDim.x)
       ev[e] = a.events ? a.events[m.ev_off + e].mean : a.ev_mean_in[m.ev_off + e];
-      endcol[e] = ABEA_NEG_INF;
-    }
   }
 }
 
@@ -240,6 +237,8 @@ struct Walk {
   uint32_t eoff, koff;
   float nx_ev, nk_m, nk_s, nk_r, nk_c;  // event E+1 and k-mer K+128
   uint32_t mv;                          // move bits of the current 32-band word
+  float best;                           // end search so far: best end score
+  int best_e;                           // and its event
 };
 
 typedef __attribute__((address_space(4))) const uint8_t c_u8;
@@ -389,7 +388,6 @@ struct ReadConst {
   c_u8 *evb;
   c_u8 *kmb;
   const KmerParam *kmp;  // the same k-mer records, for vector loads
-  float *endcol;
   uint32_t *trace;  // the read's tiles, offset by the lane
   uint32_t *moves;
   int32_t *kck;
@@ -435,10 +433,19 @@ __device__ __forceinline__ void band_general(Slots &s, Walk &w, const ReadConst 
     const float tv = in ? (float)(rc.lp_trim * (double)(et + 1)) : ABEA_NEG_INF;
     if (lane == 63) M1 = tv;
   }
+  // End search (GPU/align_cuda.cl:722-746), folded into the walk: band b
+  // holds column cell (e = b - n_k - 1, n_k - 1) when it is in the fill
+  // range, so the cells come in increasing e and a running maximum with a
+  // strict > keeps the reference's first maximum.
   const int os = (rc.n_k - 1) - K;
   if (span > 0 && (uint32_t)(os - lo) < (uint32_t)span) {
-    // every lane stores the same value: no lane-dependent branch in the walk
-    rc.endcol[E - os] = __builtin_bit_cast(float, lane_bits((rc.js & 1) ? M1 : M0, rc.js >> 1));
+    const float v = __builtin_bit_cast(float, lane_bits((rc.js & 1) ? M1 : M0, rc.js >> 1));
+    const int e = E - os;
+    const float sc = (float)((double)v + (double)(rc.n_e - e) * rc.lp_trim);
+    if (sc > w.best) {
+      w.best = sc;
+      w.best_e = e;
+    }
   }
   commit_cells(s, c, M0, M1, c.u1, c.l1);
   // the trim cell's from-code is U: bits (u1, l1) = (1, 0) in lane 63
@@ -507,7 +514,8 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
   prefetch(w, rc.evb, rc.kmb);
 }
 
-__device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int lane) {
+__device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int r,
+                                          int lane) {
   ReadConst rc;
   rc.n_e = m.n_e;
   rc.n_k = m.n_k;
@@ -519,7 +527,6 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.evb = (c_u8 *)(a.ev + m.ev_off);
   rc.kmb = (c_u8 *)(a.km + m.km_off);
   rc.kmp = a.km + m.km_off;
-  rc.endcol = a.endcol + m.ev_off;
   rc.trace = a.trace + m.trace_off + lane;
   rc.moves = a.moves + m.mv_off;
   rc.kck = a.kck + m.mv_off;
@@ -565,6 +572,8 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   w.eoff = 4u * (uint32_t)(E + 1);
   w.koff = 16u * (uint32_t)(K + 128);
   w.mv = 0;
+  w.best = ABEA_NEG_INF;
+  w.best_e = 0;
   prefetch(w, rc.evb, rc.kmb);
 
   for (int b = 2; b < n_bands;) {
@@ -585,6 +594,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   const int last = n_bands - 1;
   if ((last & 7) != 7) rc.trace[(last >> 3) * kTileWords] = s.acc << (4 * (7 - (last & 7)));
   if ((last & 31) != 31) rc.moves[last >> 5] = w.mv;
+  a.ends[r] = EndCell{w.best, w.best_e};  // same value from every lane
 }
 
 // ---------------------------------------------------------------------------
@@ -608,45 +618,103 @@ __device__ __forceinline__ int band_k(const uint32_t *mv, const int32_t *kck, in
   return base + __builtin_popcount(mv[w] & mask);
 }
 
-// Follow from-codes from cell (k, e) down to the first cell below row lo.
-// Records the first cell at or below row hi (the segment's entry), the codes
-// of the cells with lo <= row <= hi (one byte each, traceback order), and the
-// first cell below row lo (the exit; (-1, -1) when the path ends first).
-__device__ __forceinline__ void chase_segment(const uint32_t *tw, int k, int e, int lo, int hi,
-                                              uint32_t *codes, int &ent_k, int &ent_e, int &cnt,
-                                              int &ex_k, int &ex_e) {
-  ent_k = -1;
-  ent_e = -1;
-  ex_k = -1;
-  ex_e = -1;
-  cnt = 0;
-  bool have_ent = false;
-  uint32_t word = 0;
-  while (k >= 0 && e >= 0) {
-    const int b = e + k + 2;
-    if (!have_ent && b <= hi) {
-      have_ent = true;
-      ent_k = k;
-      ent_e = e;
-    }
-    if (b < lo) {
-      ex_k = k;
-      ex_e = e;
-      break;
-    }
-    const uint32_t f = trace_code(tw, b, k & 127);
-    if (b <= hi) {
-      word |= f << (8 * (cnt & 3));
-      if ((cnt & 3) == 3) {
-        codes[cnt >> 2] = word;
-        word = 0;
-      }
-      ++cnt;
-    }
-    k -= (f != FROM_U) ? 1 : 0;
-    e -= (f != FROM_L) ? 1 : 0;
+// Traceback segments: up to kChains per lane, 64 lanes.
+constexpr int kChains = 4;
+constexpr int kMaxSeg = 64 * kChains;
+
+// Per-wave LDS record of a read's traceback segments: where the path
+// following each segment's codes entered it (first cell at or below its top
+// row), how many cells it recorded, and where it left (first cell below its
+// bottom row; (-1, -1) when the path ended inside).
+struct SegInfo {
+  int ent_k[kMaxSeg], ent_e[kMaxSeg], cnt[kMaxSeg], ex_k[kMaxSeg], ex_e[kMaxSeg];
+};
+
+// Follow the from-codes of C segments at once: chain c starts at cell
+// (k[c], e[c]) and records the codes of cells in rows [lo, hi] of segment
+// sid[c] (one byte each, traceback order) at seg + sid*S, and its entry,
+// count and exit in si.  The chains are independent, so their trace loads
+// overlap; each keeps the last 8-byte trace chunk it read (4 slots of one
+// 8-band tile), which the next steps usually reuse.
+template <int C>
+__device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, int S, int b_end,
+                                             const int (&sid)[C], int (&k)[C], int (&e)[C],
+                                             SegInfo *si) {
+  int lo[C], hi[C], cnt[C], ent_k[C], ent_e[C], ex_k[C], ex_e[C], ct[C], cq[C];
+  uint32_t word[C];
+  uint2 cw[C];
+  bool act[C], have[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    act[c] = sid[c] >= 0;
+    lo[c] = 2 + sid[c] * S;
+    hi[c] = min(lo[c] + S - 1, b_end);
+    cnt[c] = 0;
+    ent_k[c] = ent_e[c] = ex_k[c] = ex_e[c] = -1;
+    ct[c] = cq[c] = -1;
+    word[c] = 0;
+    cw[c] = make_uint2(0, 0);
+    have[c] = false;
   }
-  if (cnt & 3) codes[cnt >> 2] = word;
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < C; ++c) any = any || act[c];
+    if (__builtin_amdgcn_ballot_w64(any) == 0) break;
+    int bb[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      bb[c] = e[c] + k[c] + 2;
+      if (act[c]) {
+        if (!have[c] && bb[c] <= hi[c]) {
+          have[c] = true;
+          ent_k[c] = k[c];
+          ent_e[c] = e[c];
+        }
+        if (bb[c] < lo[c]) {
+          ex_k[c] = k[c];
+          ex_e[c] = e[c];
+          act[c] = false;
+        }
+      }
+      const int t = bb[c] >> 3, q = (k[c] & 127) >> 2;
+      if (act[c] && (t != ct[c] || q != cq[c])) {
+        ct[c] = t;
+        cq[c] = q;
+        cw[c] = *(const uint2 *)(tw + t * kTileWords + 2 * q);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (act[c]) {
+        const uint32_t wd = (k[c] & 2) ? cw[c].y : cw[c].x;
+        const uint32_t bits = (wd >> (4 * (7 - (bb[c] & 7)) + 2 * (1 - (k[c] & 1)))) & 3u;
+        const uint32_t f = (bits & 1u) ? FROM_L : (bits >> 1);
+        if (bb[c] <= hi[c]) {
+          word[c] |= f << (8 * (cnt[c] & 3));
+          if ((cnt[c] & 3) == 3) {
+            ((uint32_t *)(seg + sid[c] * S))[cnt[c] >> 2] = word[c];
+            word[c] = 0;
+          }
+          ++cnt[c];
+        }
+        k[c] -= (f != FROM_U) ? 1 : 0;
+        e[c] -= (f != FROM_L) ? 1 : 0;
+        if (k[c] < 0 || e[c] < 0) act[c] = false;  // the path ended: exit stays (-1, -1)
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    if (sid[c] >= 0) {
+      if (cnt[c] & 3) ((uint32_t *)(seg + sid[c] * S))[cnt[c] >> 2] = word[c];
+      si->ent_k[sid[c]] = ent_k[c];
+      si->ent_e[sid[c]] = ent_e[c];
+      si->cnt[sid[c]] = cnt[c];
+      si->ex_k[sid[c]] = ex_k[c];
+      si->ex_e[sid[c]] = ex_e[c];
+    }
+  }
 }
 
 // Traceback of one read by one wavefront (the body of the post kernel, and
@@ -668,29 +736,18 @@ __device__ __forceinline__ void chase_segment(const uint32_t *tw, int k, int e, 
 //     reference's (traceback) order, in double; max gap from ballots.
 //  4. QC (GPU/align_cuda.cl:856-882).
 __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, int lane,
-                               int overlap, float *em_buf) {
+                               int overlap, float *em_buf, SegInfo *si) {
   const int n_e = m.n_e, n_k = m.n_k;
   const abea_output_t &o = a.out;
+  // the end search ran with the band walk (band_general)
   float best = ABEA_NEG_INF;
-  int best_e = 0x7fffffff;
+  int best_e = 0;
   if (n_k > 0 && n_e > 0) {
-    const float *endcol = a.endcol + m.ev_off;
-    for (int e = lane; e < n_e; e += 64) {
-      const float s = (float)((double)endcol[e] + (double)(n_e - e) * a.lp_trim);
-      if (s > best) {
-        best = s;
-        best_e = e;
-      }
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const float ob = __shfl_xor(best, off);
-      const int oe = __shfl_xor(best_e, off);
-      if (ob > best || (ob == best && oe < best_e)) {
-        best = ob;
-        best_e = oe;
-      }
-    }
+    // a vector load: in the fused kernel this wave has just stored it, and the
+    // scalar cache does not see vector stores
+    const EndCell ec = a.ends[r];
+    best = __builtin_bit_cast(float, uniform(__builtin_bit_cast(int, ec.score)));
+    best_e = uniform(ec.e);
   }
   if (n_k <= 0 || n_e <= 0 || best == ABEA_NEG_INF) {
     // empty read, or the band never reached the last k-mer (the reference
@@ -710,119 +767,186 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
   uint8_t *seg = a.seg + m.seg_off;
   const int e_end = uniform(best_e), k_end = n_k - 1;
   const int b_end = e_end + k_end + 2;
-  // rows 2..b_end in 64 segments of S rows (S a multiple of 4: word-aligned
-  // code storage per segment)
+  // rows 2..b_end in up to 256 segments of S rows (S a multiple of 4:
+  // word-aligned code storage per segment), kChains consecutive ones per lane
   const int rows = b_end - 1;
-  const int S = (((rows + 63) >> 6) + 3) & ~3;
+  const int S = max(4, (((rows + kMaxSeg - 1) / kMaxSeg) + 3) & ~3);
   const int nseg = (rows + S - 1) / S;
 
-  // ---- phase A: every lane follows its own segment
-  int ent_k = -1, ent_e = -1, cnt = 0, ex_k = -1, ex_e = -1;
-  const int lo = 2 + lane * S;
-  const int hi = min(lo + S - 1, b_end);
-  uint32_t *codes = (uint32_t *)(seg + lane * S);
-  if (lane < nseg) {
-    int k = k_end, e = e_end;  // exact start: the end cell itself
-    const int s = hi + overlap;
-    if (s < b_end) {
-      const int Ks = band_k(mvw, kck, s), Es = s - 2 - Ks;
-      const int slo = max(max(-Ks, Es - (n_e - 1)), 0);
-      const int shi = min(min(n_k - Ks, Es + 1), ABEA_BANDWIDTH);
-      if (shi > slo) {
-        const int off = min(max(kHalfW, slo), shi - 1);
-        k = Ks + off;
-        e = Es - off;
+  // ---- phase A: every lane follows its segments, starting overlap rows
+  // above each (from the band centre there), or at the end cell
+  {
+    int sid[kChains], k[kChains], e[kChains];
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) {
+      const int i = lane * kChains + c;
+      sid[c] = i < nseg ? i : -1;
+      k[c] = k_end;
+      e[c] = e_end;
+      const int hi = min(2 + i * S + S - 1, b_end);
+      const int s = hi + overlap;
+      if (i < nseg && s < b_end) {
+        const int Ks = band_k(mvw, kck, s), Es = s - 2 - Ks;
+        const int slo = max(max(-Ks, Es - (n_e - 1)), 0);
+        const int shi = min(min(n_k - Ks, Es + 1), ABEA_BANDWIDTH);
+        if (shi > slo) {
+          const int off = min(max(kHalfW, slo), shi - 1);
+          k[c] = Ks + off;
+          e[c] = Es - off;
+        }
       }
     }
-    chase_segment(tw, k, e, lo, hi, codes, ent_k, ent_e, cnt, ex_k, ex_e);
+    chase_chains<kChains>(tw, seg, S, b_end, sid, k, e, si);
   }
+  wave_sync();
 
   // ---- phase B: accept segments top down, following again where needed
   int tk = k_end, te = e_end;
   int P = 0;
   for (int i = nseg - 1; i >= 0; --i) {
     if (tk < 0) {  // the path ended above this segment
-      if (lane == i) cnt = 0;
+      si->cnt[i] = 0;
       continue;
     }
-    const int ik = __builtin_amdgcn_readlane(ent_k, i), ie = __builtin_amdgcn_readlane(ent_e, i);
-    if (ik != tk || ie != te) {
-      if (lane == i) chase_segment(tw, tk, te, lo, hi, codes, ent_k, ent_e, cnt, ex_k, ex_e);
+    if (si->ent_k[i] != tk || si->ent_e[i] != te) {
+      if (lane == 0) {
+        int sid1[1] = {i}, k1[1] = {tk}, e1[1] = {te};
+        chase_chains<1>(tw, seg, S, b_end, sid1, k1, e1, si);
+      }
+      wave_sync();
     }
-    P += __builtin_amdgcn_readlane(cnt, i);
-    tk = __builtin_amdgcn_readlane(ex_k, i);
-    te = __builtin_amdgcn_readlane(ex_e, i);
+    P += si->cnt[i];
+    tk = si->ex_k[i];
+    te = si->ex_e[i];
   }
-  wave_sync();  // segment codes are visible to the whole wave
+  wave_sync();  // segment codes and counts are visible to the whole wave
 
-  // ---- phase C: ascending pairs, QC statistics
+  // ---- phase C: ascending pairs, QC statistics.  Codes are taken 256 at a
+  // time (4 runs of 64 consecutive ones) so that their gathers overlap.
   const float *evp = a.ev + m.ev_off;
   const KmerParam *kmp = a.km + m.km_off;
   abea_aligned_pair_t *pairs = o.pairs + m.pair_off;
   uint8_t *from_out = o.path_from ? o.path_from + m.pair_off : nullptr;
   int ck = k_end, ce = e_end;  // position of the next pair in traceback order
+  // The reference adds the emissions one by one, in double, in traceback
+  // order.  While every partial sum is exact the order is immaterial, and a
+  // wave reduction gives the same bits: that holds when each term is 0 or
+  // at least 2^-7 in magnitude (a multiple of 2^-30) and the partial sums
+  // stay below 2^23.  Otherwise the terms are added serially, in order.
   double sum_em = 0.0;
+  bool exact = true;
   int first_k = k_end, run = 0, max_gap = 0;
   int t_base = 0;  // traceback index of the current segment's first code
   for (int i = nseg - 1; i >= 0; --i) {
-    const int ci = __builtin_amdgcn_readlane(cnt, i);
+    const int ci = si->cnt[i];
     const uint8_t *sc = seg + i * S;
-    for (int u0 = 0; u0 < ci; u0 += 64) {
-      const int u = u0 + lane;
-      const bool act = u < ci;
-      const uint32_t f = act ? sc[u] : FROM_D;
-      // move from ascending pair i-1 to pair i (= inverse of the from code)
-      const int dk = act && f != FROM_U ? 1 : 0;
-      const int de = act && f != FROM_L ? 1 : 0;
-      int pk = dk, pe = de;  // inclusive prefix over lanes
+    for (int u0 = 0; u0 < ci; u0 += 256) {
+      uint32_t f[4];
+      int kk[4], ee[4];
+      bool act[4];
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int xk = __shfl_up(pk, off);
-        const int xe = __shfl_up(pe, off);
-        if (lane >= off) {
-          pk += xk;
-          pe += xe;
-        }
+      for (int j = 0; j < 4; ++j) {
+        const int u = u0 + 64 * j + lane;
+        act[j] = u < ci;
+        f[j] = act[j] ? sc[u] : FROM_D;
       }
-      const int kk = ck - (pk - dk);
-      const int ee = ce - (pe - de);
-      const int nact = min(64, ci - u0);
-      if (act) {
-        const int idx = P - 1 - (t_base + u);
-        pairs[idx].ref_pos = kk;
-        pairs[idx].read_pos = ee;
-        if (from_out) from_out[idx] = (uint8_t)f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        // move from ascending pair i-1 to pair i (= inverse of the from code)
+        const int dk = act[j] && f[j] != FROM_U ? 1 : 0;
+        const int de = act[j] && f[j] != FROM_L ? 1 : 0;
+        int pk = dk, pe = de;  // inclusive prefix over lanes
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int xk = __shfl_up(pk, off);
+          const int xe = __shfl_up(pe, off);
+          if (lane >= off) {
+            pk += xk;
+            pe += xe;
+          }
+        }
+        kk[j] = ck - (pk - dk);
+        ee[j] = ce - (pe - de);
+        ck -= __shfl(pk, 63);
+        ce -= __shfl(pe, 63);
+      }
+      float em[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
         // indices stay inside the read even if the path were malformed
-        const KmerParam p = kmp[min(max(kk, 0), n_k - 1)];
-        em_buf[lane] = emission(evp[min(max(ee, 0), n_e - 1)], p.mean, p.stdv, p.rstdv, p.c);
-      }
-      first_k = __shfl(kk, nact - 1);
-      ck -= __shfl(pk, 63);
-      ce -= __shfl(pe, 63);
-      // max run of consecutive L codes (the reference's gap counter)
-      const uint64_t full = nact == 64 ? ~0ull : ((1ull << nact) - 1);
-      const uint64_t lm = __ballot(act && f == FROM_L) & full;
-      if (lm == full) {
-        run += nact;
-        max_gap = max(max_gap, run);
-      } else {
-        const int lead = __builtin_ctzll(~lm);
-        max_gap = max(max_gap, run + lead);
-        uint64_t x = lm;
-        int rmax = 0;
-        while (x) {
-          x &= x >> 1;
-          ++rmax;
+        const KmerParam p = kmp[min(max(kk[j], 0), n_k - 1)];
+        em[j] = emission(evp[min(max(ee[j], 0), n_e - 1)], p.mean, p.stdv, p.rstdv, p.c);
+        if (act[j]) {
+          const int idx = P - 1 - (t_base + u0 + 64 * j + lane);
+          pairs[idx].ref_pos = kk[j];
+          pairs[idx].read_pos = ee[j];
+          if (from_out) from_out[idx] = (uint8_t)f[j];
+        } else {
+          em[j] = 0.0f;
         }
-        max_gap = max(max_gap, rmax);
-        const uint64_t zeros = ~lm & full;
-        run = nact - 1 - (63 - __builtin_clzll(zeros));
       }
-      wave_sync();
-      if (lane == 0) {
-        for (int q = 0; q < nact; ++q) sum_em += (double)em_buf[q];
+      const int nall = min(256, ci - u0);
+      {
+        const int jl = (nall - 1) >> 6;  // run holding the chunk's last code
+        const int kl = jl == 0 ? kk[0] : jl == 1 ? kk[1] : jl == 2 ? kk[2] : kk[3];
+        first_k = __shfl(kl, (nall - 1) & 63);
       }
-      wave_sync();
+      // max run of consecutive L codes (the reference's gap counter), run by run
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int nact = min(64, nall - 64 * j);
+        if (nact <= 0) break;
+        const uint64_t full = nact == 64 ? ~0ull : ((1ull << nact) - 1);
+        const uint64_t lm = __ballot(act[j] && f[j] == FROM_L) & full;
+        if (lm == full) {
+          run += nact;
+          max_gap = max(max_gap, run);
+        } else {
+          const int lead = __builtin_ctzll(~lm);
+          max_gap = max(max_gap, run + lead);
+          uint64_t x = lm;
+          int rmax = 0;
+          while (x) {
+            x &= x >> 1;
+            ++rmax;
+          }
+          max_gap = max(max_gap, rmax);
+          const uint64_t zeros = ~lm & full;
+          run = nact - 1 - (63 - __builtin_clzll(zeros));
+        }
+      }
+      // emission sum
+      bool grid_ok = true;
+      double part = 0.0, mag = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t bits = __builtin_bit_cast(uint32_t, em[j]);
+        grid_ok = grid_ok && ((bits & 0x7fffffffu) == 0 || ((bits >> 23) & 0xffu) >= 120u);
+        part += (double)em[j];
+        mag += __builtin_fabs((double)em[j]);
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        part += __shfl_xor(part, off);
+        mag += __shfl_xor(mag, off);
+      }
+      exact = exact && __ballot(!grid_ok) == 0 && __builtin_fabs(sum_em) + mag < 8388608.0;
+      if (exact) {
+        sum_em += part;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) em_buf[64 * j + lane] = em[j];
+        wave_sync();
+        double acc = sum_em;
+        if (lane == 0) {
+          for (int q = 0; q < nall; ++q) acc += (double)em_buf[q];
+        }
+        const uint64_t ab = __builtin_bit_cast(uint64_t, acc);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ab);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ab >> 32));
+        sum_em = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+        wave_sync();
+      }
     }
     t_base += ci;
   }
@@ -842,7 +966,8 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
 template <int WPB, bool FUSE>
 __global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q_begin, int q_count,
                                                              int persistent, int overlap) {
-  __shared__ float em_bufs[WPB][64];
+  __shared__ float em_bufs[WPB][256];
+  __shared__ SegInfo seg_infos[FUSE ? WPB : 1];
   const int lane = threadIdx.x & 63;
   const int wib = uniform(threadIdx.x >> 6);
   int qi;
@@ -856,10 +981,10 @@ __global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q
   while (qi < q_count) {
     const int r = ((c_i32 *)a.queue)[q_begin + qi];
     const ReadMeta m = load_meta(a.meta, r);
-    if (m.n_k > 0 && m.n_e > 0) core_read(a, m, lane);
+    if (m.n_k > 0 && m.n_e > 0) core_read(a, m, r, lane);
     if constexpr (FUSE) {
       wave_sync();  // this wave's trace, moves and end column are visible to it
-      traceback_read(a, m, r, lane, overlap, em_bufs[wib]);
+      traceback_read(a, m, r, lane, overlap, em_bufs[wib], &seg_infos[wib]);
     }
     if (!persistent) break;
     int t = 0;
@@ -870,12 +995,13 @@ __global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q
 
 __global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin, int q_count,
                                                       int overlap) {
-  __shared__ float em_buf[64];
+  __shared__ float em_buf[256];
+  __shared__ SegInfo seg_info;
   const int lane = threadIdx.x;
   for (int qi = blockIdx.x; qi < q_count; qi += gridDim.x) {
     const int r = ((c_i32 *)a.queue)[q_begin + qi];
     const ReadMeta m = load_meta(a.meta, r);
-    traceback_read(a, m, r, lane, overlap, em_buf);
+    traceback_read(a, m, r, lane, overlap, em_buf, &seg_info);
   }
 }
 

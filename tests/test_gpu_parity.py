@@ -380,3 +380,41 @@ def test_refine_step_and_loop(ctx):
     assert ms > 0 and err1 < 0.35 * err0, (err0, err1)
     assert serr1 < 0.5 * serr0, (serr0, serr1)
     plan.close()
+
+
+def test_emission_sum_fallback_paths(ctx):
+    """The QC emission sum: reads whose terms fall off the exact grid (tiny
+    emissions) or whose sum grows large take the serial path; both must give
+    the oracle's double sum bit for bit."""
+    from abea_amd import synth
+    base = synth.workload("cpu64x2kb", seed=53, n=2)
+    rng = np.random.default_rng(53)
+    model = base.model.copy()
+    # events on the model means and c = -0.918938 - log_stdv ~ -1e-3: every
+    # D/U emission is about -1e-3, below the 2^-7 grid threshold
+    model[:, 2] = np.float32(np.float32(-0.918938) + np.float32(1e-3))
+    seqs, evs = [], []
+    code = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+    for L in (300, 3000):
+        sq = rng.choice(np.frombuffer(b"ACGT", np.uint8), L).tobytes()
+        ranks = [0] * (L - 5)
+        for k in range(L - 5):
+            r = 0
+            for c in sq[k:k + 6]:
+                r = (r << 2) | code[c]
+            ranks[k] = r
+        seqs.append(sq)
+        evs.append(model[np.array(ranks), 0].astype(np.float32))
+    rl = np.array([len(q) for q in seqs], np.int64)
+    ne = np.array([len(e) for e in evs], np.int64)
+    sp = np.concatenate([[0], np.cumsum(rl)[:-1]])
+    ep = np.concatenate([[0], np.cumsum(ne)[:-1]])
+    n = len(seqs)
+    b = synth.Batch(model, np.frombuffer(b"".join(seqs), np.uint8).copy(), sp, rl,
+                    np.concatenate(evs).astype(np.float32), ep, ne, np.ones(n, np.float32),
+                    np.zeros(n, np.float32), np.ones(n, np.float32), np.zeros(n, np.float32))
+    # plus long noisy reads, whose sums run large and negative
+    big = synth.workload("adjoint1024x10kb", seed=53, n=3)
+    for batch in (b, big):
+        res = run(ctx, batch, 1)
+        assert compare_to_oracle(batch, res) == []
