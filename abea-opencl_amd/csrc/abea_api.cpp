@@ -103,6 +103,7 @@ struct abea_plan {
   std::vector<int32_t> order_h;
   int64_t total_pairs = 0, trace_words = 0, move_words = 0, km_count = 0, ev_count = 0;
   int64_t n_bands_total = 0, seg_bytes = 0, tail_pad = 0;
+  int32_t n_prio = 0;  // leading queue entries run at raised wave priority
   DevBuf meta, queue, counter, km, km_rank, ev, ends, trace, moves, kck, replay, seg;
   bool replay_on = false;
   double lp_skip = 0, lp_trim = 0;
@@ -256,6 +257,17 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     });
   }
 
+  // reads within a factor of two of the longest set the critical path
+  p->n_prio = 0;
+  if (form == ABEA_PLAN_BATCHED && n > 0) {
+    auto bands = [&](int32_t i) {
+      const ReadMeta &m = p->meta_h[i];
+      return (m.n_k > 0 && m.n_e > 0) ? (int64_t)m.n_e + m.n_k + 2 : 0;
+    };
+    const int64_t top = bands(p->order_h[0]);
+    while (p->n_prio < n && 2 * bands(p->order_h[p->n_prio]) >= top && top > 0) ++p->n_prio;
+  }
+
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
   const int64_t kmc = km + p->tail_pad, evc = evmax + p->tail_pad;
@@ -305,6 +317,7 @@ KernelArgs make_args(const abea_plan *p, const abea_output_t *out) {
   a.meta = p->meta.as<ReadMeta>();
   a.queue = p->queue.as<int32_t>();
   a.n_queue = p->n;
+  a.n_prio = p->n_prio;
   a.counter = p->counter.as<int32_t>();
   a.seq = p->batch.seq;
   a.seq_ptr = p->batch.seq_ptr;

@@ -47,6 +47,7 @@ struct KernelArgs {
   const ReadMeta *meta;
   const int32_t *queue;      // read indices in processing order
   int32_t n_queue;
+  int32_t n_prio;            // queue entries [0, n_prio) run at raised wave priority
   int32_t *counter;          // persistent work-queue head
   const char *seq;
   const int64_t *seq_ptr;

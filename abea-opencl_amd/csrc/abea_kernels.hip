@@ -350,10 +350,11 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, float 
   // Suzuki's rule (GPU/align_cuda.cl:398-410) as the reference's float
   // comparison: ll (slot K) against every lane of the other register,
   // keeping the bit of slot K+99's lane.
-  const uint32_t ll0 = lane_bits(s.M0, h), ll1 = lane_bits(s.M1, h);
-  const uint32_t llb = p ? ll1 : ll0;
-  const float ll = __builtin_bit_cast(float, llb);
+  // registers of slot K's parity (lower-left) and of the other parity
+  const float mp = p ? s.M1 : s.M0;
   const float q = p ? s.M0 : s.M1;
+  const uint32_t llb = lane_bits(mp, h);
+  const float ll = __builtin_bit_cast(float, llb);
   const int lu = slot_lane(K + 99);
   uint32_t r = (uint32_t)(__builtin_amdgcn_ballot_w64(ll < q) >> lu) & 1u;
   if (ODD) {
@@ -981,6 +982,13 @@ __global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q
   while (qi < q_count) {
     const int r = ((c_i32 *)a.queue)[q_begin + qi];
     const ReadMeta m = load_meta(a.meta, r);
+    // The longest reads (queued first) set the kernel's critical path: one
+    // wave walks a read band after band.  Their waves take issue priority
+    // over the others on their SIMD.
+    if (q_begin + qi < a.n_prio)
+      __builtin_amdgcn_s_setprio(3);
+    else
+      __builtin_amdgcn_s_setprio(0);
     if (m.n_k > 0 && m.n_e > 0) core_read(a, m, r, lane);
     if constexpr (FUSE) {
       wave_sync();  // this wave's trace, moves and end column are visible to it
