@@ -599,6 +599,69 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
 }
 
 // ---------------------------------------------------------------------------
+// Wave-wide scans and reductions on DPP (row shifts within 16-lane rows, then
+// the row broadcasts of lanes 15 and 31), no LDS round trips.
+
+// inclusive prefix sum over the 64 lanes
+__device__ __forceinline__ int wave_scan_add(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+__device__ __forceinline__ double dpp_f64(double v, int ctrl, int row_mask) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  // the row broadcasts need a per-call immediate, so spell the cases out
+  uint32_t lo, hi;
+#define ABEA_DPP64(C_, R_)                                                            \
+  lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, C_, R_, 0xf, false); \
+  hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), C_, R_, 0xf, false)
+  switch (ctrl) {
+    case 0x111: ABEA_DPP64(0x111, 0xf); break;
+    case 0x112: ABEA_DPP64(0x112, 0xf); break;
+    case 0x114: ABEA_DPP64(0x114, 0xf); break;
+    case 0x118: ABEA_DPP64(0x118, 0xf); break;
+    case 0x142: ABEA_DPP64(0x142, 0xa); break;
+    default: ABEA_DPP64(0x143, 0xc); break;
+  }
+#undef ABEA_DPP64
+  (void)row_mask;
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// sum over the 64 lanes (lane 63 of an inclusive scan), broadcast.  Only for
+// sums whose every partial sum is exact: the order of additions differs
+// from a serial sum.
+__device__ __forceinline__ double wave_sum_exact_f64(double v) {
+  v += dpp_f64(v, 0x111, 0xf);
+  v += dpp_f64(v, 0x112, 0xf);
+  v += dpp_f64(v, 0x114, 0xf);
+  v += dpp_f64(v, 0x118, 0xf);
+  v += dpp_f64(v, 0x142, 0xa);
+  v += dpp_f64(v, 0x143, 0xc);
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// maximum over the 64 lanes of non-negative floats, broadcast
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  int x = __builtin_bit_cast(int, v);  // non-negative floats order as integers
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(x, 63));
+}
+
+// ---------------------------------------------------------------------------
 // Traceback.
 
 // From-code of cell (band b, slot j): the band's nibble is bits
@@ -853,23 +916,15 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        // move from ascending pair i-1 to pair i (= inverse of the from code)
-        const int dk = act[j] && f[j] != FROM_U ? 1 : 0;
-        const int de = act[j] && f[j] != FROM_L ? 1 : 0;
-        int pk = dk, pe = de;  // inclusive prefix over lanes
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const int xk = __shfl_up(pk, off);
-          const int xe = __shfl_up(pe, off);
-          if (lane >= off) {
-            pk += xk;
-            pe += xe;
-          }
-        }
-        kk[j] = ck - (pk - dk);
-        ee[j] = ce - (pe - de);
-        ck -= __shfl(pk, 63);
-        ce -= __shfl(pe, 63);
+        // move from ascending pair i-1 to pair i (= inverse of the from code):
+        // k steps in the low half, e steps in the high half, scanned together
+        const int d = (act[j] && f[j] != FROM_U ? 1 : 0) | (act[j] && f[j] != FROM_L ? 0x10000 : 0);
+        const int pd = wave_scan_add(d) - d;  // exclusive
+        kk[j] = ck - (pd & 0xffff);
+        ee[j] = ce - (pd >> 16);
+        const int tot = __builtin_amdgcn_readlane(pd + d, 63);
+        ck -= tot & 0xffff;
+        ce -= tot >> 16;
       }
       float em[4];
 #pragma unroll
@@ -890,7 +945,7 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
       {
         const int jl = (nall - 1) >> 6;  // run holding the chunk's last code
         const int kl = jl == 0 ? kk[0] : jl == 1 ? kk[1] : jl == 2 ? kk[2] : kk[3];
-        first_k = __shfl(kl, (nall - 1) & 63);
+        first_k = __builtin_amdgcn_readlane(kl, (nall - 1) & 63);
       }
       // max run of consecutive L codes (the reference's gap counter), run by run
 #pragma unroll
@@ -918,22 +973,22 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
       }
       // emission sum
       bool grid_ok = true;
-      double part = 0.0, mag = 0.0;
+      double part = 0.0;
+      float amax = 0.0f;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const uint32_t bits = __builtin_bit_cast(uint32_t, em[j]);
-        grid_ok = grid_ok && ((bits & 0x7fffffffu) == 0 || ((bits >> 23) & 0xffu) >= 120u);
+        const uint32_t ex = (bits >> 23) & 0xffu;  // inf and NaN take the serial path
+        grid_ok = grid_ok && ((bits & 0x7fffffffu) == 0 || (ex >= 120u && ex != 255u));
         part += (double)em[j];
-        mag += __builtin_fabs((double)em[j]);
+        amax = __builtin_fmaxf(amax, __builtin_fabsf(em[j]));
       }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        part += __shfl_xor(part, off);
-        mag += __shfl_xor(mag, off);
-      }
-      exact = exact && __ballot(!grid_ok) == 0 && __builtin_fabs(sum_em) + mag < 8388608.0;
+      // every partial sum is bounded by |sum| + 256 * max |term|
+      const float mag = wave_max_nonneg(amax);
+      exact = exact && __ballot(!grid_ok) == 0 &&
+              __builtin_fabs(sum_em) + 256.0 * (double)mag < 8388608.0;
       if (exact) {
-        sum_em += part;
+        sum_em += wave_sum_exact_f64(part);
       } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j) em_buf[64 * j + lane] = em[j];
