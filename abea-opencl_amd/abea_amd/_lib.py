@@ -55,6 +55,28 @@ class OutputT(C.Structure):
                 ("avg_log_emission", P)]
 
 
+class IndexPairT(C.Structure):  # index_pair_t, GPU/f5c.h:184-188
+    _fields_ = [("start", C.c_int32), ("stop", C.c_int32)]
+
+
+class EventAlignmentT(C.Structure):  # event_alignment_t, GPU/f5c.h:191-207
+    _fields_ = [("ref_kmer", C.c_char * 7), ("ref_position", C.c_int32), ("read_idx", C.c_int32),
+                ("event_idx", C.c_int32), ("rc", C.c_bool), ("model_kmer", C.c_char * 7),
+                ("hmm_state", C.c_char)]
+
+
+assert C.sizeof(EventAlignmentT) == 32
+
+
+class PostalignOutputT(C.Structure):
+    _fields_ = [("alignment", P), ("base_to_event_map", P), ("n_alignment", P),
+                ("events_per_base", P), ("scalings", P), ("read_stat_flag", P)]
+
+
+FAILED_CALIBRATION = 0x001
+FAILED_ALIGNMENT = 0x002
+
+
 class TuningT(C.Structure):
     _fields_ = [("core_waves_per_block", C.c_int32), ("core_waves_per_cu", C.c_int32),
                 ("traceback_overlap", C.c_int32), ("use_graph", C.c_int32),
@@ -68,7 +90,9 @@ EXPORTS = [
     "abea_plan_destroy", "abea_plan_forward", "abea_plan_move_words", "abea_plan_get_moves",
     "abea_plan_set_replay", "abea_plan_adjoint", "abea_plan_get_tuning", "abea_plan_set_tuning",
     "abea_plan_autotune", "abea_plan_last_times", "abea_plan_workspace_bytes",
-    "abea_plan_refine_step", "abea_plan_refine",
+    "abea_plan_refine_step", "abea_plan_refine", "abea_estimate_scalings_using_mom",
+    "abea_postalign", "abea_recalibrate_model", "abea_estimate_scalings_db", "abea_postalign_db",
+    "abea_kmer_offsets", "abea_plan_postalign", "abea_plan_estimate_scalings",
 ]
 
 _lib = None
@@ -131,6 +155,24 @@ def lib():
     L.abea_plan_refine.restype = C.c_int
     L.abea_plan_refine.argtypes = [P, C.POINTER(OutputT), C.c_int, C.c_float,
                                    C.POINTER(C.c_double)]
+    L.abea_estimate_scalings_using_mom.restype = ScalingsT
+    L.abea_estimate_scalings_using_mom.argtypes = [C.c_char_p, C.c_int32, P, EventTable]
+    L.abea_postalign.restype = C.c_int32
+    L.abea_postalign.argtypes = [P, P, C.POINTER(C.c_double), C.c_char_p, C.c_int32, P,
+                                 C.c_int32]
+    L.abea_recalibrate_model.restype = C.c_bool
+    L.abea_recalibrate_model.argtypes = [P, EventTable, C.POINTER(ScalingsT), P, C.c_int32,
+                                         C.c_bool]
+    L.abea_estimate_scalings_db.restype = C.c_int
+    L.abea_estimate_scalings_db.argtypes = [C.c_int32, P, P, P, P, P]
+    L.abea_postalign_db.restype = C.c_int
+    L.abea_postalign_db.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, P, P, P]
+    L.abea_kmer_offsets.restype = None
+    L.abea_kmer_offsets.argtypes = [C.c_int32, P, P]
+    L.abea_plan_postalign.restype = C.c_int
+    L.abea_plan_postalign.argtypes = [P, C.POINTER(OutputT), C.POINTER(PostalignOutputT), C.c_int]
+    L.abea_plan_estimate_scalings.restype = C.c_int
+    L.abea_plan_estimate_scalings.argtypes = [P, P]
     _lib = L
     return L
 

@@ -231,6 +231,52 @@ class Plan:
         self.generation += steps
         return ms.value
 
+    def postalign(self, scale_var: bool = True, records: bool = True, base_map: bool = True,
+                  scalings_out: torch.Tensor | None = None) -> dict:
+        """f5c's flow after align() for every read, from the pairs of the last
+        forward: postalign, recalibrate_model, read status.  Returns the
+        device outputs (records at the pair offsets, map entries at
+        kmer_offsets(); recalibrated scalings in a new [n, 4] tensor unless
+        scalings_out is given, which may be the batch's own scalings)."""
+        n = self.batch.n_reads
+        dev = self.pairs.device
+        if not hasattr(self, "_post"):
+            nk = np.maximum(self.batch.read_len_h.astype(np.int64) - 5, 0)
+            self.kmer_off = np.concatenate([[0], np.cumsum(nk)[:-1]]).astype(np.int64) \
+                if n else np.zeros(0, np.int64)
+            self._post = {
+                "alignment": torch.zeros((max(self.capacity, 1), 32), dtype=torch.uint8,
+                                         device=dev),
+                "base_to_event_map": torch.zeros((max(int(nk.sum()), 1), 2), dtype=torch.int32,
+                                                 device=dev),
+                "n_alignment": torch.zeros(max(n, 1), dtype=torch.int32, device=dev),
+                "events_per_base": torch.zeros(max(n, 1), dtype=torch.float64, device=dev),
+                "read_stat_flag": torch.zeros(max(n, 1), dtype=torch.int32, device=dev),
+            }
+        o = dict(self._post)
+        o["scalings"] = (torch.zeros((max(n, 1), 4), dtype=torch.float32, device=dev)
+                         if scalings_out is None else scalings_out)
+        po = _lib.PostalignOutputT(_ptr(o["alignment"]) if records else None,
+                                   _ptr(o["base_to_event_map"]) if base_map else None,
+                                   _ptr(o["n_alignment"]), _ptr(o["events_per_base"]),
+                                   _ptr(o["scalings"]), _ptr(o["read_stat_flag"]))
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_postalign(self.handle, C.byref(self.out), C.byref(po),
+                                                  int(bool(scale_var))), "abea_plan_postalign")
+        self.ctx.leave()
+        return o
+
+    def estimate_scalings(self, out: torch.Tensor | None = None) -> torch.Tensor:
+        """f5c estimate_scalings_using_mom() for every read ([n, 4] f32)."""
+        n = self.batch.n_reads
+        dst = out if out is not None else torch.zeros((max(n, 1), 4), dtype=torch.float32,
+                                                      device=self.pairs.device)
+        self.ctx.enter()
+        _lib.check(_lib.lib().abea_plan_estimate_scalings(self.handle, _ptr(dst)),
+                   "abea_plan_estimate_scalings")
+        self.ctx.leave()
+        return dst
+
     def times(self) -> list[float]:
         t = (C.c_double * 4)()
         _lib.check(_lib.lib().abea_plan_last_times(self.handle, t), "abea_plan_last_times")
@@ -286,6 +332,32 @@ class Plan:
             "avg": self.avg[:n].cpu().numpy(),
             "pair_off": self.pair_off,
         }
+
+
+# event_alignment_t (GPU/f5c.h:191-207), x86-64 layout, 32 bytes
+EVENT_ALIGNMENT_DTYPE = np.dtype({
+    "names": ["ref_kmer", "ref_position", "read_idx", "event_idx", "rc", "model_kmer",
+              "hmm_state"],
+    "formats": ["S7", "<i4", "<i4", "<i4", "u1", "S7", "S1"],
+    "offsets": [0, 8, 12, 16, 20, 21, 28], "itemsize": 32})
+
+
+def postalign_results(plan: "Plan", o: dict) -> dict:
+    """Host copies of Plan.postalign() outputs.  Read i's records are
+    records[pair_off[i]:pair_off[i] + n_alignment[i]], its map entries
+    base_to_event_map[kmer_off[i]:kmer_off[i] + max(read_len[i] - 5, 0)]."""
+    torch.cuda.synchronize(plan.ctx.device)
+    n = plan.batch.n_reads
+    return {
+        "records": o["alignment"].cpu().numpy().view(EVENT_ALIGNMENT_DTYPE).reshape(-1),
+        "base_to_event_map": o["base_to_event_map"].cpu().numpy(),
+        "n_alignment": o["n_alignment"][:n].cpu().numpy(),
+        "events_per_base": o["events_per_base"][:n].cpu().numpy(),
+        "scalings": o["scalings"][:n].cpu().numpy(),
+        "read_stat_flag": o["read_stat_flag"][:n].cpu().numpy(),
+        "pair_off": plan.pair_off,
+        "kmer_off": plan.kmer_off,
+    }
 
 
 def move_offsets(read_len: np.ndarray, n_events: np.ndarray) -> np.ndarray:

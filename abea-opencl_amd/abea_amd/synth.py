@@ -115,8 +115,15 @@ def kmer_ranks(codes: np.ndarray) -> np.ndarray:
 
 
 def generate(lengths, seed: int = 0, model: np.ndarray | None = None, with_aos: bool = False,
-             chunk_bases: int = 1 << 24) -> Batch:
-    """Generate reads of the given lengths (bases)."""
+             chunk_bases: int = 1 << 24, skip: float = 0.1) -> Batch:
+    """Generate reads of the given lengths (bases).
+
+    Each k-mer emits a geometric(0.55) number of events (mean 1.8), or none
+    with probability `skip` (10% in the benchmark workloads).  At 10% most
+    reads fail f5c's QC on the average log-emission (each skipped k-mer adds
+    a poorly matching cell to the path); skip = 0.015 gives about 0.9% skip
+    pairs on the traced paths, the rate of the real reads in f5c's
+    dump_small, which all pass QC."""
     lengths = np.asarray(lengths, np.int64)
     rng = np.random.default_rng(seed)
     if model is None:
@@ -156,7 +163,7 @@ def generate(lengths, seed: int = 0, model: np.ndarray | None = None, with_aos: 
         ranks = ranks_all[sel]
         nkt = ranks.shape[0]
         cnt = rng.geometric(0.55, nkt).astype(np.int32)
-        cnt[rng.random(nkt) < 0.1] = 0
+        cnt[rng.random(nkt) < skip] = 0
         read_of_kmer = np.repeat(np.arange(start, end), nk)
         ev_rank = np.repeat(ranks, cnt)
         ev_read = np.repeat(read_of_kmer, cnt)
@@ -202,6 +209,9 @@ WORKLOADS = {
     "batched16384_ln8kb": dict(n=16384, median=8000, sigma=0.52, lo=500, hi=200000),
     "adjoint1024x10kb": dict(n=1024, fixed=10000),
     "refine8192x20kb": dict(n=8192, fixed=20000),
+    # reads that pass QC (real-data skip rate), for postalign/recalibration
+    "postalign16384_ln8kb": dict(n=16384, median=8000, sigma=0.52, lo=500, hi=200000,
+                                 skip=0.015),
 }
 
 
@@ -214,7 +224,8 @@ def workload_lengths(name: str, seed: int = 0, n: int | None = None) -> np.ndarr
 
 
 def workload(name: str, seed: int = 0, n: int | None = None, with_aos: bool = False) -> Batch:
-    return generate(workload_lengths(name, seed, n), seed=seed, with_aos=with_aos)
+    return generate(workload_lengths(name, seed, n), seed=seed, with_aos=with_aos,
+                    skip=WORKLOADS[name].get("skip", 0.1))
 
 
 def perturb_scalings(b: Batch, seed: int = 0, shift_pm: float = 5.0, scale_pm: float = 0.10):

@@ -13,8 +13,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libabea.so")
-SOURCES = [os.path.join(HERE, "csrc", "abea_kernels.hip"), os.path.join(HERE, "csrc", "abea_api.cpp")]
-HEADERS = [os.path.join(ROOT, "include", "abea.h"), os.path.join(HERE, "csrc", "abea_internal.h")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in
+           ("abea_kernels.hip", "abea_postalign.hip", "abea_api.cpp")]
+HEADERS = [os.path.join(ROOT, "include", "abea.h"), os.path.join(HERE, "csrc", "abea_internal.h"),
+           os.path.join(HERE, "csrc", "abea_wave.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -102,9 +102,11 @@ struct abea_plan {
   std::vector<ReadMeta> meta_h;
   std::vector<int32_t> order_h;
   int64_t total_pairs = 0, trace_words = 0, move_words = 0, km_count = 0, ev_count = 0;
+  int64_t total_kmers = 0;  // base-to-event map entries (sum of max(n_k, 0))
   int64_t n_bands_total = 0, seg_bytes = 0, tail_pad = 0;
   int32_t n_prio = 0;  // leading queue entries run at raised wave priority
   DevBuf meta, queue, counter, km, km_rank, ev, ends, trace, moves, kck, replay, seg;
+  DevBuf terms;  // recalibration scratch, allocated by the first postalign
   bool replay_on = false;
   double lp_skip = 0, lp_trim = 0;
   abea_tuning_t tune{1, 24, 128, 1, 1};
@@ -201,7 +203,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   p->n = n;
   p->form = form;
   p->meta_h.assign(n, ReadMeta{});
-  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0, sg = 0, maxlen = 0;
+  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0, sg = 0, maxlen = 0, bm = 0;
   for (int32_t i = 0; i < n; ++i) {
     ReadMeta &m = p->meta_h[i];
     const int32_t n_e = b->n_events_host[i];
@@ -214,6 +216,8 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     m.pair_off = pr;
     pr += (int64_t)n_e + std::max<int32_t>(n_k, 0);
     m.km_off = km;
+    m.bmap_off = bm;
+    bm += std::max<int32_t>(n_k, 0);
     m.trace_off = tr;
     m.mv_off = mv;
     m.seg_off = sg;
@@ -232,6 +236,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     }
   }
   p->total_pairs = pr;
+  p->total_kmers = bm;
   p->trace_words = tr;
   p->seg_bytes = sg;
   p->move_words = mv;
@@ -495,7 +500,8 @@ abea_status_t abea_plan_last_times(const abea_plan_t *pc, double times_ms[4]) {
 int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
   if (!p) return 0;
   return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->km_rank.n + p->ev.n +
-                   p->ends.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n);
+                   p->ends.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n +
+                   p->terms.n);
 }
 
 int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }
@@ -557,6 +563,67 @@ abea_status_t abea_plan_refine_step(abea_plan_t *p, const abea_output_t *fwd,
   HIP_TRY(hipSetDevice(p->ctx->device));
   const KernelArgs a = make_args(p, fwd);
   HIP_TRY(abea::launch_refine(a, scalings_out, damping, p->ctx->stream));
+  return ABEA_OK;
+}
+
+void abea_kmer_offsets(int32_t n_reads, const int32_t *read_len_host, int64_t *offsets) {
+  int64_t t = 0;
+  for (int32_t i = 0; i < n_reads; ++i) {
+    offsets[i] = t;
+    t += std::max<int64_t>((int64_t)read_len_host[i] - ABEA_KMER_SIZE + 1, 0);
+  }
+}
+
+abea_status_t abea_plan_postalign(abea_plan_t *p, const abea_output_t *fwd,
+                                  const abea_postalign_output_t *out, int scale_var) {
+  if (!p || !fwd || !out) return fail(ABEA_ERR_INVALID, "postalign: null argument");
+  if (!fwd->pairs || !fwd->n_pairs)
+    return fail(ABEA_ERR_INVALID, "postalign: the forward must record pairs and n_pairs");
+  if (!out->n_alignment || !out->scalings || !out->read_stat_flag)
+    return fail(ABEA_ERR_INVALID, "postalign: n_alignment, scalings and read_stat_flag are required");
+  if (p->n == 0) return ABEA_OK;
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  abea::PostArgs a{};
+  a.n_reads = p->n;
+  a.meta = p->meta.as<ReadMeta>();
+  a.order = p->form == ABEA_PLAN_BATCHED ? p->queue.as<int32_t>() : nullptr;
+  a.seq = p->batch.seq;
+  a.seq_ptr = p->batch.seq_ptr;
+  a.events = p->batch.events;  // records win over means, as in the pre kernel
+  a.ev_mean = p->batch.events ? nullptr : p->batch.ev_mean;
+  a.model = p->batch.model;
+  a.scal_in = p->batch.scalings;
+  a.pairs = fwd->pairs;
+  a.n_pairs = fwd->n_pairs;
+  a.rec = out->alignment;
+  a.bmap = out->base_to_event_map;
+  a.n_rec = out->n_alignment;
+  a.epb = out->events_per_base;
+  a.scal_out = out->scalings;
+  a.flag = out->read_stat_flag;
+  abea_status_t st = p->terms.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(p->total_kmers, 1));
+  if (st) return st;
+  a.terms = p->terms.as<float>();
+  a.recalibrate = 1;
+  a.scale_var = scale_var ? 1 : 0;
+  HIP_TRY(abea::launch_postalign(a, p->ctx->stream));
+  return ABEA_OK;
+}
+
+abea_status_t abea_plan_estimate_scalings(abea_plan_t *p, abea_scalings_t *out) {
+  if (!p || !out) return fail(ABEA_ERR_INVALID, "estimate_scalings: null argument");
+  if (p->n == 0) return ABEA_OK;
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  abea::MomArgs a{};
+  a.n_reads = p->n;
+  a.meta = p->meta.as<ReadMeta>();
+  a.seq = p->batch.seq;
+  a.seq_ptr = p->batch.seq_ptr;
+  a.events = p->batch.events;  // records win over means, as in the pre kernel
+  a.ev_mean = p->batch.events ? nullptr : p->batch.ev_mean;
+  a.model = p->batch.model;
+  a.out = out;
+  HIP_TRY(abea::launch_mom(a, p->ctx->stream));
   return ABEA_OK;
 }
 
@@ -690,6 +757,8 @@ struct HostRun {
   abea_plan *plan = nullptr;
   DevBuf seq, seq_ptr, read_len, events, ev_ptr, n_events, scal, model, pairs, from, n_pairs,
       path_len, score;
+  // post-align and MOM entries
+  DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms;
   ~HostRun() {
     delete plan;
     abea_ctx_destroy(ctx);
@@ -847,6 +916,353 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
         (abea_aligned_pair_t *)malloc(sizeof(abea_aligned_pair_t) * (size_t)std::max(np, 1));
     if (!event_align_pairs[i]) return fail(ABEA_ERR_NOMEM, "abea_align_db: malloc failed");
     if (np) std::memcpy(event_align_pairs[i], pairs.data() + off[i], sizeof(abea_aligned_pair_t) * np);
+  }
+  return ABEA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// f5c's steps either side of align() over host buffers: flatten, upload, one
+// launch, read back.  Per-read metadata for these kernels is built here (no
+// plan: they need no trace or staging workspace).
+namespace {
+
+struct HostPost {
+  int32_t n = 0;
+  std::vector<ReadMeta> meta;
+  std::vector<char> seq;
+  std::vector<int64_t> seq_ptr;
+  std::vector<abea_event_t> ev;
+};
+
+// Reads i = 0..n-1 with read_len[i] bases and n_e[i] events; pair/record
+// slots by pair_cnt[i] (may be null) and map slots by max(read_len - 5, 0).
+void host_post_layout(HostPost &h, int32_t n, char *const *read, const int32_t *read_len,
+                      const abea_event_table *et, const int32_t *pair_cnt) {
+  h.n = n;
+  h.meta.assign(n, ReadMeta{});
+  h.seq_ptr.resize(n);
+  int64_t sl = 0, se = 0, pr = 0, bm = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    ReadMeta &m = h.meta[i];
+    m.n_k = read_len[i] - ABEA_KMER_SIZE + 1;
+    m.n_e = et ? (int32_t)et[i].n : 0x7fffffff;
+    m.ev_off = se;
+    m.pair_off = pr;
+    m.bmap_off = bm;
+    h.seq_ptr[i] = sl;
+    sl += std::max(read_len[i], 0);
+    se += et ? (int64_t)et[i].n : 0;
+    pr += pair_cnt ? std::max(pair_cnt[i], 0) : 0;
+    bm += std::max(m.n_k, 0);
+  }
+  h.seq.resize((size_t)sl);
+  h.ev.resize((size_t)se);
+  for (int32_t i = 0; i < n; ++i) {
+    if (read_len[i] > 0) std::memcpy(h.seq.data() + h.seq_ptr[i], read[i], (size_t)read_len[i]);
+    if (et && et[i].n)
+      std::memcpy(h.ev.data() + h.meta[i].ev_off, et[i].event, sizeof(abea_event_t) * et[i].n);
+  }
+}
+
+abea_status_t host_post_upload(HostRun *r, const HostPost &h, const abea_model_t *model,
+                               hipStream_t s) {
+  abea_status_t st;
+  if ((st = upload(r->pmeta, h.meta.data(), h.meta.size(), s))) return st;
+  if ((st = upload(r->seq, h.seq.data(), h.seq.size(), s))) return st;
+  if ((st = upload(r->seq_ptr, h.seq_ptr.data(), h.seq_ptr.size(), s))) return st;
+  if ((st = upload(r->events, h.ev.data(), h.ev.size(), s))) return st;
+  if (model && (st = upload(r->model, model, (size_t)ABEA_NUM_KMER, s))) return st;
+  return ABEA_OK;
+}
+
+template <class T>
+abea_status_t download(T *h, const DevBuf &d, size_t count, hipStream_t s) {
+  if (count) HIP_TRY(hipMemcpyAsync(h, d.p, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+  return ABEA_OK;
+}
+
+// MOM over n reads into scal_h.
+abea_status_t run_mom(int32_t n, char *const *read, const int32_t *read_len,
+                      const abea_event_table *et, const abea_model_t *model,
+                      abea_scalings_t *scal_h) {
+  HostRun *r = nullptr;
+  abea_status_t st = host_run_get(&r);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(r->ctx->device));
+  hipStream_t s = r->ctx->stream;
+  HostPost h;
+  host_post_layout(h, n, read, read_len, et, nullptr);
+  if ((st = host_post_upload(r, h, model, s))) return st;
+  if ((st = r->scal_out.ensure(sizeof(abea_scalings_t) * (size_t)std::max(n, 1)))) return st;
+  abea::MomArgs a{};
+  a.n_reads = n;
+  a.meta = r->pmeta.as<ReadMeta>();
+  a.seq = r->seq.as<char>();
+  a.seq_ptr = r->seq_ptr.as<int64_t>();
+  a.events = r->events.as<abea_event_t>();
+  a.model = r->model.as<abea_model_t>();
+  a.out = r->scal_out.as<abea_scalings_t>();
+  HIP_TRY(abea::launch_mom(a, s));
+  if ((st = download(scal_h, r->scal_out, (size_t)n, s))) return st;
+  HIP_TRY(hipStreamSynchronize(s));
+  return ABEA_OK;
+}
+
+// postalign (+ recalibration when et and model are given) over n reads whose
+// pairs are given per read.  Results stay in r's buffers; flag_h, n_rec_h,
+// epb_h and scal_h (when recalibrating) are read back.
+abea_status_t run_postalign(HostRun *r, HostPost &h, char *const *read, const int32_t *read_len,
+                            const abea_event_table *et, const abea_model_t *model,
+                            abea_aligned_pair_t *const *pairs, const int32_t *n_pairs,
+                            abea_scalings_t *scal_h, int recalibrate,
+                            std::vector<int32_t> &flag_h, std::vector<int32_t> &n_rec_h,
+                            std::vector<double> &epb_h) {
+  const int32_t n = h.n;
+  hipStream_t s = r->ctx->stream;
+  abea_status_t st;
+  if ((st = host_post_upload(r, h, recalibrate ? model : nullptr, s))) return st;
+  int64_t np_tot = 0, bm_tot = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    np_tot += std::max(n_pairs[i], 0);
+    bm_tot += std::max(h.meta[i].n_k, 0);
+  }
+  std::vector<abea_aligned_pair_t> flat((size_t)np_tot);
+  for (int32_t i = 0; i < n; ++i)
+    if (n_pairs[i] > 0)
+      std::memcpy(flat.data() + h.meta[i].pair_off, pairs[i],
+                  sizeof(abea_aligned_pair_t) * (size_t)n_pairs[i]);
+  if ((st = upload(r->pairs, flat.data(), flat.size(), s))) return st;
+  if ((st = upload(r->n_pairs, n_pairs, (size_t)n, s))) return st;
+  if (recalibrate && (st = upload(r->scal, scal_h, (size_t)n, s))) return st;
+  if (!recalibrate && (st = r->scal.ensure(sizeof(abea_scalings_t) * (size_t)std::max(n, 1))))
+    return st;
+  if ((st = r->rec.ensure(sizeof(abea_event_alignment_t) * (size_t)std::max<int64_t>(np_tot, 1))))
+    return st;
+  if ((st = r->bmap.ensure(sizeof(abea_index_pair_t) * (size_t)std::max<int64_t>(bm_tot, 1))))
+    return st;
+  if ((st = r->n_rec.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
+  if ((st = r->epb.ensure(sizeof(double) * (size_t)std::max(n, 1)))) return st;
+  if ((st = r->flag.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
+  if ((st = r->scal_out.ensure(sizeof(abea_scalings_t) * (size_t)std::max(n, 1)))) return st;
+  if (recalibrate &&
+      (st = r->terms.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(bm_tot, 1))))
+    return st;
+  abea::PostArgs a{};
+  a.n_reads = n;
+  a.meta = r->pmeta.as<ReadMeta>();
+  a.seq = r->seq.as<char>();
+  a.seq_ptr = r->seq_ptr.as<int64_t>();
+  a.events = r->events.as<abea_event_t>();
+  a.model = r->model.as<abea_model_t>();
+  a.scal_in = r->scal.as<abea_scalings_t>();
+  a.pairs = r->pairs.as<abea_aligned_pair_t>();
+  a.n_pairs = r->n_pairs.as<int32_t>();
+  a.rec = r->rec.as<abea_event_alignment_t>();
+  a.bmap = r->bmap.as<abea_index_pair_t>();
+  a.n_rec = r->n_rec.as<int32_t>();
+  a.epb = r->epb.as<double>();
+  a.scal_out = r->scal_out.as<abea_scalings_t>();
+  a.flag = r->flag.as<int32_t>();
+  a.terms = recalibrate ? r->terms.as<float>() : nullptr;
+  a.recalibrate = recalibrate;
+  a.scale_var = 1;
+  HIP_TRY(abea::launch_postalign(a, s));
+  flag_h.resize(n);
+  n_rec_h.resize(n);
+  epb_h.resize(n);
+  if ((st = download(flag_h.data(), r->flag, (size_t)n, s))) return st;
+  if ((st = download(n_rec_h.data(), r->n_rec, (size_t)n, s))) return st;
+  if ((st = download(epb_h.data(), r->epb, (size_t)n, s))) return st;
+  if (recalibrate && (st = download(scal_h, r->scal_out, (size_t)n, s))) return st;
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < n; ++i)
+    if (flag_h[i] & abea::kBadPath)
+      return fail(ABEA_ERR_INVALID,
+                  "postalign: the pairs of read %d are not an alignment path (steps to the "
+                  "next event and/or k-mer, k-mers in [0, %d)%s)",
+                  i, h.meta[i].n_k, recalibrate ? ", events in range" : "");
+  return ABEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+abea_scalings_t abea_estimate_scalings_using_mom(char *sequence, int32_t sequence_len,
+                                                 abea_model_t *pore_model, abea_event_table et) {
+  abea_scalings_t out;
+  out.scale = out.shift = out.var = out.log_var = std::nanf("");
+  if (!sequence || !pore_model || (et.n && !et.event) || sequence_len < 0) {
+    fail(ABEA_ERR_INVALID, "estimate_scalings_using_mom: null argument");
+    return out;
+  }
+  char *rd[1] = {sequence};
+  if (run_mom(1, rd, &sequence_len, &et, pore_model, &out)) out.scale = out.shift = std::nanf("");
+  return out;
+}
+
+abea_status_t abea_estimate_scalings_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
+                                        const abea_event_table *et, const abea_model_t *model,
+                                        abea_scalings_t *scalings) {
+  if (n_bam_rec < 0 || (n_bam_rec > 0 && (!read || !read_len || !et || !model || !scalings)))
+    return fail(ABEA_ERR_INVALID, "estimate_scalings_db: null argument");
+  if (n_bam_rec == 0) return ABEA_OK;
+  return run_mom(n_bam_rec, read, read_len, et, model, scalings);
+}
+
+int32_t abea_postalign(abea_event_alignment_t *alignment, abea_index_pair_t *base_to_event_map,
+                       double *events_per_base, char *sequence, int32_t n_kmers,
+                       abea_aligned_pair_t *event_alignment, int32_t n_events) {
+  if (!alignment || !base_to_event_map || !events_per_base || !sequence || !event_alignment) {
+    fail(ABEA_ERR_INVALID, "postalign: null argument");
+    return -1;
+  }
+  if (n_kmers <= 0 || n_events <= 0) {
+    fail(ABEA_ERR_INVALID, "postalign: needs n_kmers > 0 and n_events > 0 pairs (got %d, %d)",
+         n_kmers, n_events);
+    return -1;
+  }
+  HostRun *r = nullptr;
+  if (host_run_get(&r)) return -1;
+  if (hipSetDevice(r->ctx->device) != hipSuccess) {
+    fail(ABEA_ERR_HIP, "postalign: hipSetDevice failed");
+    return -1;
+  }
+  const int32_t len = n_kmers + ABEA_KMER_SIZE - 1;
+  char *rd[1] = {sequence};
+  HostPost h;
+  host_post_layout(h, 1, rd, &len, nullptr, &n_events);
+  std::vector<int32_t> flag, n_rec;
+  std::vector<double> epb;
+  abea_aligned_pair_t *pp[1] = {event_alignment};
+  if (run_postalign(r, h, rd, &len, nullptr, nullptr, pp, &n_events, nullptr, 0, flag, n_rec,
+                    epb))
+    return -1;
+  hipStream_t s = r->ctx->stream;
+  if (download(alignment, r->rec, (size_t)n_rec[0], s) ||
+      download(base_to_event_map, r->bmap, (size_t)n_kmers, s))
+    return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    fail(ABEA_ERR_HIP, "postalign: stream synchronize failed");
+    return -1;
+  }
+  *events_per_base = epb[0];
+  return n_rec[0];
+}
+
+bool abea_recalibrate_model(abea_model_t *pore_model, abea_event_table et,
+                            abea_scalings_t *scallings,
+                            const abea_event_alignment_t *alignment_output,
+                            int32_t num_alignments, bool scale_var) {
+  if (!pore_model || !scallings || (num_alignments > 0 && !alignment_output) ||
+      (et.n && !et.event) || num_alignments < 0) {
+    fail(ABEA_ERR_INVALID, "recalibrate_model: null argument");
+    return false;
+  }
+  HostRun *r = nullptr;
+  if (host_run_get(&r)) return false;
+  hipStream_t s = r->ctx->stream;
+  auto go = [&]() -> abea_status_t {
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    abea_status_t st;
+    if ((st = upload(r->model, pore_model, (size_t)ABEA_NUM_KMER, s))) return st;
+    if ((st = upload(r->events, et.event, (size_t)et.n, s))) return st;
+    if ((st = upload(r->rec, alignment_output, (size_t)num_alignments, s))) return st;
+    if ((st = upload(r->scal, scallings, 1, s))) return st;
+    if ((st = r->status.ensure(sizeof(int32_t)))) return st;
+    abea::RecalArgs a{};
+    a.rec = r->rec.as<abea_event_alignment_t>();
+    a.n_rec = num_alignments;
+    a.events = r->events.as<abea_event_t>();
+    a.n_events = (int32_t)std::min<size_t>(et.n, 0x7fffffff);
+    a.model = r->model.as<abea_model_t>();
+    a.scal = r->scal.as<abea_scalings_t>();
+    a.status = r->status.as<int32_t>();
+    a.scale_var = scale_var ? 1 : 0;
+    HIP_TRY(abea::launch_recalibrate_records(a, s));
+    return ABEA_OK;
+  };
+  if (go()) return false;
+  int32_t status = 0;
+  abea_scalings_t sc;
+  if (download(&status, r->status, 1, s) || download(&sc, r->scal, 1, s)) return false;
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    fail(ABEA_ERR_HIP, "recalibrate_model: stream synchronize failed");
+    return false;
+  }
+  if (status < 0) {
+    fail(ABEA_ERR_INVALID, "recalibrate_model: an 'M' record's event_idx is outside the event table");
+    return false;
+  }
+  if (status == 0) return false;
+  *scallings = sc;
+  return true;
+}
+
+abea_status_t abea_postalign_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
+                                const abea_event_table *et, const abea_model_t *model,
+                                abea_aligned_pair_t *const *event_align_pairs,
+                                const int32_t *n_event_align_pairs, abea_scalings_t *scalings,
+                                abea_event_alignment_t **event_alignment,
+                                int32_t *n_event_alignment, double *events_per_base,
+                                abea_index_pair_t **base_to_event_map, int32_t *read_stat_flag) {
+  if (n_bam_rec < 0 ||
+      (n_bam_rec > 0 && (!read || !read_len || !et || !model || !event_align_pairs ||
+                         !n_event_align_pairs || !scalings || !event_alignment ||
+                         !n_event_alignment || !events_per_base || !base_to_event_map ||
+                         !read_stat_flag)))
+    return fail(ABEA_ERR_INVALID, "postalign_db: null argument");
+  if (n_bam_rec == 0) return ABEA_OK;
+  for (int32_t i = 0; i < n_bam_rec; ++i)
+    if (n_event_align_pairs[i] > 0 && !event_align_pairs[i])
+      return fail(ABEA_ERR_INVALID, "postalign_db: read %d has pairs but a null array", i);
+  HostRun *r = nullptr;
+  abea_status_t st = host_run_get(&r);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(r->ctx->device));
+  HostPost h;
+  host_post_layout(h, n_bam_rec, read, read_len, et, n_event_align_pairs);
+  std::vector<int32_t> flag, n_rec;
+  std::vector<double> epb;
+  if ((st = run_postalign(r, h, read, read_len, et, model, event_align_pairs, n_event_align_pairs,
+                          scalings, 1, flag, n_rec, epb)))
+    return st;
+  std::vector<abea_event_alignment_t> rec_h;
+  std::vector<abea_index_pair_t> bmap_h;
+  int64_t nr = 0, nb = 0;
+  for (int32_t i = 0; i < n_bam_rec; ++i) {
+    nr += std::max(n_event_align_pairs[i], 0);
+    nb += std::max(h.meta[i].n_k, 0);
+  }
+  rec_h.resize((size_t)nr);
+  bmap_h.resize((size_t)nb);
+  hipStream_t s = r->ctx->stream;
+  if ((st = download(rec_h.data(), r->rec, rec_h.size(), s))) return st;
+  if ((st = download(bmap_h.data(), r->bmap, bmap_h.size(), s))) return st;
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < n_bam_rec; ++i) {
+    read_stat_flag[i] = flag[i];
+    event_alignment[i] = nullptr;
+    base_to_event_map[i] = nullptr;
+    n_event_alignment[i] = 0;
+    events_per_base[i] = 0.0;
+    if (n_event_align_pairs[i] <= 0) continue;
+    const ReadMeta &m = h.meta[i];
+    // f5c allocates n_event_align_pairs records and n_kmers map entries
+    event_alignment[i] = (abea_event_alignment_t *)malloc(sizeof(abea_event_alignment_t) *
+                                                          (size_t)n_event_align_pairs[i]);
+    base_to_event_map[i] =
+        (abea_index_pair_t *)malloc(sizeof(abea_index_pair_t) * (size_t)std::max(m.n_k, 1));
+    if (!event_alignment[i] || !base_to_event_map[i])
+      return fail(ABEA_ERR_NOMEM, "postalign_db: malloc failed");
+    std::memcpy(event_alignment[i], rec_h.data() + m.pair_off,
+                sizeof(abea_event_alignment_t) * (size_t)n_rec[i]);
+    std::memcpy(base_to_event_map[i], bmap_h.data() + m.bmap_off,
+                sizeof(abea_index_pair_t) * (size_t)m.n_k);
+    n_event_alignment[i] = n_rec[i];
+    events_per_base[i] = epb[i];
   }
   return ABEA_OK;
 }

@@ -25,7 +25,7 @@ struct alignas(16) ReadMeta {
   int32_t n_k;        // k-mers = read_len - 5 (<= 0 for reads that are skipped)
   double lp_stay;     // per-read transition log-probabilities (host glibc log/exp)
   double lp_step;
-  int64_t reserved;
+  int64_t bmap_off;   // offset of the read's base-to-event map entries (prefix of max(n_k, 0))
 };
 static_assert(sizeof(ReadMeta) == 80, "ReadMeta must stay 80 bytes");
 
@@ -69,6 +69,62 @@ struct KernelArgs {
   // outputs
   abea_output_t out;
 };
+
+// Inputs and outputs of the post-align kernels (abea_postalign.hip): f5c's
+// postalign + recalibrate_model from AlignedPairs, one wave per read.
+struct PostArgs {
+  int32_t n_reads;
+  const ReadMeta *meta;          // n_e, n_k, ev_off, pair_off (pairs and records), bmap_off
+  const int32_t *order;          // read processed by wave w: order[w] (null: w)
+  const char *seq;
+  const int64_t *seq_ptr;
+  const abea_event_t *events;    // event records, or
+  const float *ev_mean;          // SoA event means (one of the two)
+  const abea_model_t *model;
+  const abea_scalings_t *scal_in;
+  const abea_aligned_pair_t *pairs;
+  const int32_t *n_pairs;        // per read: f5c n_event_align_pairs
+  abea_event_alignment_t *rec;   // optional
+  abea_index_pair_t *bmap;       // optional
+  int32_t *n_rec;
+  double *epb;                   // optional
+  abea_scalings_t *scal_out;
+  int32_t *flag;
+  float *terms;                  // recalibration scratch: 3 floats per k-mer (bmap_off layout)
+  int32_t recalibrate;           // 0: postalign only (no event means needed)
+  int32_t scale_var;
+};
+
+// recalibrate_model from event alignment records (one read, one wave).
+struct RecalArgs {
+  const abea_event_alignment_t *rec;
+  int32_t n_rec;
+  const abea_event_t *events;
+  int32_t n_events;
+  const abea_model_t *model;
+  abea_scalings_t *scal;         // in/out
+  int32_t *status;               // 1 recalibrated, 0 not, -1 bad record
+  int32_t scale_var;
+};
+
+// estimate_scalings_using_mom, one wave per read.
+struct MomArgs {
+  int32_t n_reads;
+  const ReadMeta *meta;          // n_e, n_k, ev_off
+  const char *seq;
+  const int64_t *seq_ptr;
+  const abea_event_t *events;
+  const float *ev_mean;
+  const abea_model_t *model;
+  abea_scalings_t *out;
+};
+
+// flag bit for a read whose pairs are not an alignment path (kernel guard)
+constexpr int32_t kBadPath = 0x100;
+
+hipError_t launch_postalign(const PostArgs &a, hipStream_t s);
+hipError_t launch_recalibrate_records(const RecalArgs &a, hipStream_t s);
+hipError_t launch_mom(const MomArgs &a, hipStream_t s);
 
 // Kernel launchers (abea_kernels.hip).
 hipError_t launch_pre(const KernelArgs &a, int q_begin, int q_count, hipStream_t s);

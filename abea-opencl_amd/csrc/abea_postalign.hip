@@ -1,0 +1,774 @@
+// Post-align kernels for gfx950 (CDNA4, wave64), one wavefront per read:
+//
+//   postalign    f5c postalign() then recalibrate_model() and the read
+//                status (GPU/f5cmisc.h:51-54, flags GPU/f5c.h:49-51) from
+//                align()'s AlignedPairs
+//   recal_records recalibrate_model() from event alignment records
+//   mom          estimate_scalings_using_mom() (GPU/f5cmisc.h:47)
+//
+// The reference only declares these functions; their bodies are f5c's ports
+// of nanopolish.  What they compute:
+//   postalign: each event on the path belongs to the k-mer of its first pair;
+//     base_to_event_map[k] = {first, last} event of k-mer k or {-1, -1};
+//     events_per_base = (last event - first event) / n_kmers; one
+//     event_alignment_t per event, in k-mer (= event) order, hmm_state 'M'
+//     when the k-mer rank differs from the previous record's, else 'E'.
+//   recalibrate_model: weighted least squares of event mean on level mean,
+//     weights 1/level_stdv^2, over the 'M' records, solved by Cramer's rule;
+//     var = sqrt(mean squared weighted residual); at least 200 'M' records.
+//   MOM: shift = mean(event means) - mean(levels); scale = mean((x -
+//     shift)^2) / mean(level^2); var = log_var = 0.
+//
+// postalign in parallel.  On an alignment path each step goes to the next
+// event (U), the next k-mer (L) or both (D).  Call pair p "new" when it is
+// the first pair of its event (p = 0 or e_p != e_{p-1}).  Then, from the
+// pair's neighbours alone:
+//   e_p is k_p's first event  unless pair p-1 is a new pair on k-mer k_p
+//   e_p is k_p's last event   unless pair p+1 is a new pair on k-mer k_p
+//   k_p has no event          when p is not new (entered by L) and pair p+1
+//                             is on another k-mer
+// and the records are the new pairs in path order (record e_p - e_0).  The
+// previous record's k-mer, for hmm_state, is the largest k-mer of the new
+// pairs before p: a DPP max-scan per 64 pairs with a carry.  Every map entry
+// is written once, so no ordering between lanes or chunks is needed.
+//
+// Sums.  The least-squares and moment sums are serial double sums in record
+// (event, k-mer) order whose results are rounded to float.  Lanes form the
+// terms in parallel and stage them in LDS; the sums add them back in order,
+// so they round exactly as the serial CPU loop does.  Division and
+// square root are the correctly rounded IEEE operations; log(var) comes from
+// the device math library (<= 1 ulp in double before rounding to float).
+#include <hip/hip_runtime.h>
+
+#include "abea_internal.h"
+#include "abea_wave.h"
+
+namespace abea {
+namespace {
+
+constexpr int kMinMRecords = 200;     // minNumEventsToRescale of recalibrate_model
+constexpr float kMinCalibrationVar = ABEA_MIN_CALIBRATION_VAR;
+constexpr int kWavesPerBlock = 4;
+
+__device__ __forceinline__ float event_mean(const abea_event_t *ev, const float *evm, int64_t i) {
+  return evm ? evm[i] : ev[i].mean;
+}
+
+// sums of the weighted least-squares normal equations
+struct Normal {
+  double a00 = 0, a01 = 0, a11 = 0, b0 = 0, b1 = 0;
+};
+
+// In-order sums.  Each chunk's terms go to the wave's LDS stage, compacted
+// to the contributing lanes in lane order; the sums then read them back in
+// that order (every lane reads the same address: a broadcast) and add them
+// one at a time, as the serial CPU loop does.
+constexpr int kStage = 6;  // doubles per staged record (48 B keeps 16-byte alignment)
+
+struct alignas(16) Stage {
+  double t[64 * kStage];
+};
+
+// One 'M' record's terms (x = event mean, level mean mu, level stdv sd).
+struct Terms {
+  double w, mw, mmw, xw, mxw;
+};
+
+__device__ __forceinline__ Terms normal_terms(float xf, float muf, float sdf) {
+  const double x = xf, mu = muf, sd = sdf;
+  Terms t;
+  t.w = 1.0 / (sd * sd);
+  t.mw = mu * t.w;
+  t.mmw = mu * mu * t.w;
+  t.xw = x * t.w;
+  t.mxw = mu * x * t.w;
+  return t;
+}
+
+// Stage the terms of the lanes with m set (record order = lane order) and
+// add them to the normal-equation sums; returns how many there were.
+__device__ __forceinline__ int add_terms(Stage &st, bool m, const Terms &t, Normal &n) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
+  if (m) {
+    double *q = st.t + kStage * lanes_below(mask);
+    *reinterpret_cast<double2 *>(q) = make_double2(t.w, t.mw);
+    *reinterpret_cast<double2 *>(q + 2) = make_double2(t.mmw, t.xw);
+    q[4] = t.mxw;
+  }
+  wave_lds_sync();
+  const int cnt = __builtin_popcountll(mask);
+  for (int j = 0; j < cnt; ++j) {
+    const double *q = st.t + kStage * j;
+    const double2 p0 = *reinterpret_cast<const double2 *>(q);
+    const double2 p1 = *reinterpret_cast<const double2 *>(q + 2);
+    n.a00 += p0.x;
+    n.a01 += p0.y;
+    n.a11 += p1.x;
+    n.b0 += p1.y;
+    n.b1 += q[4];
+  }
+  wave_lds_sync();
+  return cnt;
+}
+
+// One running sum over the lanes with m set, in lane order.
+__device__ __forceinline__ void add_in_order(Stage &st, bool m, double t, double &acc) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
+  if (m) st.t[lanes_below(mask)] = t;
+  wave_lds_sync();
+  const int cnt = __builtin_popcountll(mask);
+  for (int j = 0; j < cnt; ++j) acc += st.t[j];
+  wave_lds_sync();
+}
+
+// Two running sums over the lanes with m set, in lane order.
+__device__ __forceinline__ void add_in_order2(Stage &st, bool m, double t0, double t1,
+                                              double &s0, double &s1) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
+  if (m) *reinterpret_cast<double2 *>(st.t + 2 * lanes_below(mask)) = make_double2(t0, t1);
+  wave_lds_sync();
+  const int cnt = __builtin_popcountll(mask);
+  for (int j = 0; j < cnt; ++j) {
+    const double2 p = *reinterpret_cast<const double2 *>(st.t + 2 * j);
+    s0 += p.x;
+    s1 += p.y;
+  }
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ double var_term(float xf, float muf, float sdf, double shift,
+                                           double scale) {
+  const double x = xf, mu = muf, sd = sdf;
+  const double y = x - shift - scale * mu;
+  return y * y / (sd * sd);
+}
+
+// Cramer's rule on the 2x2 normal equations: (shift, scale).
+__device__ __forceinline__ void solve(const Normal &n, double &shift, double &scale) {
+  const double a10 = n.a01;
+  const double det = n.a00 * n.a11 - n.a01 * a10;
+  shift = -(n.a01 * n.b1 - n.a11 * n.b0) / det;
+  scale = (n.a00 * n.b1 - a10 * n.b0) / det;
+}
+
+__device__ __forceinline__ abea_scalings_t finish(double shift, double scale, double var) {
+  abea_scalings_t o;
+  o.scale = (float)scale;
+  o.shift = (float)shift;
+  o.var = (float)var;
+  o.log_var = (float)log(var);
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// postalign + recalibrate_model from pairs.
+
+struct PairView {
+  const abea_aligned_pair_t *P;
+  int32_t np, nk, n_e;
+  const char *S;
+};
+
+// Per-lane decisions for pair p of a chunk.
+struct PairStep {
+  int k, e;
+  bool valid;  // p < np
+  bool bad;    // not a step of an alignment path, or out of range
+  bool is_new, first, last, none;
+  bool m;      // the pair's record is an 'M' record
+  uint32_t lo, hi;  // the k-mer's bases 0-3 and 4-5
+  uint32_t rank;
+};
+
+// What one chunk hands to the next: its last pair, whether that pair was
+// new, and the rank of the last record so far (-1: none yet).
+struct Carry {
+  int k = -1, e = -1, is_new = 0, rank = -1;
+};
+
+__device__ __forceinline__ int2 load_pair(const abea_aligned_pair_t *P, int i) {
+  const int2 *q = reinterpret_cast<const int2 *>(P);
+  return q[i];
+}
+
+// Bases k..k+5 of the read (k in [0, len - 6]): a 4-byte and a 2-byte load
+// at the bases' own (unaligned) address, both inside the read.
+__device__ __forceinline__ void kmer_bytes(const char *S, int k, uint32_t &lo, uint32_t &hi) {
+  uint16_t h;
+  __builtin_memcpy(&lo, S + k, 4);
+  __builtin_memcpy(&h, S + k + 4, 2);
+  hi = h;
+}
+
+// Base codes of four bases packed in a word (A,C,G,T -> 0..3, anything else
+// -> 0, as GPU/align_cuda.cl:22-35), two bits per byte, without branches:
+// ((c >> 1) ^ (c >> 2)) & 3 is the code of A, C, G and T, and an exact
+// per-byte zero test of c ^ base keeps it only for those four.
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t y) {
+  return ~(((y & 0x7f7f7f7fu) + 0x7f7f7f7fu) | y | 0x7f7f7f7fu);  // 0x80 where a byte is 0
+}
+
+__device__ __forceinline__ uint32_t base_codes4(uint32_t x) {
+  const uint32_t code = ((x >> 1) ^ (x >> 2)) & 0x03030303u;
+  const uint32_t ok = zero_bytes(x ^ 0x41414141u) | zero_bytes(x ^ 0x43434343u) |
+                      zero_bytes(x ^ 0x47474747u) | zero_bytes(x ^ 0x54545454u);
+  return code & ((ok >> 7) * 3u);
+}
+
+// 6-mer rank of bases 0-3 (lo) and 4-5 (hi), first base most significant
+// (GPU/align_cuda.cl:39-51); the multiply gathers the four 2-bit codes of lo
+// into bits 24-31 in reverse byte order.
+__device__ __forceinline__ uint32_t rank_lohi(uint32_t lo, uint32_t hi) {
+  const uint32_t c4 = base_codes4(lo), c2 = base_codes4(hi);
+  return (((c4 * 0x40100401u) >> 24) << 4) | ((c2 & 3u) << 2) | ((c2 >> 8) & 3u);
+}
+
+// A chunk's raw loads: its pairs (lane 63 also loads the next chunk's first)
+// and each pair's k-mer bases.  Issued ahead of the chunk's decisions.
+struct ChunkLoad {
+  int2 c, nx;
+  uint32_t lo, hi;
+};
+
+// Loads are unconditional, from clamped (in-bounds) indices, so that they
+// can be issued ahead: a load under a branch would be waited for at the
+// branch's join.  Validity is applied where the values are used.
+__device__ __forceinline__ void load_pairs(const PairView &v, int base, int lane, ChunkLoad &L) {
+  L.c = load_pair(v.P, min(base + lane, v.np - 1));
+  L.nx = load_pair(v.P, min(base + 64, v.np - 1));
+}
+
+__device__ __forceinline__ void load_kmers(const PairView &v, ChunkLoad &L) {
+  kmer_bytes(v.S, min(max(L.c.x, 0), v.nk - 1), L.lo, L.hi);
+}
+
+// All lanes must call.  Neighbouring pairs come from the neighbouring lanes
+// (DPP lane shifts); lane 0 takes pair p-1 from the carry.
+__device__ __forceinline__ PairStep pair_step(const PairView &v, int base, int lane,
+                                              const ChunkLoad &L, bool need_events, Carry &cy) {
+  PairStep s{};
+  const int p = base + lane;
+  s.valid = p < v.np;
+  const int2 c = s.valid ? L.c : make_int2(-1, -1);
+  const int nk_ = __builtin_amdgcn_update_dpp(L.nx.x, c.x, 0x130, 0xf, 0xf, false);  // wave_shl:1
+  const int ne_ = __builtin_amdgcn_update_dpp(L.nx.y, c.y, 0x130, 0xf, 0xf, false);
+  const int ak = __builtin_amdgcn_update_dpp(cy.k, c.x, 0x138, 0xf, 0xf, false);  // wave_shr:1
+  const int ae = __builtin_amdgcn_update_dpp(cy.e, c.y, 0x138, 0xf, 0xf, false);
+  s.k = c.x;
+  s.e = c.y;
+  bool ok = c.x >= 0 && c.x < v.nk && c.y >= 0 && (!need_events || c.y < v.n_e);
+  if (p >= 1) {
+    const int dk = c.x - ak, de = c.y - ae;
+    ok = ok && (dk == 0 || dk == 1) && (de == 0 || de == 1) && (dk | de) != 0;
+  }
+  s.bad = s.valid && !ok;
+  const bool live = s.valid && ok;
+  s.is_new = live && (p == 0 || c.y != ae);
+  const bool prev_new =
+      __builtin_amdgcn_update_dpp(cy.is_new, (int)s.is_new, 0x138, 0xf, 0xf, false) != 0;
+  s.first = s.is_new && !(p >= 1 && ak == c.x && prev_new);
+  s.last = s.is_new && !(p + 1 < v.np && ne_ != c.y && nk_ == c.x);
+  s.none = live && !s.is_new && (p + 1 == v.np || nk_ != c.x);
+  s.lo = live ? L.lo : 0;
+  s.hi = live ? L.hi : 0;
+  s.rank = live ? rank_lohi(s.lo, s.hi) : 0;
+
+  // rank of the previous record: that of the last new pair before this one
+  const int incl = wave_scan_max(s.is_new ? lane : -1);
+  const int excl = __builtin_amdgcn_update_dpp(-1, incl, 0x138, 0xf, 0xf, false);  // wave_shr:1
+  const int from = __builtin_amdgcn_ds_bpermute(max(excl, 0) << 2, (int)s.rank);
+  const int prank = excl >= 0 ? from : cy.rank;
+  s.m = s.first && (prank < 0 || (uint32_t)prank != s.rank);
+
+  const int last_new = __builtin_amdgcn_readlane(incl, 63);
+  if (last_new >= 0) cy.rank = __builtin_amdgcn_readlane((int)s.rank, last_new);
+  cy.k = __builtin_amdgcn_readlane(c.x, 63);
+  cy.e = __builtin_amdgcn_readlane(c.y, 63);
+  cy.is_new = __builtin_amdgcn_readlane((int)s.is_new, 63);
+  return s;
+}
+
+// The 32 bytes of a pair's event_alignment_t record (x86-64 layout).
+__device__ __forceinline__ void record_words(const PairStep &s, uint4 &w0, uint4 &w1) {
+  w0.x = s.lo;                          // ref_kmer[0..3]
+  w0.y = s.hi;                          // ref_kmer[4..5], NUL, pad
+  w0.z = (uint32_t)s.k;                 // ref_position
+  w0.w = 0xffffffffu;                   // read_idx = -1
+  w1.x = (uint32_t)s.e;                 // event_idx
+  w1.y = s.lo << 8;                     // rc = 0, model_kmer[0..2]
+  w1.z = (s.lo >> 24) | (s.hi << 8);    // model_kmer[3..5], NUL
+  w1.w = s.m ? 'M' : 'E';               // hmm_state, pad
+}
+
+// A chunk's records are consecutive (its new pairs have consecutive events):
+// assemble them in LDS and store them as one contiguous block, 16 bytes per
+// lane per store.
+__device__ __forceinline__ void write_records(Stage &st, abea_event_alignment_t *R, int e0,
+                                              const PairStep &s, int lane) {
+  const uint64_t nm = __builtin_amdgcn_ballot_w64(s.is_new);
+  if (!nm) return;
+  uint4 *L = reinterpret_cast<uint4 *>(st.t);
+  if (s.is_new) {
+    uint4 w0, w1;
+    record_words(s, w0, w1);
+    const int j = lanes_below(nm);
+    L[2 * j] = w0;
+    L[2 * j + 1] = w1;
+  }
+  wave_lds_sync();
+  const int e_first = __builtin_amdgcn_readlane(s.e, __builtin_ctzll(nm));
+  uint4 *dst = reinterpret_cast<uint4 *>(R + (e_first - e0));
+  const int words = 2 * __builtin_popcountll(nm);
+  if (lane < words) dst[lane] = L[lane];
+  if (64 + lane < words) dst[64 + lane] = L[64 + lane];
+  wave_lds_sync();
+}
+
+// The five normal-equation sums over n 'M' triples (x, mu, sd) in record
+// order.  Each chunk's terms are staged in LDS, one record per 48 bytes; lane
+// c < 5 then runs sum c over the chunk's records in order (one LDS read and
+// one add per record), and lanes 0..4 hold the five sums at the end.
+__device__ __forceinline__ Normal normal_sums(Stage &st, const float *T, int n, int lane) {
+  double acc = 0.0;
+  const int c = lane < 5 ? lane : 4;
+  const float3 *T3 = reinterpret_cast<const float3 *>(T);
+  float3 nx = T3[min(lane, n - 1)];
+  for (int base = 0; base < n; base += 64) {
+    const float3 q = nx;  // lanes past the end hold a repeat; their terms are not added
+    nx = T3[min(base + 64 + lane, n - 1)];
+    const Terms t = normal_terms(q.x, q.y, q.z);
+    double *w = st.t + kStage * lane;
+    *reinterpret_cast<double2 *>(w) = make_double2(t.w, t.mw);
+    *reinterpret_cast<double2 *>(w + 2) = make_double2(t.mmw, t.xw);
+    w[4] = t.mxw;
+    wave_lds_sync();
+    const int cnt = min(64, n - base);
+    for (int j = 0; j < cnt; ++j) acc += st.t[kStage * j + c];
+    wave_lds_sync();
+  }
+  Normal nm;
+  nm.a00 = readlane_f64(acc, 0);
+  nm.a01 = readlane_f64(acc, 1);
+  nm.a11 = readlane_f64(acc, 2);
+  nm.b0 = readlane_f64(acc, 3);
+  nm.b1 = readlane_f64(acc, 4);
+  return nm;
+}
+
+// Sum of the squared weighted residuals over the n 'M' triples, in order.
+__device__ __forceinline__ double var_sum(Stage &st, const float *T, int n, int lane,
+                                          double shift, double scale) {
+  double acc = 0.0;
+  const float3 *T3 = reinterpret_cast<const float3 *>(T);
+  float3 nx = T3[min(lane, n - 1)];
+  for (int base = 0; base < n; base += 64) {
+    const float3 q = nx;
+    nx = T3[min(base + 64 + lane, n - 1)];
+    st.t[lane] = var_term(q.x, q.y, q.z, shift, scale);
+    wave_lds_sync();
+    const int cnt = min(64, n - base);
+    for (int j = 0; j < cnt; ++j) acc += st.t[j];
+    wave_lds_sync();
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void abea_postalign_kernel(PostArgs a) {
+  __shared__ Stage stages[kWavesPerBlock];
+  Stage &st = stages[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int w = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (w >= a.n_reads) return;
+  const int r = uniform(a.order ? a.order[w] : w);
+  const abea_scalings_t sc = a.scal_in[r];
+  const ReadMeta m = a.meta[r];
+  const int np = uniform(a.n_pairs[r]);
+  if (np <= 0) {
+    if (lane == 0) {
+      a.n_rec[r] = 0;
+      if (a.epb) a.epb[r] = 0.0;
+      a.scal_out[r] = sc;
+      a.flag[r] = ABEA_FAILED_ALIGNMENT;
+    }
+    return;
+  }
+  PairView v;
+  v.P = a.pairs + m.pair_off;
+  v.np = np;
+  v.nk = uniform(m.n_k);
+  v.n_e = uniform(m.n_e);
+  v.S = a.seq + a.seq_ptr[r];
+  const int64_t evb = m.ev_off;
+  abea_event_alignment_t *R = a.rec ? a.rec + m.pair_off : nullptr;
+  abea_index_pair_t *B = a.bmap ? a.bmap + m.bmap_off : nullptr;
+  if (v.nk <= 0) {  // pairs on a read without k-mers cannot be a path
+    if (lane == 0) {
+      a.n_rec[r] = 0;
+      a.scal_out[r] = sc;
+      a.flag[r] = kBadPath;
+    }
+    return;
+  }
+  const int2 p0 = load_pair(v.P, 0), pN = load_pair(v.P, np - 1);
+  const int e0 = uniform(p0.y), eN = uniform(pN.y);
+  const int k0 = uniform(p0.x), kN = uniform(pN.x);
+
+  // k-mers before the path's first and after its last have no events
+  if (B) {
+    const int2 none = make_int2(-1, -1);
+    for (int k = lane; k < min(k0, v.nk); k += 64) reinterpret_cast<int2 *>(B)[k] = none;
+    for (int k = max(kN + 1, 0) + lane; k < v.nk; k += 64) reinterpret_cast<int2 *>(B)[k] = none;
+  }
+
+  const bool recal = a.recalibrate != 0;
+  // the 'M' records' (event mean, level mean, level stdv), in record order;
+  // a read has at most one 'M' record per k-mer
+  float *T = recal ? a.terms + 3 * m.bmap_off : nullptr;
+  int n_m = 0;
+  Carry cy;
+  bool bad = false;
+  // Software pipeline, in issue order: the gathers of chunk c (model row and
+  // event mean of each lane's pair) are issued at the end of iteration c-1,
+  // before the k-mer loads of chunk c+1 and the pair loads of chunk c+2, so
+  // that waiting for them (the load counter is in order) never waits for the
+  // younger prefetches or for iteration c's stores.
+  auto gather = [&](const ChunkLoad &L, float &x, float &mu, float &sd) {
+    if (!recal) return;
+    const abea_model_t md = a.model[rank_lohi(L.lo, L.hi)];
+    mu = md.level_mean;
+    sd = md.level_stdv;
+    x = event_mean(a.events, a.ev_mean, evb + min(max(L.c.y, 0), v.n_e - 1));
+  };
+  ChunkLoad cur, nxt;
+  float gx = 0.f, gmu = 0.f, gsd = 0.f;
+  load_pairs(v, 0, lane, cur);
+  load_kmers(v, cur);
+  gather(cur, gx, gmu, gsd);
+  load_pairs(v, 64, lane, nxt);
+  for (int base = 0; base < np; base += 64) {
+    ChunkLoad after;
+    load_kmers(v, nxt);
+    load_pairs(v, base + 128, lane, after);
+    const PairStep s = pair_step(v, base, lane, cur, recal, cy);
+    if (__builtin_amdgcn_ballot_w64(s.bad)) {
+      bad = true;
+      break;
+    }
+    if (B) {
+      int2 *B2 = reinterpret_cast<int2 *>(B);
+      if (s.first && s.last) {
+        B2[s.k] = make_int2(s.e, s.e);
+      } else {
+        if (s.first) B[s.k].start = s.e;
+        if (s.last) B[s.k].stop = s.e;
+      }
+      if (s.none) B2[s.k] = make_int2(-1, -1);
+    }
+    if (R) write_records(st, R, e0, s, lane);
+    if (recal) {
+      const uint64_t mm = __builtin_amdgcn_ballot_w64(s.m);
+      if (s.m) {
+        float *q = T + 3 * (n_m + lanes_below(mm));
+        q[0] = gx;
+        q[1] = gmu;
+        q[2] = gsd;
+      }
+      n_m += __builtin_popcountll(mm);
+    }
+    gather(nxt, gx, gmu, gsd);
+    cur = nxt;
+    nxt = after;
+  }
+
+  if (bad) {
+    if (lane == 0) {
+      a.n_rec[r] = 0;
+      a.scal_out[r] = sc;
+      a.flag[r] = kBadPath;
+    }
+    return;
+  }
+
+  abea_scalings_t out = sc;
+  int32_t flag = 0;
+  if (recal) {
+    if (n_m >= kMinMRecords) {
+      wave_sync();  // the triples this wave stored are read back by other lanes
+      double shift, scale;
+      solve(normal_sums(st, T, n_m, lane), shift, scale);
+      double var = 1.0;
+      if (a.scale_var) {
+        var = var_sum(st, T, n_m, lane, shift, scale);
+        var /= n_m;
+        var = sqrt(var);
+      }
+      out = finish(shift, scale, var);
+      if (out.var > kMinCalibrationVar) flag = ABEA_FAILED_CALIBRATION;
+    } else {
+      flag = ABEA_FAILED_CALIBRATION;
+    }
+  }
+  if (lane == 0) {
+    a.n_rec[r] = eN - e0 + 1;
+    if (a.epb) a.epb[r] = (double)(eN - e0) / v.nk;
+    a.scal_out[r] = out;
+    a.flag[r] = flag;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// recalibrate_model from event alignment records (a caller's own records).
+
+struct RecView {
+  bool valid, bad, m;
+  float x;
+  uint32_t rank;
+};
+
+__device__ __forceinline__ RecView rec_view(const RecalArgs &a, int i) {
+  RecView v{};
+  v.valid = i < a.n_rec;
+  if (!v.valid) return v;
+  const uint4 *q = reinterpret_cast<const uint4 *>(a.rec + i);
+  const uint4 w0 = q[0], w1 = q[1];
+  const int e = (int)w1.x;
+  v.m = (w1.w & 0xff) == 'M';
+  if (v.m) {
+    if (e < 0 || e >= a.n_events) {
+      v.bad = true;
+      v.m = false;
+      return v;
+    }
+    v.rank = rank_lohi(w0.x, w0.y & 0xffffu);
+    v.x = a.events[e].mean;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(64) void abea_recal_records_kernel(RecalArgs a) {
+  __shared__ Stage st;
+  const int lane = threadIdx.x;
+  int n_m = 0;
+  bool bad = false;
+  for (int base = 0; base < a.n_rec; base += 64) {
+    const RecView v = rec_view(a, base + lane);
+    if (__builtin_amdgcn_ballot_w64(v.bad)) {
+      bad = true;
+      break;
+    }
+    n_m += __builtin_popcountll(__builtin_amdgcn_ballot_w64(v.m));
+  }
+  if (bad) {
+    if (lane == 0) *a.status = -1;
+    return;
+  }
+  if (n_m < kMinMRecords) {
+    if (lane == 0) *a.status = 0;
+    return;
+  }
+  Normal nm;
+  for (int base = 0; base < a.n_rec; base += 64) {
+    const RecView v = rec_view(a, base + lane);
+    Terms t{};
+    if (v.m) {
+      const abea_model_t md = a.model[v.rank];
+      t = normal_terms(v.x, md.level_mean, md.level_stdv);
+    }
+    add_terms(st, v.m, t, nm);
+  }
+  double shift, scale;
+  solve(nm, shift, scale);
+  double var = 1.0;
+  if (a.scale_var) {
+    var = 0.0;
+    for (int base = 0; base < a.n_rec; base += 64) {
+      const RecView v = rec_view(a, base + lane);
+      double t = 0.0;
+      if (v.m) {
+        const abea_model_t md = a.model[v.rank];
+        t = var_term(v.x, md.level_mean, md.level_stdv, shift, scale);
+      }
+      add_in_order(st, v.m, t, var);
+    }
+    var /= n_m;
+    var = sqrt(var);
+  }
+  if (lane == 0) {
+    *a.scal = finish(shift, scale, var);
+    *a.status = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// estimate_scalings_using_mom.
+
+// Sum of float terms in parallel (each lane adds its own, then a tree), exact
+// when provable: if every term is finite and a multiple of 2^g, and the sum of
+// magnitudes stays below 2^(52+g) (one bit of margin for the rounding of that
+// bound), every partial sum in any order is a double exactly, so the result
+// equals the serial sum.
+struct ExactSum {
+  double s = 0.0, bound = 0.0;  // this lane's partial sums
+  int gmin = 1 << 20;
+  bool finite = true;
+};
+
+__device__ __forceinline__ void exact_add(ExactSum &es, float x) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  const int eb = (int)((u >> 23) & 0xff);
+  if ((u & 0x7fffffffu) != 0) es.gmin = min(es.gmin, eb ? eb - 150 : -149);  // ulp exponent
+  es.finite = es.finite && eb != 0xff;
+  es.s += (double)x;
+  es.bound += fabs((double)x);
+}
+
+// The total (every lane's partial sums, tree-added); true when it provably
+// equals the serial sum.
+__device__ __forceinline__ bool exact_total(const ExactSum &es, double &total) {
+  total = wave_sum_exact_f64(es.s);
+  const double bound = wave_sum_exact_f64(es.bound);
+  const int gmin = -__builtin_amdgcn_readlane(wave_scan_max(-es.gmin), 63);
+  const bool finite = __builtin_amdgcn_ballot_w64(!es.finite) == 0;
+  return finite && bound <= ldexp(1.0, min(52 + gmin, 1000));
+}
+
+// estimate_scalings_using_mom, one wave per read.  Pass 1 sums the event
+// means and the k-mer levels (exact in parallel when provable, else again
+// serially); pass 2 runs the two serial sums of squares side by side, the
+// levels' in lane 0 and the events' in lane 1.  Each pass loads the next
+// chunk's event means and bases while it works on the current one; the
+// level means sit in LDS.
+constexpr int kMomWaves = 8;
+
+struct MomRead {
+  const abea_event_t *events;
+  const float *ev_mean;
+  int64_t evb;
+  const char *S;
+  int n_e, nk, len;
+};
+
+struct MomLoad {
+  float x;
+  uint32_t lo, hi;
+};
+
+// Unconditional loads from clamped indices (see load_pairs); a read with no
+// events or no k-mers loads nothing of that kind.
+__device__ __forceinline__ void mom_load(const MomRead &d, int i, MomLoad &L) {
+  L.x = 0.f;
+  L.lo = L.hi = 0;
+  if (d.n_e > 0) L.x = event_mean(d.events, d.ev_mean, d.evb + min(i, d.n_e - 1));
+  if (d.nk > 0) kmer_bytes(d.S, min(i, d.nk - 1), L.lo, L.hi);
+}
+
+__global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
+  __shared__ double stages[kMomWaves][128];
+  __shared__ float levels[ABEA_NUM_KMER];
+  for (int i = threadIdx.x; i < ABEA_NUM_KMER; i += blockDim.x) levels[i] = a.model[i].level_mean;
+  __syncthreads();
+  double *st = stages[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int r = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (r >= a.n_reads) return;
+  const ReadMeta m = a.meta[r];
+  MomRead d;
+  d.events = a.events;
+  d.ev_mean = a.ev_mean;
+  d.evb = m.ev_off;
+  d.S = a.seq + a.seq_ptr[r];
+  d.n_e = uniform(m.n_e);
+  d.nk = uniform(m.n_k);
+  d.len = d.nk + ABEA_KMER_SIZE - 1;
+  const int n_e = d.n_e, nk = d.nk;
+  const int n_max = max(n_e, nk);
+
+  ExactSum ex, el;
+  MomLoad nxt;
+  mom_load(d, lane, nxt);
+  for (int base = 0; base < n_max; base += 64) {
+    const MomLoad cur = nxt;
+    mom_load(d, base + 64 + lane, nxt);
+    const int i = base + lane;
+    if (i < n_e) exact_add(ex, cur.x);
+    if (i < nk) exact_add(el, levels[rank_lohi(cur.lo, cur.hi)]);
+  }
+  double ev_sum, km_sum;
+  const bool ex_ok = exact_total(ex, ev_sum), el_ok = exact_total(el, km_sum);
+  if (!ex_ok || !el_ok) {
+    double acc = 0.0;
+    mom_load(d, lane, nxt);
+    for (int base = 0; base < n_max; base += 64) {
+      const MomLoad cur = nxt;
+      mom_load(d, base + 64 + lane, nxt);
+      const int i = base + lane;
+      st[lane] = i < n_e ? (double)cur.x : 0.0;
+      st[64 + lane] = i < nk ? (double)levels[rank_lohi(cur.lo, cur.hi)] : 0.0;
+      wave_lds_sync();
+      const int c = (lane & 1) ? 64 : 0;
+      const int cnt = min(64, n_max - base);
+      for (int j = 0; j < cnt; ++j) acc += st[c + j];
+      wave_lds_sync();
+    }
+    if (!ex_ok) ev_sum = readlane_f64(acc, 0);
+    if (!el_ok) km_sum = readlane_f64(acc, 1);
+  }
+  const double shift = ev_sum / n_e - km_sum / nk;
+
+  double acc = 0.0;
+  mom_load(d, lane, nxt);
+  for (int base = 0; base < n_max; base += 64) {
+    const MomLoad cur = nxt;
+    mom_load(d, base + 64 + lane, nxt);
+    const int i = base + lane;
+    double l2 = 0.0, d2 = 0.0;
+    if (i < nk) {
+      const double l = levels[rank_lohi(cur.lo, cur.hi)];
+      l2 = l * l;
+    }
+    if (i < n_e) {
+      const double dx = (double)cur.x - shift;
+      d2 = dx * dx;
+    }
+    st[lane] = l2;
+    st[64 + lane] = d2;
+    wave_lds_sync();
+    const int c = (lane & 1) ? 64 : 0;
+    const int cnt = min(64, n_max - base);
+    for (int j = 0; j < cnt; ++j) acc += st[c + j];
+    wave_lds_sync();
+  }
+  const double km_sq = readlane_f64(acc, 0), ev_sq = readlane_f64(acc, 1);
+  const double scale = (ev_sq / n_e) / (km_sq / nk);
+  if (lane == 0) {
+    abea_scalings_t o;
+    o.scale = (float)scale;
+    o.shift = (float)shift;
+    o.var = 0.0f;
+    o.log_var = 0.0f;
+    a.out[r] = o;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_postalign(const PostArgs &a, hipStream_t s) {
+  if (a.n_reads <= 0) return hipSuccess;
+  const int blocks = (a.n_reads + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL(abea_postalign_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_recalibrate_records(const RecalArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(abea_recal_records_kernel, dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mom(const MomArgs &a, hipStream_t s) {
+  if (a.n_reads <= 0) return hipSuccess;
+  const int blocks = (a.n_reads + kMomWaves - 1) / kMomWaves;
+  hipLaunchKernelGGL(abea_mom_kernel, dim3(blocks), dim3(64 * kMomWaves), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace abea

@@ -16,6 +16,7 @@
 #ifndef ABEA_H
 #define ABEA_H
 
+#include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -60,6 +61,27 @@ typedef struct {
   int32_t read_pos;
 } abea_aligned_pair_t; /* AlignedPair, GPU/f5c.h:177-181 */
 
+typedef struct {
+  int32_t start;
+  int32_t stop; /* inclusive */
+} abea_index_pair_t; /* index_pair_t, GPU/f5c.h:184-188 */
+
+typedef struct {
+  char ref_kmer[ABEA_KMER_SIZE + 1];
+  int32_t ref_position;
+  int32_t read_idx;
+  int32_t event_idx;
+  bool rc;
+  char model_kmer[ABEA_KMER_SIZE + 1];
+  char hmm_state; /* 'M' or 'E' */
+} abea_event_alignment_t; /* event_alignment_t, GPU/f5c.h:191-207 (32 bytes) */
+
+/* read status bits, db_t.read_stat_flag (GPU/f5c.h:49-51) */
+#define ABEA_FAILED_CALIBRATION 0x001
+#define ABEA_FAILED_ALIGNMENT 0x002
+/* f5c's bound on the recalibrated var (read fails calibration above it) */
+#define ABEA_MIN_CALIBRATION_VAR 2.5f
+
 typedef enum {
   ABEA_OK = 0,
   ABEA_ERR_INVALID = 1,  /* bad argument */
@@ -96,6 +118,61 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
                             const abea_scalings_t *scalings,
                             abea_aligned_pair_t **event_align_pairs,
                             int32_t *n_event_align_pairs);
+
+/* ---------------------------------------------------------------------- */
+/* (1b) The f5c steps either side of align(), per read, host buffers.  Each
+ * runs on the calling thread's default context (one small launch); use the
+ * batched or plan forms below for many reads.  On a device error they set
+ * abea_last_error() and return: NaN scalings (MOM), -1 (postalign), false
+ * (recalibrate_model). */
+
+/* f5c estimate_scalings_using_mom(), GPU/f5cmisc.h:47: method-of-moments
+ * shift and scale from the event means and the read's k-mer levels; var and
+ * log_var are 0, as in the scalings f5c passes to align(). */
+abea_scalings_t abea_estimate_scalings_using_mom(char *sequence, int32_t sequence_len,
+                                                 abea_model_t *pore_model, abea_event_table et);
+
+/* f5c postalign(), GPU/f5cmisc.h:51-52: an aligned read's AlignedPairs
+ * (event_alignment, n_events of them, as align() returned) become the
+ * base-to-event map (n_kmers entries), events_per_base and one
+ * event_alignment_t per event on the path (at most n_events records).
+ * The pairs must form an alignment path (ascending, each step moving to the
+ * next event, the next k-mer or both, k-mers in [0, n_kmers)); other input
+ * is rejected with -1.  Returns the number of records. */
+int32_t abea_postalign(abea_event_alignment_t *alignment, abea_index_pair_t *base_to_event_map,
+                       double *events_per_base, char *sequence, int32_t n_kmers,
+                       abea_aligned_pair_t *event_alignment, int32_t n_events);
+
+/* f5c recalibrate_model(), GPU/f5cmisc.h:53-54: weighted least-squares
+ * shift/scale (and, with scale_var, var and log_var) from the 'M' records;
+ * needs at least 200 of them, otherwise returns false and leaves *scallings
+ * unchanged. */
+bool abea_recalibrate_model(abea_model_t *pore_model, abea_event_table et,
+                            abea_scalings_t *scallings,
+                            const abea_event_alignment_t *alignment_output,
+                            int32_t num_alignments, bool scale_var);
+
+/* Batched forms over f5c's db_t arrays (GPU/f5c.h:291-348), one launch per
+ * batch.  abea_estimate_scalings_db writes scalings[i] for every read.
+ * abea_postalign_db runs f5c's per-read flow after align_cuda(): for reads
+ * with n_event_align_pairs[i] > 0 it malloc()s event_alignment[i]
+ * (n_event_align_pairs[i] records) and base_to_event_map[i] (read_len[i] - 5
+ * entries), runs postalign and recalibrate_model(scale_var = true) and sets
+ * read_stat_flag[i] to ABEA_FAILED_CALIBRATION when recalibration did not
+ * run or var > ABEA_MIN_CALIBRATION_VAR; other reads get
+ * ABEA_FAILED_ALIGNMENT, NULL arrays, n_event_alignment[i] = 0 and
+ * events_per_base[i] = 0.  scalings[] is updated in place, as f5c does.
+ * The caller frees the malloc()ed arrays. */
+abea_status_t abea_estimate_scalings_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
+                                        const abea_event_table *et, const abea_model_t *model,
+                                        abea_scalings_t *scalings);
+abea_status_t abea_postalign_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
+                                const abea_event_table *et, const abea_model_t *model,
+                                abea_aligned_pair_t *const *event_align_pairs,
+                                const int32_t *n_event_align_pairs, abea_scalings_t *scalings,
+                                abea_event_alignment_t **event_alignment,
+                                int32_t *n_event_alignment, double *events_per_base,
+                                abea_index_pair_t **base_to_event_map, int32_t *read_stat_flag);
 
 /* ---------------------------------------------------------------------- */
 /* (3) Device-resident batches.  All pointers in abea_batch_t and
@@ -217,6 +294,34 @@ abea_status_t abea_plan_refine_step(abea_plan_t *plan, const abea_output_t *fwd,
  * ms_per_step (optional) receives the average device time of one step. */
 abea_status_t abea_plan_refine(abea_plan_t *plan, const abea_output_t *out, int steps,
                                float damping, double *ms_per_step);
+
+/* Post-align outputs (device pointers).  Records of read i start at its pair
+ * offset (abea_pair_offsets()); its base-to-event map entries at
+ * abea_kmer_offsets(), the exclusive prefix sum of max(read_len - 5, 0). */
+typedef struct {
+  abea_event_alignment_t *alignment;     /* optional: event alignment records */
+  abea_index_pair_t *base_to_event_map;  /* optional */
+  int32_t *n_alignment;                  /* f5c n_event_alignment (0 when align's QC failed) */
+  double *events_per_base;               /* optional */
+  abea_scalings_t *scalings;             /* recalibrated scalings; a read that is not
+                                            recalibrated keeps its input scalings.  May be
+                                            the batch's own scalings array. */
+  int32_t *read_stat_flag;               /* ABEA_FAILED_* bits */
+} abea_postalign_output_t;
+
+void abea_kmer_offsets(int32_t n_reads, const int32_t *read_len_host, int64_t *offsets);
+
+/* f5c's flow after align() (postalign, then recalibrate_model with
+ * scale_var, then the read status) for every read of the batch, from the
+ * pairs and n_pairs of a forward (fwd).  Asynchronous on the context stream.
+ * No reference batched counterpart: f5c runs these per read on host threads
+ * after align_cuda() (GPU/f5cmisc.h:51-54). */
+abea_status_t abea_plan_postalign(abea_plan_t *plan, const abea_output_t *fwd,
+                                  const abea_postalign_output_t *out, int scale_var);
+/* f5c estimate_scalings_using_mom() for every read of the batch into out[]
+ * (device, n_reads records; may be the batch's own scalings array).
+ * Asynchronous on the context stream. */
+abea_status_t abea_plan_estimate_scalings(abea_plan_t *plan, abea_scalings_t *out);
 
 /* Per-kernel device time (ms) of the last forward/adjoint, hipEvent-timed
  * on the plan's stream: [0]=pre [1]=core [2]=post [3]=adjoint. */

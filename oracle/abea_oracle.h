@@ -103,6 +103,37 @@ int orc_align_batch(int32_t n_reads, const char *seqs, const int64_t *seq_ptr,
                     const int64_t *pair_ptr, int32_t *n_pairs, float *scores,
                     int n_threads);
 
+/* ---- the steps either side of align(): postalign.c ---------------------- */
+typedef struct { int32_t start, stop; } orc_index_pair_t; /* index_pair_t, GPU/f5c.h:184-188 */
+typedef struct {                                          /* event_alignment_t, GPU/f5c.h:191-207 */
+  char ref_kmer[7];
+  int32_t ref_position;
+  int32_t read_idx;
+  int32_t event_idx;
+  uint8_t rc; /* bool */
+  char model_kmer[7];
+  char hmm_state;
+} orc_event_alignment_t;
+typedef struct { float scale, shift, var, log_var; } orc_scalings_t; /* GPU/f5c.h:151-167 */
+
+/* f5c postalign(), GPU/f5cmisc.h:51-52: AlignedPairs -> base-to-event map,
+ * events per base and the event alignment records.  Returns the number of
+ * records. */
+int32_t orc_postalign(orc_event_alignment_t *alignment, orc_index_pair_t *base_to_event_map,
+                      double *events_per_base, const char *sequence, int32_t n_kmers,
+                      const orc_pair_t *pairs, int32_t n_pairs);
+
+/* f5c recalibrate_model(), GPU/f5cmisc.h:53-54, with the event means in
+ * place of the event table.  Returns 1 when the scalings were re-estimated. */
+int orc_recalibrate_model(const orc_model_t *model, const float *ev_mean, orc_scalings_t *sc,
+                          const orc_event_alignment_t *alignment, int32_t n_alignment,
+                          int scale_var);
+
+/* f5c estimate_scalings_using_mom(), GPU/f5cmisc.h:47. */
+orc_scalings_t orc_estimate_scalings_mom(const char *sequence, int32_t sequence_len,
+                                         const orc_model_t *model, const float *ev_mean,
+                                         int32_t n_events);
+
 #ifdef __cplusplus
 }
 #endif

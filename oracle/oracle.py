@@ -37,6 +37,18 @@ class Info(C.Structure):
     ]
 
 
+class Scalings(C.Structure):
+    _fields_ = [("scale", C.c_float), ("shift", C.c_float), ("var", C.c_float),
+                ("log_var", C.c_float)]
+
+
+# event_alignment_t (GPU/f5c.h:191-207), x86-64 layout, 32 bytes
+EVENT_ALIGNMENT_DTYPE = np.dtype({
+    "names": ["ref_kmer", "ref_position", "read_idx", "event_idx", "rc", "model_kmer",
+              "hmm_state"],
+    "formats": ["S7", "<i4", "<i4", "<i4", "u1", "S7", "S1"],
+    "offsets": [0, 8, 12, 16, 20, 21, 28], "itemsize": 32})
+
 _lib = None
 
 
@@ -68,6 +80,12 @@ def lib():
         L.orc_path_score.restype = C.c_double
         L.orc_path_score.argtypes = [P, P, C.c_int32, C.c_char_p, P, C.c_int32, P, C.c_float,
                                      C.c_float]
+        L.orc_postalign.restype = C.c_int32
+        L.orc_postalign.argtypes = [P, P, P, P, C.c_int32, P, C.c_int32]
+        L.orc_recalibrate_model.restype = C.c_int
+        L.orc_recalibrate_model.argtypes = [P, P, P, P, C.c_int32, C.c_int]
+        L.orc_estimate_scalings_mom.restype = Scalings
+        L.orc_estimate_scalings_mom.argtypes = [P, C.c_int32, P, P, C.c_int32]
         L.orc_align_batch.restype = C.c_int
         L.orc_align_batch.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int]
         _lib = L
@@ -197,3 +215,70 @@ def align_batch(batch, n_threads: int = 0, want_pairs: bool = False):
         _p(np.ascontiguousarray(batch.shift, np.float32)),
         _p(pairs), _p(pair_ptr), _p(n_pairs), _p(scores), int(n_threads))
     return n_pairs, scores, int(used), (pairs, pair_ptr)
+
+
+def postalign(seq: bytes, pairs: np.ndarray):
+    """f5c postalign() on one read's AlignedPairs.  Returns (records
+    [EVENT_ALIGNMENT_DTYPE], base_to_event_map int32[n_kmers, 2],
+    events_per_base)."""
+    seq = bytes(seq)
+    n_k = len(seq) - 5
+    pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    rec = np.zeros(max(pairs.shape[0], 1), EVENT_ALIGNMENT_DTYPE)
+    bmap = np.zeros((max(n_k, 1), 2), np.int32)
+    epb = C.c_double(0.0)
+    sbuf = C.create_string_buffer(seq, len(seq) + 1)
+    n = lib().orc_postalign(_p(rec), _p(bmap), C.byref(epb), C.cast(sbuf, C.c_void_p), n_k,
+                            _p(pairs), int(pairs.shape[0]))
+    # copy as bytes: a structured copy would not keep the (zeroed) padding
+    raw = rec.view(np.uint8).reshape(-1, EVENT_ALIGNMENT_DTYPE.itemsize)[:n].copy()
+    return raw.view(EVENT_ALIGNMENT_DTYPE).reshape(-1), bmap[:n_k].copy(), float(epb.value)
+
+
+def recalibrate(model, ev_mean, scalings, records, scale_var: bool = True):
+    """f5c recalibrate_model().  scalings = (scale, shift, var, log_var);
+    returns (recalibrated, new scalings as float32[4])."""
+    model = np.ascontiguousarray(model, np.float32).reshape(4096, 3)
+    ev = np.ascontiguousarray(ev_mean, np.float32)
+    rec = np.ascontiguousarray(records, EVENT_ALIGNMENT_DTYPE)
+    sc = Scalings(*[float(v) for v in scalings])
+    ok = lib().orc_recalibrate_model(_p(model), _p(ev), C.byref(sc), _p(rec), int(rec.shape[0]),
+                                     int(bool(scale_var)))
+    return bool(ok), np.array([sc.scale, sc.shift, sc.var, sc.log_var], np.float32)
+
+
+def estimate_scalings_mom(seq: bytes, ev_mean, model):
+    """f5c estimate_scalings_using_mom(); returns float32[4] (scale, shift,
+    var, log_var)."""
+    seq = bytes(seq)
+    model = np.ascontiguousarray(model, np.float32).reshape(4096, 3)
+    ev = np.ascontiguousarray(ev_mean, np.float32)
+    sbuf = C.create_string_buffer(seq, len(seq) + 1)
+    sc = lib().orc_estimate_scalings_mom(C.cast(sbuf, C.c_void_p), len(seq), _p(model), _p(ev),
+                                         int(ev.shape[0]))
+    return np.array([sc.scale, sc.shift, sc.var, sc.log_var], np.float32)
+
+
+FAILED_CALIBRATION = 0x1  # GPU/f5c.h:50
+FAILED_ALIGNMENT = 0x2    # GPU/f5c.h:51
+MIN_CALIBRATION_VAR = 2.5
+
+
+def process_read(seq: bytes, ev_mean, model, scalings, scale_var: bool = True):
+    """f5c's per-read flow after align(): QC-passed pairs -> postalign ->
+    recalibrate_model -> read status flag (FAILED_ALIGNMENT when align's QC
+    failed; FAILED_CALIBRATION when recalibration did not run or var >
+    2.5).  Returns a dict."""
+    sc = np.asarray(scalings, np.float32)
+    r = align(seq, ev_mean, model, float(sc[0]), float(sc[1]))
+    out = {"align": r, "records": np.zeros(0, EVENT_ALIGNMENT_DTYPE), "map": None,
+           "events_per_base": 0.0, "scalings": sc.copy(), "flag": 0, "calibrated": False}
+    if r["n_pairs"] == 0:
+        out["flag"] = FAILED_ALIGNMENT
+        return out
+    rec, bmap, epb = postalign(seq, r["pairs"])
+    ok, new = recalibrate(model, ev_mean, sc, rec, scale_var)
+    out.update(records=rec, map=bmap, events_per_base=epb, scalings=new, calibrated=ok)
+    if not ok or new[2] > MIN_CALIBRATION_VAR:
+        out["flag"] = FAILED_CALIBRATION
+    return out

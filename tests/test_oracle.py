@@ -142,7 +142,8 @@ def _mutant(name):
     so = os.path.join(ROOT, "oracle", "_build", f"liborc_{name.lower()}.so")
     flags = ["-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", f"-D{name}"]
     subprocess.run(["gcc"] + flags + ["-o", so, os.path.join(ROOT, "oracle", "abea_oracle.c"),
-                                      "-lm"], check=True)
+                                      os.path.join(ROOT, "oracle", "postalign.c"), "-lm"],
+                   check=True)
     return so
 
 
