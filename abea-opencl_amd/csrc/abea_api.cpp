@@ -106,7 +106,13 @@ struct abea_plan {
   int64_t n_bands_total = 0, seg_bytes = 0, tail_pad = 0;
   int32_t n_prio = 0;  // leading queue entries run at raised wave priority
   DevBuf meta, queue, counter, km, km_rank, ev, ends, trace, moves, kck, replay, seg;
-  DevBuf terms;  // recalibration scratch, allocated by the first postalign
+  // postalign scratch, allocated by the first postalign call: per-k-mer
+  // recalibration terms, per-read walk status (zeroed once, then kept zero by
+  // the finish kernel) and the walk's work items
+  DevBuf terms, pstat, items;
+  int32_t n_items = 0;
+  bool post_ready = false;
+  std::vector<int2> items_h;
   bool replay_on = false;
   double lp_skip = 0, lp_trim = 0;
   abea_tuning_t tune{1, 24, 128, 1, 1};
@@ -303,6 +309,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
   }
   p->replay_on = false;
+  p->post_ready = false;
   if (p->gexec) {
     (void)hipGraphExecDestroy(p->gexec);
     p->gexec = nullptr;
@@ -501,7 +508,7 @@ int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
   if (!p) return 0;
   return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->km_rank.n + p->ev.n +
                    p->ends.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n +
-                   p->terms.n);
+                   p->terms.n + p->pstat.n + p->items.n);
 }
 
 int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }
@@ -601,9 +608,34 @@ abea_status_t abea_plan_postalign(abea_plan_t *p, const abea_output_t *fwd,
   a.epb = out->events_per_base;
   a.scal_out = out->scalings;
   a.flag = out->read_stat_flag;
-  abea_status_t st = p->terms.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(p->total_kmers, 1));
-  if (st) return st;
-  a.terms = p->terms.as<float>();
+  if (!p->post_ready) {
+    // walk work items in queue order, by pair capacity (the pair counts are
+    // on the device); items past a read's last pair exit at once
+    p->items_h.clear();
+    const int seg = abea::postalign_segment_pairs();
+    for (int32_t i = 0; i < p->n; ++i) {
+      const int32_t r = p->order_h[i];
+      const ReadMeta &m = p->meta_h[r];
+      const int64_t cap = (int64_t)m.n_e + std::max(m.n_k, 0);
+      for (int64_t q = 0; q * seg < cap; ++q) p->items_h.push_back(make_int2(r, (int)q));
+    }
+    p->n_items = (int32_t)p->items_h.size();
+    abea_status_t st;
+    if ((st = p->items.ensure(sizeof(int2) * std::max<size_t>(p->items_h.size(), 1)))) return st;
+    if ((st = p->terms.ensure(sizeof(float4) * (size_t)std::max<int64_t>(p->total_kmers, 1))))
+      return st;
+    if ((st = p->pstat.ensure(sizeof(int32_t) * (size_t)std::max(p->n, 1)))) return st;
+    if (p->n_items)
+      HIP_TRY(hipMemcpyAsync(p->items.p, p->items_h.data(), sizeof(int2) * p->items_h.size(),
+                             hipMemcpyHostToDevice, p->ctx->stream));
+    HIP_TRY(hipMemsetAsync(p->pstat.p, 0, sizeof(int32_t) * (size_t)std::max(p->n, 1),
+                           p->ctx->stream));
+    p->post_ready = true;
+  }
+  a.items = p->items.as<int2>();
+  a.n_items = p->n_items;
+  a.pstat = p->pstat.as<int32_t>();
+  a.terms = p->terms.as<float4>();
   a.recalibrate = 1;
   a.scale_var = scale_var ? 1 : 0;
   HIP_TRY(abea::launch_postalign(a, p->ctx->stream));
@@ -758,7 +790,7 @@ struct HostRun {
   DevBuf seq, seq_ptr, read_len, events, ev_ptr, n_events, scal, model, pairs, from, n_pairs,
       path_len, score;
   // post-align and MOM entries
-  DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms;
+  DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms, pstat, items;
   ~HostRun() {
     delete plan;
     abea_ctx_destroy(ctx);
@@ -1047,8 +1079,15 @@ abea_status_t run_postalign(HostRun *r, HostPost &h, char *const *read, const in
   if ((st = r->flag.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
   if ((st = r->scal_out.ensure(sizeof(abea_scalings_t) * (size_t)std::max(n, 1)))) return st;
   if (recalibrate &&
-      (st = r->terms.ensure(sizeof(float) * 3 * (size_t)std::max<int64_t>(bm_tot, 1))))
+      (st = r->terms.ensure(sizeof(float4) * (size_t)std::max<int64_t>(bm_tot, 1))))
     return st;
+  std::vector<int2> items;
+  const int seg = abea::postalign_segment_pairs();
+  for (int32_t i = 0; i < n; ++i)
+    for (int64_t q = 0; q * seg < (int64_t)n_pairs[i]; ++q) items.push_back(make_int2(i, (int)q));
+  if ((st = upload(r->items, items.data(), items.size(), s))) return st;
+  if ((st = r->pstat.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
+  HIP_TRY(hipMemsetAsync(r->pstat.p, 0, sizeof(int32_t) * (size_t)std::max(n, 1), s));
   abea::PostArgs a{};
   a.n_reads = n;
   a.meta = r->pmeta.as<ReadMeta>();
@@ -1065,7 +1104,10 @@ abea_status_t run_postalign(HostRun *r, HostPost &h, char *const *read, const in
   a.epb = r->epb.as<double>();
   a.scal_out = r->scal_out.as<abea_scalings_t>();
   a.flag = r->flag.as<int32_t>();
-  a.terms = recalibrate ? r->terms.as<float>() : nullptr;
+  a.items = r->items.as<int2>();
+  a.n_items = (int32_t)items.size();
+  a.pstat = r->pstat.as<int32_t>();
+  a.terms = recalibrate ? r->terms.as<float4>() : nullptr;
   a.recalibrate = recalibrate;
   a.scale_var = 1;
   HIP_TRY(abea::launch_postalign(a, s));

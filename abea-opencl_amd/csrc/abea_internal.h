@@ -90,7 +90,11 @@ struct PostArgs {
   double *epb;                   // optional
   abea_scalings_t *scal_out;
   int32_t *flag;
-  float *terms;                  // recalibration scratch: 3 floats per k-mer (bmap_off layout)
+  const int2 *items;             // walk work items: (read, segment of kSegPairs pairs)
+  int32_t n_items;
+  int32_t *pstat;                // per read: the walk found a bad path (the finish kernel
+                                 // reads and clears it; zero before the first call)
+  float4 *terms;                 // recalibration scratch: one float4 per k-mer (bmap_off layout)
   int32_t recalibrate;           // 0: postalign only (no event means needed)
   int32_t scale_var;
 };
@@ -123,6 +127,7 @@ struct MomArgs {
 constexpr int32_t kBadPath = 0x100;
 
 hipError_t launch_postalign(const PostArgs &a, hipStream_t s);
+int postalign_segment_pairs();
 hipError_t launch_recalibrate_records(const RecalArgs &a, hipStream_t s);
 hipError_t launch_mom(const MomArgs &a, hipStream_t s);
 

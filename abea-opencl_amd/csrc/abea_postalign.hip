@@ -161,7 +161,10 @@ __device__ __forceinline__ abea_scalings_t finish(double shift, double scale, do
 }
 
 // ---------------------------------------------------------------------------
-// postalign + recalibrate_model from pairs.
+// postalign + recalibrate_model from pairs: a walk kernel over segments of
+// each read's pairs, then a finish kernel per read.
+
+constexpr int kSegPairs = 2048;  // pairs per walk work item (32 chunks)
 
 struct PairView {
   const abea_aligned_pair_t *P;
@@ -172,7 +175,7 @@ struct PairView {
 // Per-lane decisions for pair p of a chunk.
 struct PairStep {
   int k, e;
-  bool valid;  // p < np
+  bool valid;  // p inside the segment
   bool bad;    // not a step of an alignment path, or out of range
   bool is_new, first, last, none;
   bool m;      // the pair's record is an 'M' record
@@ -223,16 +226,15 @@ __device__ __forceinline__ uint32_t rank_lohi(uint32_t lo, uint32_t hi) {
   return (((c4 * 0x40100401u) >> 24) << 4) | ((c2 & 3u) << 2) | ((c2 >> 8) & 3u);
 }
 
-// A chunk's raw loads: its pairs (lane 63 also loads the next chunk's first)
-// and each pair's k-mer bases.  Issued ahead of the chunk's decisions.
+// A chunk's raw loads: its pairs (and the next chunk's first pair) and each
+// pair's k-mer bases.  Loads are unconditional, from clamped (in-bounds)
+// indices, so that they can be issued ahead: a load under a branch would be
+// waited for at the branch's join.  Validity is applied where they are used.
 struct ChunkLoad {
   int2 c, nx;
   uint32_t lo, hi;
 };
 
-// Loads are unconditional, from clamped (in-bounds) indices, so that they
-// can be issued ahead: a load under a branch would be waited for at the
-// branch's join.  Validity is applied where the values are used.
 __device__ __forceinline__ void load_pairs(const PairView &v, int base, int lane, ChunkLoad &L) {
   L.c = load_pair(v.P, min(base + lane, v.np - 1));
   L.nx = load_pair(v.P, min(base + 64, v.np - 1));
@@ -242,21 +244,28 @@ __device__ __forceinline__ void load_kmers(const PairView &v, ChunkLoad &L) {
   kmer_bytes(v.S, min(max(L.c.x, 0), v.nk - 1), L.lo, L.hi);
 }
 
-// All lanes must call.  Neighbouring pairs come from the neighbouring lanes
-// (DPP lane shifts); lane 0 takes pair p-1 from the carry.
-__device__ __forceinline__ PairStep pair_step(const PairView &v, int base, int lane,
-                                              const ChunkLoad &L, bool need_events, Carry &cy) {
+// All lanes must call.  Pairs p < end belong to this work item; the path's
+// pair p+1 exists when p + 1 < np.  Neighbouring pairs come from the
+// neighbouring lanes (DPP lane shifts); lane 0 takes pair p-1 from the carry.
+// A pair is bad when its step from pair p-1 is not D, U or L, its k-mer or
+// event is out of range, or its record index e - e0 is outside [0, np).
+__device__ __forceinline__ PairStep pair_step(const PairView &v, int base, int lane, int end,
+                                              int e0, const ChunkLoad &L, bool need_events,
+                                              Carry &cy) {
   PairStep s{};
   const int p = base + lane;
-  s.valid = p < v.np;
+  s.valid = p < end;
+  // neighbours from the loaded pairs themselves: pair p+1 may lie past this
+  // work item's end (it belongs to the next one), but it is the path's
   const int2 c = s.valid ? L.c : make_int2(-1, -1);
-  const int nk_ = __builtin_amdgcn_update_dpp(L.nx.x, c.x, 0x130, 0xf, 0xf, false);  // wave_shl:1
-  const int ne_ = __builtin_amdgcn_update_dpp(L.nx.y, c.y, 0x130, 0xf, 0xf, false);
-  const int ak = __builtin_amdgcn_update_dpp(cy.k, c.x, 0x138, 0xf, 0xf, false);  // wave_shr:1
-  const int ae = __builtin_amdgcn_update_dpp(cy.e, c.y, 0x138, 0xf, 0xf, false);
+  const int nk_ = __builtin_amdgcn_update_dpp(L.nx.x, L.c.x, 0x130, 0xf, 0xf, false);  // wave_shl:1
+  const int ne_ = __builtin_amdgcn_update_dpp(L.nx.y, L.c.y, 0x130, 0xf, 0xf, false);
+  const int ak = __builtin_amdgcn_update_dpp(cy.k, L.c.x, 0x138, 0xf, 0xf, false);  // wave_shr:1
+  const int ae = __builtin_amdgcn_update_dpp(cy.e, L.c.y, 0x138, 0xf, 0xf, false);
   s.k = c.x;
   s.e = c.y;
-  bool ok = c.x >= 0 && c.x < v.nk && c.y >= 0 && (!need_events || c.y < v.n_e);
+  bool ok = c.x >= 0 && c.x < v.nk && c.y >= 0 && (!need_events || c.y < v.n_e) &&
+            c.y - e0 >= 0 && c.y - e0 < v.np;
   if (p >= 1) {
     const int dk = c.x - ak, de = c.y - ae;
     ok = ok && (dk == 0 || dk == 1) && (de == 0 || de == 1) && (dk | de) != 0;
@@ -282,8 +291,8 @@ __device__ __forceinline__ PairStep pair_step(const PairView &v, int base, int l
 
   const int last_new = __builtin_amdgcn_readlane(incl, 63);
   if (last_new >= 0) cy.rank = __builtin_amdgcn_readlane((int)s.rank, last_new);
-  cy.k = __builtin_amdgcn_readlane(c.x, 63);
-  cy.e = __builtin_amdgcn_readlane(c.y, 63);
+  cy.k = __builtin_amdgcn_readlane(L.c.x, 63);
+  cy.e = __builtin_amdgcn_readlane(L.c.y, 63);
   cy.is_new = __builtin_amdgcn_readlane((int)s.is_new, 63);
   return s;
 }
@@ -324,114 +333,81 @@ __device__ __forceinline__ void write_records(Stage &st, abea_event_alignment_t 
   wave_lds_sync();
 }
 
-// The five normal-equation sums over n 'M' triples (x, mu, sd) in record
-// order.  Each chunk's terms are staged in LDS, one record per 48 bytes; lane
-// c < 5 then runs sum c over the chunk's records in order (one LDS read and
-// one add per record), and lanes 0..4 hold the five sums at the end.
-__device__ __forceinline__ Normal normal_sums(Stage &st, const float *T, int n, int lane) {
-  double acc = 0.0;
-  const int c = lane < 5 ? lane : 4;
-  const float3 *T3 = reinterpret_cast<const float3 *>(T);
-  float3 nx = T3[min(lane, n - 1)];
-  for (int base = 0; base < n; base += 64) {
-    const float3 q = nx;  // lanes past the end hold a repeat; their terms are not added
-    nx = T3[min(base + 64 + lane, n - 1)];
-    const Terms t = normal_terms(q.x, q.y, q.z);
-    double *w = st.t + kStage * lane;
-    *reinterpret_cast<double2 *>(w) = make_double2(t.w, t.mw);
-    *reinterpret_cast<double2 *>(w + 2) = make_double2(t.mmw, t.xw);
-    w[4] = t.mxw;
-    wave_lds_sync();
-    const int cnt = min(64, n - base);
-    for (int j = 0; j < cnt; ++j) acc += st.t[kStage * j + c];
-    wave_lds_sync();
+// The rank of the record of the last new pair at or before pair q (q >= 0):
+// step back over pairs that repeat their predecessor's event.
+__device__ __forceinline__ int last_record_rank(const PairView &v, int q) {
+  int2 c = load_pair(v.P, q);
+  while (q > 0) {
+    const int2 b = load_pair(v.P, q - 1);
+    if (b.y != c.y) break;
+    c = b;
+    --q;
   }
-  Normal nm;
-  nm.a00 = readlane_f64(acc, 0);
-  nm.a01 = readlane_f64(acc, 1);
-  nm.a11 = readlane_f64(acc, 2);
-  nm.b0 = readlane_f64(acc, 3);
-  nm.b1 = readlane_f64(acc, 4);
-  return nm;
+  uint32_t lo, hi;
+  kmer_bytes(v.S, min(max(c.x, 0), v.nk - 1), lo, hi);
+  return (int)rank_lohi(lo, hi);
 }
 
-// Sum of the squared weighted residuals over the n 'M' triples, in order.
-__device__ __forceinline__ double var_sum(Stage &st, const float *T, int n, int lane,
-                                          double shift, double scale) {
-  double acc = 0.0;
-  const float3 *T3 = reinterpret_cast<const float3 *>(T);
-  float3 nx = T3[min(lane, n - 1)];
-  for (int base = 0; base < n; base += 64) {
-    const float3 q = nx;
-    nx = T3[min(base + 64 + lane, n - 1)];
-    st.t[lane] = var_term(q.x, q.y, q.z, shift, scale);
-    wave_lds_sync();
-    const int cnt = min(64, n - base);
-    for (int j = 0; j < cnt; ++j) acc += st.t[j];
-    wave_lds_sync();
-  }
-  return acc;
-}
-
-__global__ __launch_bounds__(256) void abea_postalign_kernel(PostArgs a) {
+// Walk kernel: one wave per (read, segment of kSegPairs pairs).  Writes the
+// map entries, the records and, per k-mer on the path, the (event mean,
+// level mean, level stdv, 'M') of the k-mer's first record -- each k-mer of
+// the path has exactly one pair that writes it: the first pair of its events
+// or, for a k-mer without events, the pair that passes it.  A bad path sets
+// the read's status word.
+__global__ __launch_bounds__(256) void abea_postalign_walk_kernel(PostArgs a) {
   __shared__ Stage stages[kWavesPerBlock];
   Stage &st = stages[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int w = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  if (w >= a.n_reads) return;
-  const int r = uniform(a.order ? a.order[w] : w);
-  const abea_scalings_t sc = a.scal_in[r];
-  const ReadMeta m = a.meta[r];
+  if (w >= a.n_items) return;
+  const int2 item = a.items[w];
+  const int r = uniform(item.x);
+  const int begin = uniform(item.y) * kSegPairs;
   const int np = uniform(a.n_pairs[r]);
-  if (np <= 0) {
-    if (lane == 0) {
-      a.n_rec[r] = 0;
-      if (a.epb) a.epb[r] = 0.0;
-      a.scal_out[r] = sc;
-      a.flag[r] = ABEA_FAILED_ALIGNMENT;
-    }
-    return;
-  }
+  if (begin >= np) return;
+  const ReadMeta m = a.meta[r];
   PairView v;
   v.P = a.pairs + m.pair_off;
   v.np = np;
   v.nk = uniform(m.n_k);
   v.n_e = uniform(m.n_e);
   v.S = a.seq + a.seq_ptr[r];
+  const bool recal = a.recalibrate != 0;
+  // pairs on a read without k-mers (or, to recalibrate, without events)
+  // cannot be a path
+  if (v.nk <= 0 || (recal && v.n_e <= 0)) {
+    if (lane == 0) a.pstat[r] = 1;
+    return;
+  }
+  const int end = min(begin + kSegPairs, np);
   const int64_t evb = m.ev_off;
   abea_event_alignment_t *R = a.rec ? a.rec + m.pair_off : nullptr;
   abea_index_pair_t *B = a.bmap ? a.bmap + m.bmap_off : nullptr;
-  if (v.nk <= 0) {  // pairs on a read without k-mers cannot be a path
-    if (lane == 0) {
-      a.n_rec[r] = 0;
-      a.scal_out[r] = sc;
-      a.flag[r] = kBadPath;
-    }
-    return;
-  }
-  const int2 p0 = load_pair(v.P, 0), pN = load_pair(v.P, np - 1);
-  const int e0 = uniform(p0.y), eN = uniform(pN.y);
-  const int k0 = uniform(p0.x), kN = uniform(pN.x);
+  float4 *T = recal ? a.terms + m.bmap_off : nullptr;
+  const int2 p0 = load_pair(v.P, 0);
+  const int e0 = uniform(p0.y);
 
-  // k-mers before the path's first and after its last have no events
-  if (B) {
+  if (begin == 0 && B) {  // k-mers before the path's first and after its last
+    const int k0 = uniform(p0.x), kN = uniform(load_pair(v.P, np - 1).x);
     const int2 none = make_int2(-1, -1);
     for (int k = lane; k < min(k0, v.nk); k += 64) reinterpret_cast<int2 *>(B)[k] = none;
     for (int k = max(kN + 1, 0) + lane; k < v.nk; k += 64) reinterpret_cast<int2 *>(B)[k] = none;
   }
 
-  const bool recal = a.recalibrate != 0;
-  // the 'M' records' (event mean, level mean, level stdv), in record order;
-  // a read has at most one 'M' record per k-mer
-  float *T = recal ? a.terms + 3 * m.bmap_off : nullptr;
-  int n_m = 0;
   Carry cy;
-  bool bad = false;
+  if (begin > 0) {
+    const int2 q1 = load_pair(v.P, begin - 1);
+    cy.k = uniform(q1.x);
+    cy.e = uniform(q1.y);
+    cy.is_new = begin - 1 == 0 || uniform(load_pair(v.P, begin - 2).y) != cy.e;
+    cy.rank = uniform(last_record_rank(v, begin - 1));
+  }
+
   // Software pipeline, in issue order: the gathers of chunk c (model row and
   // event mean of each lane's pair) are issued at the end of iteration c-1,
   // before the k-mer loads of chunk c+1 and the pair loads of chunk c+2, so
-  // that waiting for them (the load counter is in order) never waits for the
-  // younger prefetches or for iteration c's stores.
+  // that waiting for them (the load counter is in order) does not wait for
+  // the younger prefetches.
   auto gather = [&](const ChunkLoad &L, float &x, float &mu, float &sd) {
     if (!recal) return;
     const abea_model_t md = a.model[rank_lohi(L.lo, L.hi)];
@@ -441,18 +417,18 @@ __global__ __launch_bounds__(256) void abea_postalign_kernel(PostArgs a) {
   };
   ChunkLoad cur, nxt;
   float gx = 0.f, gmu = 0.f, gsd = 0.f;
-  load_pairs(v, 0, lane, cur);
+  load_pairs(v, begin, lane, cur);
   load_kmers(v, cur);
   gather(cur, gx, gmu, gsd);
-  load_pairs(v, 64, lane, nxt);
-  for (int base = 0; base < np; base += 64) {
+  load_pairs(v, begin + 64, lane, nxt);
+  for (int base = begin; base < end; base += 64) {
     ChunkLoad after;
     load_kmers(v, nxt);
     load_pairs(v, base + 128, lane, after);
-    const PairStep s = pair_step(v, base, lane, cur, recal, cy);
+    const PairStep s = pair_step(v, base, lane, end, e0, cur, recal, cy);
     if (__builtin_amdgcn_ballot_w64(s.bad)) {
-      bad = true;
-      break;
+      if (lane == 0) a.pstat[r] = 1;
+      return;
     }
     if (B) {
       int2 *B2 = reinterpret_cast<int2 *>(B);
@@ -465,40 +441,121 @@ __global__ __launch_bounds__(256) void abea_postalign_kernel(PostArgs a) {
       if (s.none) B2[s.k] = make_int2(-1, -1);
     }
     if (R) write_records(st, R, e0, s, lane);
-    if (recal) {
-      const uint64_t mm = __builtin_amdgcn_ballot_w64(s.m);
-      if (s.m) {
-        float *q = T + 3 * (n_m + lanes_below(mm));
-        q[0] = gx;
-        q[1] = gmu;
-        q[2] = gsd;
-      }
-      n_m += __builtin_popcountll(mm);
-    }
+    if (recal && (s.first || s.none))
+      T[s.k] = s.m ? make_float4(gx, gmu, gsd, 1.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     gather(nxt, gx, gmu, gsd);
     cur = nxt;
     nxt = after;
   }
+}
 
-  if (bad) {
+// The five normal-equation sums over the 'M' entries of T[k0..kN] (the
+// k-mers' first records, in record order).  Each chunk's terms are staged in
+// LDS, compacted to the 'M' entries, one record per 48 bytes; lane c < 5 then
+// runs sum c over them in order (one LDS read and one add per record), and
+// lanes 0..4 hold the five sums at the end.  n_m receives the count.
+__device__ __forceinline__ Normal normal_sums(Stage &st, const float4 *T, int k0, int kN,
+                                              int lane, int &n_m) {
+  double acc = 0.0;
+  const int c = lane < 5 ? lane : 4;
+  n_m = 0;
+  float4 nx = T[min(k0 + lane, kN)];
+  for (int base = k0; base <= kN; base += 64) {
+    const float4 q = nx;
+    nx = T[min(base + 64 + lane, kN)];
+    const bool mm = base + lane <= kN && q.w != 0.f;
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(mm);
+    const Terms t = normal_terms(q.x, q.y, q.z);
+    if (mm) {
+      double *w = st.t + kStage * lanes_below(mask);
+      *reinterpret_cast<double2 *>(w) = make_double2(t.w, t.mw);
+      *reinterpret_cast<double2 *>(w + 2) = make_double2(t.mmw, t.xw);
+      w[4] = t.mxw;
+    }
+    wave_lds_sync();
+    const int cnt = __builtin_popcountll(mask);
+    for (int j = 0; j < cnt; ++j) acc += st.t[kStage * j + c];
+    wave_lds_sync();
+    n_m += cnt;
+  }
+  Normal nm;
+  nm.a00 = readlane_f64(acc, 0);
+  nm.a01 = readlane_f64(acc, 1);
+  nm.a11 = readlane_f64(acc, 2);
+  nm.b0 = readlane_f64(acc, 3);
+  nm.b1 = readlane_f64(acc, 4);
+  return nm;
+}
+
+// Sum of the squared weighted residuals over the same entries, in order.
+__device__ __forceinline__ double var_sum(Stage &st, const float4 *T, int k0, int kN, int lane,
+                                          double shift, double scale) {
+  double acc = 0.0;
+  float4 nx = T[min(k0 + lane, kN)];
+  for (int base = k0; base <= kN; base += 64) {
+    const float4 q = nx;
+    nx = T[min(base + 64 + lane, kN)];
+    const bool mm = base + lane <= kN && q.w != 0.f;
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(mm);
+    const double t = var_term(q.x, q.y, q.z, shift, scale);
+    if (mm) st.t[lanes_below(mask)] = t;
+    wave_lds_sync();
+    const int cnt = __builtin_popcountll(mask);
+    for (int j = 0; j < cnt; ++j) acc += st.t[j];
+    wave_lds_sync();
+  }
+  return acc;
+}
+
+// Finish kernel: one wave per read.  Counts and status, then (with
+// recalibration) the in-order sums, the solve and the variance.
+__global__ __launch_bounds__(256) void abea_postalign_finish_kernel(PostArgs a) {
+  __shared__ Stage stages[kWavesPerBlock];
+  Stage &st = stages[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int w = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (w >= a.n_reads) return;
+  const int r = uniform(a.order ? a.order[w] : w);
+  const abea_scalings_t sc = a.scal_in[r];
+  const int np = uniform(a.n_pairs[r]);
+  if (np <= 0) {
     if (lane == 0) {
+      a.n_rec[r] = 0;
+      if (a.epb) a.epb[r] = 0.0;
+      a.scal_out[r] = sc;
+      a.flag[r] = ABEA_FAILED_ALIGNMENT;
+    }
+    return;
+  }
+  // the walk's verdict (a vector load: the walk kernel stored it)
+  if (uniform(*(volatile int32_t *)&a.pstat[r]) != 0) {
+    if (lane == 0) {
+      a.pstat[r] = 0;
       a.n_rec[r] = 0;
       a.scal_out[r] = sc;
       a.flag[r] = kBadPath;
     }
     return;
   }
+  const ReadMeta m = a.meta[r];
+  const abea_aligned_pair_t *P = a.pairs + m.pair_off;
+  const int2 p0 = load_pair(P, 0), pN = load_pair(P, np - 1);
+  const int e0 = uniform(p0.y), eN = uniform(pN.y);
+  const int k0 = uniform(p0.x), kN = uniform(pN.x);
+  const int nk = uniform(m.n_k);
 
   abea_scalings_t out = sc;
   int32_t flag = 0;
-  if (recal) {
+  if (a.recalibrate) {
+    const float4 *T = a.terms + m.bmap_off;
+    int n_m = 0;
+    const Normal nm = normal_sums(st, T, k0, kN, lane, n_m);
     if (n_m >= kMinMRecords) {
-      wave_sync();  // the triples this wave stored are read back by other lanes
       double shift, scale;
-      solve(normal_sums(st, T, n_m, lane), shift, scale);
+      solve(nm, shift, scale);
       double var = 1.0;
       if (a.scale_var) {
-        var = var_sum(st, T, n_m, lane, shift, scale);
+        var = var_sum(st, T, k0, kN, lane, shift, scale);
         var /= n_m;
         var = sqrt(var);
       }
@@ -510,7 +567,7 @@ __global__ __launch_bounds__(256) void abea_postalign_kernel(PostArgs a) {
   }
   if (lane == 0) {
     a.n_rec[r] = eN - e0 + 1;
-    if (a.epb) a.epb[r] = (double)(eN - e0) / v.nk;
+    if (a.epb) a.epb[r] = (double)(eN - e0) / nk;
     a.scal_out[r] = out;
     a.flag[r] = flag;
   }
@@ -754,10 +811,19 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
 
 hipError_t launch_postalign(const PostArgs &a, hipStream_t s) {
   if (a.n_reads <= 0) return hipSuccess;
+  if (a.n_items > 0) {
+    const int wb = (a.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(abea_postalign_walk_kernel, dim3(wb), dim3(64 * kWavesPerBlock), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const int blocks = (a.n_reads + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL(abea_postalign_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, a);
+  hipLaunchKernelGGL(abea_postalign_finish_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s,
+                     a);
   return hipGetLastError();
 }
+
+int postalign_segment_pairs() { return kSegPairs; }
 
 hipError_t launch_recalibrate_records(const RecalArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(abea_recal_records_kernel, dim3(1), dim3(64), 0, s, a);

@@ -229,6 +229,56 @@ def test_per_read_dropins(ctx):
         del keep
 
 
+def _boundary_paths():
+    """Paths whose length or whose runs of U and L steps straddle the walk's
+    2048-pair work-item boundaries, over sequences with homopolymer stretches
+    (repeated k-mers: 'E' records that start a new k-mer)."""
+    rng = np.random.default_rng(77)
+    out = []
+    for n_target, pattern in ((2047, None), (2048, None), (2049, None), (4100, "L"),
+                              (4100, "U"), (6200, "mix")):
+        moves = list(rng.choice(3, n_target - 1, p=[0.5, 0.4, 0.1]))
+        for b in (2048, 4096):
+            if b - 3 < len(moves) and pattern in ("L", "U", "mix"):
+                mv = {"L": 2, "U": 1}.get(pattern, 2)
+                for j in range(b - 4, min(b + 3, len(moves))):
+                    moves[j] = mv
+                if pattern == "mix" and b + 5 < len(moves):
+                    moves[b - 1], moves[b] = 1, 2
+        k = e = 0
+        pairs = [(0, 0)]
+        for mv in moves:
+            if mv == 0:
+                k, e = k + 1, e + 1
+            elif mv == 1:
+                e += 1
+            else:
+                k += 1
+            pairs.append((k, e))
+        L = k + 6
+        seq = bytearray(rng.choice(np.frombuffer(b"ACGT", np.uint8), L).tobytes())
+        for start in range(0, L - 12, 300):  # homopolymer stretches
+            seq[start:start + 9] = b"A" * 9
+        out.append((bytes(seq), np.array(pairs, np.int32)))
+    return out
+
+
+def test_postalign_segment_boundaries(ctx):
+    import oracle
+    L, lib = _lib()
+    for seq, pairs in _boundary_paths():
+        nk = len(seq) - 5
+        rec = np.zeros(pairs.shape[0], np.dtype((np.void, 32)))
+        bmap = np.zeros((nk, 2), np.int32)
+        epb = C.c_double()
+        n = lib.abea_postalign(rec.ctypes.data, bmap.ctypes.data, C.byref(epb), seq, nk,
+                               pairs.ctypes.data, pairs.shape[0])
+        w_rec, w_map, w_epb = oracle.postalign(seq, pairs)
+        assert n == len(w_rec), (pairs.shape[0], n, len(w_rec), lib.abea_last_error())
+        assert _first_diff(rec[:n].tobytes(), w_rec.tobytes()) is None, pairs.shape[0]
+        assert np.array_equal(bmap, w_map) and epb.value == w_epb
+
+
 def test_per_read_dropin_errors(ctx):
     L, lib = _lib()
     seq = b"ACGTACGTACGTACGT"
