@@ -30,9 +30,10 @@ HBM_PEAK_GBS = 8000.0
 W = 100
 # HBM bytes per launch of the core kernel on the headline workload, from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings
-# (profiles/abea_kernels_batched16384_ln8kb.md gives the runs).  Known for
-# the fused single-wave-workgroup variant the autotuner picks here.
-TRAFFIC_BYTES = {"abea_core_kernel<1, true>": 34140534728}
+# (profiles/abea_kernels_batched16384_ln8kb.md gives the runs), for the
+# fused-traceback launch shapes the autotuner picks here.
+TRAFFIC_BYTES = {"abea_core_kernel<4, true>": 44133278310,
+                 "abea_core_kernel<1, true>": 44455266202}
 
 
 def log(*a):
@@ -70,6 +71,66 @@ def timed_steps(plan, steps, torch):
     return (time.perf_counter() - t0) / steps
 
 
+def postalign_leg(args, ctx, tune) -> dict:
+    """f5c's steps around align() on 16,384 reads that pass align()'s QC (log-
+    normal median 8 kb, the real-data k-mer skip rate): MOM scalings, forward,
+    then postalign + recalibrate_model, with and without the event alignment
+    records."""
+    import numpy as np
+    import torch
+
+    from abea_amd import engine, synth
+    from abea_amd._lib import PLAN_BATCHED
+
+    b5 = synth.workload("postalign16384_ln8kb", seed=args.seed + 5)
+    d5 = engine.DeviceBatch.upload(b5, 0)
+    p5 = engine.Plan(ctx, d5, PLAN_BATCHED)
+    if tune:
+        p5.set_tuning(**{k: v for k, v in tune.items()})
+    p5.forward()
+    torch.cuda.synchronize()
+
+    def best_ms(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * min(ts)
+
+    fwd_ms = best_ms(p5.forward)
+    mom_ms = best_ms(lambda: p5.estimate_scalings())
+    full_ms = best_ms(lambda: p5.postalign())
+    nrec_ms = best_ms(lambda: p5.postalign(records=False))
+    pr = engine.postalign_results(p5, p5.postalign())
+    n_pairs = int(p5.n_pairs[:b5.n_reads].sum().item())
+    n_rec = int(pr["n_alignment"].sum())
+    n_km = int(np.maximum(b5.read_len.astype(np.int64) - 5, 0).sum())
+    # algorithmic bytes: pairs read (8 B each), records written (32 B),
+    # map entries written (8 B per k-mer), an event mean read (4 B) per
+    # record (an upper bound: only 'M' records need one); model rows are
+    # cache-resident
+    post_bytes = 8 * n_pairs + 32 * n_rec + 8 * n_km + 4 * n_rec
+    flags = np.bincount(pr["read_stat_flag"], minlength=3)
+    res = {
+        "workload": "postalign16384_ln8kb", "reads": b5.n_reads, "pairs": n_pairs,
+        "records": n_rec, "forward_ms": round(fwd_ms, 3),
+        "estimate_scalings_ms": round(mom_ms, 3),
+        "postalign_recalibrate_ms": round(full_ms, 3),
+        "postalign_recalibrate_no_records_ms": round(nrec_ms, 3),
+        "alg_bytes": post_bytes,
+        "achieved_gbs": round(post_bytes / (full_ms * 1e-3) / 1e9, 1),
+        "pairs_per_s": round(n_pairs / (full_ms * 1e-3), 1),
+        "status": {"ok": int(flags[0]), "failed_calibration": int(flags[1]),
+                   "failed_alignment": int(flags[2])}}
+    log(f"[bench] postalign: {res}")
+    del p5, d5, b5, pr
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +145,11 @@ def main():
                          "1 GiB device copy to calibrate HBM byte counters")
     ap.add_argument("--eager", action="store_true",
                     help="time eager launches instead of the captured graph")
+    ap.add_argument("--leg", choices=["postalign"], default=None,
+                    help="run only this extra measurement (its JSON on stdout)")
+    ap.add_argument("--tuning", default=None,
+                    help="WPB,WPC,FUSE,OVERLAP: use this core launch shape instead of "
+                         "autotuning (e.g. to profile the shape a default run picked)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: reads are independent, so N GPUs would each "
@@ -97,6 +163,10 @@ def main():
 
     from abea_amd import engine, synth
     from abea_amd._lib import PLAN_BATCHED, PLAN_PLAIN
+
+    if args.leg == "postalign":
+        print(json.dumps(postalign_leg(args, engine.Context(0), None)), flush=True)
+        return
 
     t = time.perf_counter()
     wl = "batched16384_ln8kb"
@@ -120,7 +190,11 @@ def main():
         torch.cuda.synchronize()
         log("[bench] calibration copy: 1 GiB read + 1 GiB written")
         del src, dst
-    else:
+    if args.tuning:
+        wpb, wpc, fuse, ovl = (int(x) for x in args.tuning.split(","))
+        plan.set_tuning(core_waves_per_block=wpb, core_waves_per_cu=wpc, fuse_traceback=fuse,
+                        traceback_overlap=ovl)
+    elif not args.profile:
         t = time.perf_counter()
         plan.autotune(reps=1)
         log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s")
@@ -229,6 +303,9 @@ def main():
                           "adjoint_over_forward": round(s_adj / s_fwd, 4)}
         log(f"[bench] adjoint: {out['adjoint']}")
         del p3, d3
+
+        out["postalign"] = postalign_leg(args, ctx, tune)
+        torch.cuda.empty_cache()
 
         # scaling refinement: 8,192 x 20 kb reads, scalings perturbed (shift
         # +-5, scale +-10%), five graph-replayed forward + Gauss-Newton steps
