@@ -83,6 +83,21 @@ __device__ __forceinline__ int32_t order_key(uint32_t u) {
 
 constexpr uint32_t kNegInfBits = 0xff800000u;
 
+// 0/1 results of uniform compares, computed on the scalar unit.  Written out
+// so that the bit stays a scalar integer: a C++ bool of a uniform compare is
+// otherwise carried as a lane mask and turned back into an integer with two
+// vector instructions.
+__device__ __forceinline__ uint32_t s_lt_i32(int32_t a, int32_t b) {
+  uint32_t r;
+  asm("s_cmp_lt_i32 %1, %2\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(a), "s"(b) : "scc");
+  return r;
+}
+__device__ __forceinline__ uint32_t s_eq_u32(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("s_cmp_eq_u32 %1, %2\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(a), "s"(b) : "scc");
+  return r;
+}
+
 __device__ __forceinline__ uint64_t rotl64(uint64_t x, int s) {
   return s ? ((x << s) | (x >> (64 - s))) : x;
 }
@@ -125,6 +140,9 @@ __device__ __forceinline__ float emission(float x, float m, float s, float rs, f
 // D unless U is strictly larger, then takes U when it equals the running
 // maximum, and likewise for L: on the values that occur (no NaN, no -0) that
 // is "later candidate wins ties", i.e. >=.  u / l report the two decisions.
+// The score is the maximum of the three whichever wins a tie (equal floats
+// have equal bits here), and L wins exactly when sl equals that maximum, so
+// the cell costs one max3 and two compares.
 __device__ __forceinline__ float dp_cell(double diag, double up, double left, float em,
                                          double lp_step, double lp_stay, double lp_skip,
                                          bool &u, bool &l) {
@@ -133,9 +151,9 @@ __device__ __forceinline__ float dp_cell(double diag, double up, double left, fl
   const float su = (float)((up + lp_stay) + emd);
   const float sl = (float)(left + lp_skip);
   u = su >= sd;
-  const float m1 = u ? su : sd;
-  l = sl >= m1;
-  return l ? sl : m1;
+  const float m = __builtin_fmaxf(__builtin_fmaxf(sd, su), sl);
+  l = sl == m;
+  return m;
 }
 
 // lane `lane` of v takes the wave-uniform value x (v_writelane_b32)
@@ -324,44 +342,43 @@ __device__ __forceinline__ int slot_lane(int k) { return (k >> 1) & 63; }
 // One interior band: every window cell is inside the matrix, no trim cell,
 // the last k-mer is not in the window.  Branch-free; the scalar state is the
 // lower-left k-mer K, the parity of K only selecting registers and lanes.
+// The events and k-mer parameters under the slots come from the tile's
+// gathered operands (tile_fast).
 // Masking: an in-window cell reads only in-window cells of the previous two
 // bands or the two slots just outside the window (K-1 and K+100), so those
 // two are set to -inf and the other out-of-window slots are left as they are
-// (never read by an in-window cell).  The k-mer refill of a right move is
-// deferred to the end of the tile (tile_fast).
+// (never read by an in-window cell).
 template <bool ODD>
-__device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, float &nx_ev, int i,
-                                          uint32_t ebase, c_u8 *evb, double lp_step,
-                                          double lp_stay, double lp_skip) {
-  const int p = K & 1, h = slot_lane(K);
-  // Suzuki's rule (GPU/align_cuda.cl:398-410) as the reference's float
-  // comparison: ll (slot K) against every lane of the other register,
-  // keeping the bit of slot K+99's lane.
-  // registers of slot K's parity (lower-left) and of the other parity
-  const float mp = p ? s.M1 : s.M0;
-  const float q = p ? s.M0 : s.M1;
-  const uint32_t llb = lane_bits(mp, h);
-  const float ll = __builtin_bit_cast(float, llb);
-  const int lu = slot_lane(K + 99);
-  uint32_t r = (uint32_t)(__builtin_amdgcn_ballot_w64(ll < q) >> lu) & 1u;
+__device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i, float ev0,
+                                          float ev1, const KmerParam &k0, const KmerParam &k1,
+                                          double lp_step, double lp_stay, double lp_skip) {
+  // Suzuki's rule (GPU/align_cuda.cl:398-410) on the scalar unit: slot K
+  // (lane h, parity p) and slot K+99 (lane lu, the other parity) are read
+  // out of the two registers with lane indices chosen by p, then compared
+  // as the reference's floats through their order keys.
+  const int p = K & 1, h = slot_lane(K), lu = slot_lane(K + 99);
+  const uint32_t a0 = lane_bits(s.M0, p ? lu : h);
+  const uint32_t a1 = lane_bits(s.M1, p ? h : lu);
+  const uint32_t llb = p ? a1 : a0, urb = p ? a0 : a1;
+  uint32_t r = s_lt_i32(order_key(llb), order_key(urb));
   if (ODD) {
     // both ends outside the matrix: right on odd bands
-    const uint32_t ur_out = (uint32_t)(__builtin_amdgcn_ballot_w64(q == ABEA_NEG_INF) >> lu) & 1u;
-    r |= ur_out & (llb == kNegInfBits ? 1u : 0u);
+    r |= s_eq_u32(llb, kNegInfBits) & s_eq_u32(urb, kNegInfBits);
   }
-  // Events move up one slot; slot K takes event E+1.  The other register
-  // takes the same write in slot K+101, above the window, whose event value is
-  // never read again (values move up one slot per band, the window at most
-  // one).
-  const float e_rot = rotate_up(s.ev1);
-  const int d = slot_lane(K + 101);
-  s.ev1 = put_lane(s.ev0, nx_ev, p ? h : d);
-  s.ev0 = put_lane(e_rot, nx_ev, p ? d : h);
-  // the move; E + 1 = b - 1 - K
   K += (int)r;
   gm |= r << i;
-  nx_ev = *(c_f32 *)(evb + (ebase - 4u * (uint32_t)K));
-  const Cells c = compute_cells(s, lp_step, lp_stay, lp_skip);
+  Slots t = s;
+  t.ev0 = ev0;
+  t.ev1 = ev1;
+  t.km0m = k0.mean;
+  t.km0s = k0.stdv;
+  t.km0r = k0.rstdv;
+  t.km0c = k0.c;
+  t.km1m = k1.mean;
+  t.km1s = k1.stdv;
+  t.km1r = k1.rstdv;
+  t.km1c = k1.c;
+  const Cells c = compute_cells(t, lp_step, lp_stay, lp_skip);
   const int p2 = K & 1;
   const int lb = slot_lane(K - 1), lt = slot_lane(K + 100);
   const float ninf = ABEA_NEG_INF;
@@ -375,6 +392,7 @@ struct ReadConst {
   double lp_stay, lp_step, lp_skip, lp_trim;
   c_u8 *evb;
   c_u8 *kmb;
+  const float *evp;      // the same events, for vector loads
   const KmerParam *kmp;  // the same k-mer records, for vector loads
   uint32_t *trace;  // the read's tiles, offset by the lane
   uint32_t *moves;
@@ -456,38 +474,80 @@ __device__ __forceinline__ void band_any(Slots &s, Walk &w, const ReadConst &rc,
     band_general<0>(s, w, rc, b, lane);
 }
 
+// Operands of one 8-band tile, gathered from the staged arrays: the events
+// under each lane's two slots on the 8 bands, and the two slots' k-mers.
+struct TileOps {
+  float e0[8], e1[8];
+  KmerParam k0, k1;
+};
+
+// Smallest lower-left event E0 of a fast tile: the gathers of the tile and
+// of the next one then read events at indices >= E0 - 126 >= 1.
+constexpr int kFastMinE = 127;
+
 // Bands b0 .. b0+7 (b0 a multiple of 8) when all of them are interior.
-// The slots a right move vacates (slot K takes k-mer K+128) are among
-// K0 .. K0+7; their k-mers are loaded when the tile starts and merged when it
-// ends, for the slots actually vacated.  A vacated slot re-enters the window
-// only after 29 more right moves, so the merge is in time.
+//
+// Slot j holds k-mer K0 + ((j - K0) mod 128) for every k-mer the window can
+// reach in the tile (K grows by at most 8, so the window stays inside
+// [K0, K0+107]), and its event on band b is b - 2 - k.  So the tile's
+// operands follow from K0 alone, and so do the next tile's: its window lies
+// inside [K0, K0+115], where this tile's slot assignment still holds (the
+// slots vacated here are below it).  Each tile therefore gathers the next
+// one's operands into N (the first tile of a run gathers its own first,
+// tile_gather).
+//
+// Index bounds: events b0 - 2 - K0 - off + i = E0 + 1 - off + i lie in
+// [E0 - 126, E0 + 16] = [1, n_e + 7] for E0 >= kFastMinE and E0 + 8 <= n_e - 1,
+// and k-mers K0 + off <= K0 + 127 < n_k + 19; the staged arrays carry that
+// much slack past every read.
+struct TileAddr {
+  const float *ep0, *ep1;        // event of slot 2l / 2l+1 on the tile's first band
+  const KmerParam *kq0, *kq1;    // k-mer of slot 2l / 2l+1
+};
+__device__ __forceinline__ TileAddr tile_addr(const ReadConst &rc, int b0, int K0, int lane) {
+  const uint32_t off0 = (uint32_t)(2 * lane - K0) & 127u, off1 = (uint32_t)(2 * lane + 1 - K0) & 127u;
+  const uint32_t eb = (uint32_t)(b0 - 2 - K0);  // >= off + 1, see above
+  TileAddr t;
+  t.ep0 = (const float *)((const char *)rc.evp + 4u * (eb - off0));
+  t.ep1 = (const float *)((const char *)rc.evp + 4u * (eb - off1));
+  t.kq0 = (const KmerParam *)((const char *)(rc.kmp + K0) + 16u * off0);
+  t.kq1 = (const KmerParam *)((const char *)(rc.kmp + K0) + 16u * off1);
+  return t;
+}
+
+// The first tile of a run gathers its own operands.
+__device__ __forceinline__ void tile_gather(TileOps &T, const Walk &w, const ReadConst &rc, int b0,
+                                            int lane) {
+  const TileAddr t = tile_addr(rc, b0, walk_k(w), lane);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    T.e0[i] = t.ep0[i];
+    T.e1[i] = t.ep1[i];
+  }
+  T.k0 = *t.kq0;
+  T.k1 = *t.kq1;
+}
+
 __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc, int b0,
-                                          int lane) {
+                                          int lane, const TileOps &T, TileOps &N) {
   const int K0 = walk_k(w);
   int K = K0;
   uint32_t gm = 0;
-  const int off0 = (2 * lane - K0) & 127, off1 = (2 * lane + 1 - K0) & 127;
-  const KmerParam st0 = rc.kmp[K0 + 128 + (off0 < 8 ? off0 : 0)];
-  const KmerParam st1 = rc.kmp[K0 + 128 + (off1 < 8 ? off1 : 0)];
-  float nx_ev = w.nx_ev;
+  const TileAddr t = tile_addr(rc, b0, K0, lane);
+  const float *ep0 = t.ep0, *ep1 = t.ep1;
+  N.k0 = *t.kq0;
+  N.k1 = *t.kq1;
 #pragma unroll
   for (int i = 0; i < 8; i += 2) {
-    // byte offset of event E+1 after band b0+i is 4 (b0 + i - 1 - K)
-    band_fast<false>(s, K, gm, nx_ev, i, 4u * (uint32_t)(b0 + i - 1), rc.evb, rc.lp_step,
-                     rc.lp_stay, rc.lp_skip);
-    band_fast<true>(s, K, gm, nx_ev, i + 1, 4u * (uint32_t)(b0 + i), rc.evb, rc.lp_step,
+    band_fast<false>(s, K, gm, i, T.e0[i], T.e1[i], T.k0, T.k1, rc.lp_step, rc.lp_stay,
+                     rc.lp_skip);
+    N.e0[i] = ep0[8 + i];
+    N.e1[i] = ep1[8 + i];
+    band_fast<true>(s, K, gm, i + 1, T.e0[i + 1], T.e1[i + 1], T.k0, T.k1, rc.lp_step,
                     rc.lp_stay, rc.lp_skip);
+    N.e0[i + 1] = ep0[9 + i];
+    N.e1[i + 1] = ep1[9 + i];
   }
-  const int n = K - K0;  // right moves in this tile
-  const bool v0 = off0 < n, v1 = off1 < n;
-  s.km0m = v0 ? st0.mean : s.km0m;
-  s.km0s = v0 ? st0.stdv : s.km0s;
-  s.km0r = v0 ? st0.rstdv : s.km0r;
-  s.km0c = v0 ? st0.c : s.km0c;
-  s.km1m = v1 ? st1.mean : s.km1m;
-  s.km1s = v1 ? st1.stdv : s.km1s;
-  s.km1r = v1 ? st1.rstdv : s.km1r;
-  s.km1c = v1 ? st1.c : s.km1c;
   w.koff = 16u * (uint32_t)(K + 128);
   w.eoff = 4u * (uint32_t)(b0 + 6 - K);  // E + 1 = b - 1 - K after band b = b0+7
   w.mv |= gm << (b0 & 31);
@@ -498,8 +558,29 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
     rc.kck[b0 >> 5] = K;
     w.mv = 0;
   }
-  // the general path expects both prefetches current
-  prefetch(w, rc.evb, rc.kmb);
+}
+
+// Fast tiles leave the slot registers' events and k-mers, and the walk's
+// scalar prefetches, behind; the general band expects what band b-1 left there: slot j with k-mer K + o and event
+// E - o, o = (j - K) mod 128 (E + K = b - 3).  A negative event index
+// belongs to a slot outside the window and is clamped.  After a fast tile
+// K >= 0, E <= n_e - 1 and K + 127 < n_k + 27, inside the staged slack.
+__device__ __forceinline__ void restore_slots(Slots &s, Walk &w, const ReadConst &rc,
+                                              int lane) {
+  const int K = walk_k(w), E = walk_e(w);
+  const int o0 = (2 * lane - K) & 127, o1 = (2 * lane + 1 - K) & 127;
+  s.ev0 = rc.evp[max(E - o0, 0)];
+  s.ev1 = rc.evp[max(E - o1, 0)];
+  const KmerParam p0 = rc.kmp[K + o0], p1 = rc.kmp[K + o1];
+  s.km0m = p0.mean;
+  s.km0s = p0.stdv;
+  s.km0r = p0.rstdv;
+  s.km0c = p0.c;
+  s.km1m = p1.mean;
+  s.km1s = p1.stdv;
+  s.km1r = p1.rstdv;
+  s.km1c = p1.c;
+  prefetch(w, rc.evb, rc.kmb);  // event E+1 and k-mer K+128 for the next band
 }
 
 __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int r,
@@ -514,6 +595,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.lp_trim = a.lp_trim;
   rc.evb = (c_u8 *)(a.ev + m.ev_off);
   rc.kmb = (c_u8 *)(a.km + m.km_off);
+  rc.evp = a.ev + m.ev_off;
   rc.kmp = a.km + m.km_off;
   rc.trace = a.trace + m.trace_off + lane;
   rc.moves = a.moves + m.mv_off;
@@ -564,20 +646,32 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   w.best_e = 0;
   prefetch(w, rc.evb, rc.kmb);
 
-  for (int b = 2; b < n_bands;) {
-    // a whole tile of interior bands takes the fast path: over 8 bands K and
-    // E grow by at most 8
+  // A whole tile of interior bands takes the fast path (over 8 bands K and E
+  // grow by at most 8).  Fast tiles come in pairs that trade the two operand
+  // buffers, so a tile's gathers for the next one land where that tile reads.
+  auto tile_ok = [&](int b) {
     const int K0 = walk_k(w), E0 = walk_e(w);
-    const bool fast = (b & 7) == 0 && b + 8 <= n_bands && !rc.replay && K0 >= 0 &&
-                      E0 >= ABEA_BANDWIDTH - 1 && K0 + 8 < n_k - ABEA_BANDWIDTH &&
-                      E0 + 8 <= n_e - 1;
-    if (fast) {
-      tile_fast(s, w, rc, b, lane);
-      b += 8;
-    } else {
-      band_any(s, w, rc, b, lane);
-      ++b;
+    return (b & 7) == 0 && b + 8 <= n_bands && !rc.replay && K0 >= 0 && E0 >= kFastMinE &&
+           K0 + 8 < n_k - ABEA_BANDWIDTH && E0 + 8 <= n_e - 1;
+  };
+  // The slot registers' events and k-mers are dead inside a run of fast
+  // tiles and rebuilt when it ends.
+  for (int b = 2; b < n_bands;) {
+    if (tile_ok(b)) {
+      TileOps T0, T1;
+      tile_gather(T0, w, rc, b, lane);
+      do {
+        tile_fast(s, w, rc, b, lane, T0, T1);
+        b += 8;
+        if (!tile_ok(b)) break;
+        tile_fast(s, w, rc, b, lane, T1, T0);
+        b += 8;
+      } while (tile_ok(b));
+      if (b >= n_bands) break;
+      restore_slots(s, w, rc, lane);
     }
+    band_any(s, w, rc, b, lane);
+    ++b;
   }
   const int last = n_bands - 1;
   if ((last & 7) != 7) rc.trace[(last >> 3) * kTileWords] = s.acc << (4 * (7 - (last & 7)));
