@@ -4,8 +4,8 @@
 // Orchestration replaces the reference's align_ocl() (GPU/align_cuda.c:
 // 391-943): flatten -> buffers -> pre/core/post -> readback.  Differences by
 // design: device buffers are planned once per batch and reused, reads are
-// queued longest-first for a persistent core, and the three launches are
-// replayed as one hipGraph.
+// queued longest-first for a persistent core that also stages each read (the
+// reference's pre kernel), and the launches are replayed as one hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -367,7 +367,6 @@ abea_status_t enqueue_forward(abea_plan *p, const KernelArgs &a, bool with_event
   if (p->form == ABEA_PLAN_PLAIN) {
     // reference-shaped: one launch set per read, reads in input order
     for (int32_t i = 0; i < p->n; ++i) {
-      HIP_TRY(abea::launch_pre(a, i, 1, s));
       HIP_TRY(abea::launch_core(a, i, 1, false, 1, 1, 0, t.traceback_overlap, s));
       HIP_TRY(abea::launch_post(a, i, 1, t.traceback_overlap, s));
     }
@@ -378,7 +377,7 @@ abea_status_t enqueue_forward(abea_plan *p, const KernelArgs &a, bool with_event
     }
     return ABEA_OK;
   }
-  HIP_TRY(abea::launch_pre(a, 0, p->n, s));
+  HIP_TRY(abea::launch_queue_reset(a, s));
   if (with_events) HIP_TRY(hipEventRecord(p->evt[1], s));
   const int wpb = t.core_waves_per_block;
   int grid = std::max(1, (p->ctx->n_cu * t.core_waves_per_cu) / wpb);
@@ -596,7 +595,7 @@ abea_status_t abea_plan_postalign(abea_plan_t *p, const abea_output_t *fwd,
   a.order = p->form == ABEA_PLAN_BATCHED ? p->queue.as<int32_t>() : nullptr;
   a.seq = p->batch.seq;
   a.seq_ptr = p->batch.seq_ptr;
-  a.events = p->batch.events;  // records win over means, as in the pre kernel
+  a.events = p->batch.events;  // records win over means, as in the staging
   a.ev_mean = p->batch.events ? nullptr : p->batch.ev_mean;
   a.model = p->batch.model;
   a.scal_in = p->batch.scalings;
@@ -651,7 +650,7 @@ abea_status_t abea_plan_estimate_scalings(abea_plan_t *p, abea_scalings_t *out) 
   a.meta = p->meta.as<ReadMeta>();
   a.seq = p->batch.seq;
   a.seq_ptr = p->batch.seq_ptr;
-  a.events = p->batch.events;  // records win over means, as in the pre kernel
+  a.events = p->batch.events;  // records win over means, as in the staging
   a.ev_mean = p->batch.events ? nullptr : p->batch.ev_mean;
   a.model = p->batch.model;
   a.out = out;

@@ -29,7 +29,7 @@ struct alignas(16) ReadMeta {
 };
 static_assert(sizeof(ReadMeta) == 80, "ReadMeta must stay 80 bytes");
 
-// Staged per-k-mer emission parameters written by the pre kernel.
+// Staged per-k-mer emission parameters, written by the core wave of the read.
 struct alignas(16) KmerParam {
   float mean;   // scale * level_mean + shift (f32 multiply then add)
   float stdv;   // level_stdv
@@ -132,7 +132,8 @@ hipError_t launch_recalibrate_records(const RecalArgs &a, hipStream_t s);
 hipError_t launch_mom(const MomArgs &a, hipStream_t s);
 
 // Kernel launchers (abea_kernels.hip).
-hipError_t launch_pre(const KernelArgs &a, int q_begin, int q_count, hipStream_t s);
+// Clears the persistent core's queue head (before each batched core launch).
+hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s);
 // fuse != 0: the core wave also does its read's traceback; otherwise the
 // post kernel does.  overlap: rows each traceback segment starts above itself.
 hipError_t launch_core(const KernelArgs &a, int q_begin, int q_count, bool persistent,

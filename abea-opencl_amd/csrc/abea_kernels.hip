@@ -1,10 +1,10 @@
 // ABEA kernels for gfx950 (CDNA4, wave64).
 //
-//   pre      stage per-k-mer emission parameters and SoA event means
-//            (replaces align_kernel_pre_2d, GPU/align_cuda.cl:162-276)
-//   core     adaptive band walk, one wavefront per read; optionally followed
-//            by that read's traceback in the same wave
-//            (replaces align_kernel_core_2d_shm, GPU/align_cuda.cl:297-609)
+//   core     per read, one wavefront: staging of the per-k-mer emission
+//            parameters and SoA event means (replaces align_kernel_pre_2d,
+//            GPU/align_cuda.cl:162-276), then the adaptive band walk
+//            (replaces align_kernel_core_2d_shm, GPU/align_cuda.cl:297-609),
+//            optionally followed by the read's traceback
 //   post     best end cell, traceback over the trace, QC
 //            (replaces align_kernel_post, GPU/align_cuda.cl:618-899)
 //   adjoint  gradient of the best-path score along the traced path (new)
@@ -117,8 +117,8 @@ __device__ __forceinline__ bool lane_in(uint64_t mask) {
 }
 
 // f5c emission, GPU/align_cuda.cl:511-528: a = (x - m) / s;
-// em = (-0.918938f - log_s) + (-0.5a)a, with c = -0.918938f - log_s staged by
-// pre.  The quotient is the correctly rounded (x - m) / s, obtained from the
+// em = (-0.918938f - log_s) + (-0.5a)a, with c = -0.918938f - log_s staged
+// per k-mer.  The quotient is the correctly rounded (x - m) / s, obtained from the
 // correctly rounded reciprocal rs = 1/s by one fused residual correction
 // (Markstein): q0 = n*rs, r = fma(-q0, s, n) (exact), a = fma(r, rs, q0).
 // This equals IEEE division for the finite, non-underflowing quotients that
@@ -189,34 +189,62 @@ __device__ __forceinline__ uint32_t shift_in4(uint32_t acc, uint64_t u0, uint64_
 }
 
 // ---------------------------------------------------------------------------
-// pre: one workgroup per read (grid-stride), coalesced over k-mers and events.
-__global__ __launch_bounds__(256) void abea_pre_kernel(KernelArgs a, int q_begin, int q_count) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0;  // persistent core's queue head
-  for (int qi = blockIdx.x; qi < q_count; qi += gridDim.x) {
-    const int r = ((c_i32 *)a.queue)[q_begin + qi];
-    const ReadMeta m = load_meta(a.meta, r);
-    if (m.n_k <= 0 || m.n_e <= 0) continue;
-    const char *s = a.seq + a.seq_ptr[r];
-    const abea_scalings_t sc = a.scalings[r];
-    KmerParam *km = a.km + m.km_off;
-    uint16_t *kr = a.km_rank + m.km_off;
-    for (int k = threadIdx.x; k < m.n_k; k += blockDim.x) {
-      uint32_t rank = 0;
+// Staging of one read by the wave that walks it (in place of
+// align_kernel_pre_2d, GPU/align_cuda.cl:162-276): per-k-mer emission
+// parameters and model ranks, and the event means as a plain float array.
+// Four k-mers (events) per lane and round, their loads issued (from clamped
+// indices) before any of the round's stores.  The wave's later loads see the
+// stores without a cache operation: a CU's vector L1 stays coherent with its
+// own stores, and the walk reads the staged arrays through vector loads only.
+__device__ __forceinline__ void stage_read(const KernelArgs &a, const ReadMeta &m, int r,
+                                           int lane) {
+  const char *S = a.seq + a.seq_ptr[r];
+  const abea_scalings_t sc = a.scalings[r];
+  KmerParam *km = a.km + m.km_off;
+  uint16_t *kr = a.km_rank + m.km_off;
+  for (int k0 = 0; k0 < m.n_k; k0 += 256) {
+    uint32_t rank[4];
 #pragma unroll
-      for (int i = 0; i < ABEA_KMER_SIZE; ++i) rank = (rank << 2) | base_code(s[k + i]);
-      const abea_model_t md = a.model[rank];
-      KmerParam p;
-      p.mean = sc.scale * md.level_mean + sc.shift;  // two roundings, as the reference
-      p.stdv = md.level_stdv;                        // scaling.var unused, GPU/align_cuda.cl:515
-      p.rstdv = 1.0f / md.level_stdv;                // IEEE (correctly rounded) divide
-      p.c = -0.918938f - md.level_log_stdv;          // cached log, GPU/align_cuda.cl:517-518
-      km[k] = p;
-      kr[k] = (uint16_t)rank;
+    for (int u = 0; u < 4; ++u) {
+      uint32_t lo, hi;
+      kmer_bytes(S, min(k0 + 64 * u + lane, m.n_k - 1), lo, hi);
+      rank[u] = rank_lohi(lo, hi);
     }
-    float *ev = a.ev + m.ev_off;
-    for (int e = threadIdx.x; e < m.n_e; e += blockDim.x)
-      ev[e] = a.events ? a.events[m.ev_off + e].mean : a.ev_mean_in[m.ev_off + e];
+    abea_model_t md[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) md[u] = a.model[rank[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 64 * u + lane;
+      if (k < m.n_k) {
+        KmerParam p;
+        p.mean = sc.scale * md[u].level_mean + sc.shift;  // two roundings, as the reference
+        p.stdv = md[u].level_stdv;                        // scaling.var unused, GPU/align_cuda.cl:515
+        p.rstdv = 1.0f / md[u].level_stdv;                // IEEE (correctly rounded) divide
+        p.c = -0.918938f - md[u].level_log_stdv;          // cached log, GPU/align_cuda.cl:517-518
+        km[k] = p;
+        kr[k] = (uint16_t)rank[u];
+      }
+    }
   }
+  float *ev = a.ev + m.ev_off;
+  for (int e0 = 0; e0 < m.n_e; e0 += 256) {
+    float v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = m.ev_off + min(e0 + 64 * u + lane, m.n_e - 1);
+      v[u] = a.events ? a.events[e].mean : a.ev_mean_in[e];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (e0 + 64 * u + lane < m.n_e) ev[e0 + 64 * u + lane] = v[u];
+  }
+  wave_sync();
+}
+
+// The persistent core's queue head, cleared before each batched launch.
+__global__ void abea_queue_reset_kernel(int *counter) {
+  if (threadIdx.x == 0) *counter = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -246,15 +274,20 @@ struct Walk {
   int best_e;                           // and its event
 };
 
-typedef __attribute__((address_space(4))) const uint8_t c_u8;
 
-__device__ __forceinline__ void prefetch(Walk &w, c_u8 *evb, c_u8 *kmb) {
-  w.nx_ev = *(c_f32 *)(evb + w.eoff);
-  c_f32 *kp = (c_f32 *)(kmb + w.koff);
-  w.nk_m = kp[0];
-  w.nk_s = kp[1];
-  w.nk_r = kp[2];
-  w.nk_c = kp[3];
+// Event E+1 and k-mer K+128 as wave-uniform values.  Vector loads: the
+// staged arrays are written in the same kernel, which rules out the scalar
+// cache.
+__device__ __forceinline__ float uniform_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ void prefetch(Walk &w, const float *evp, const KmerParam *kmp) {
+  w.nx_ev = uniform_f32(*(const float *)((const char *)evp + w.eoff));
+  const KmerParam k = *(const KmerParam *)((const char *)kmp + w.koff);
+  w.nk_m = uniform_f32(k.mean);
+  w.nk_s = uniform_f32(k.stdv);
+  w.nk_r = uniform_f32(k.rstdv);
+  w.nk_c = uniform_f32(k.c);
 }
 
 __device__ __forceinline__ int walk_k(const Walk &w) { return (int)(w.koff >> 4) - 128; }
@@ -390,10 +423,8 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i,
 struct ReadConst {
   int n_e, n_k, js;
   double lp_stay, lp_step, lp_skip, lp_trim;
-  c_u8 *evb;
-  c_u8 *kmb;
-  const float *evp;      // the same events, for vector loads
-  const KmerParam *kmp;  // the same k-mer records, for vector loads
+  const float *evp;      // the read's staged events
+  const KmerParam *kmp;  // the read's staged k-mer records
   uint32_t *trace;  // the read's tiles, offset by the lane
   uint32_t *moves;
   int32_t *kck;
@@ -422,7 +453,7 @@ __device__ __forceinline__ void band_general(Slots &s, Walk &w, const ReadConst 
     ++E;
   }
   w.mv |= (uint32_t)right << (b & 31);
-  prefetch(w, rc.evb, rc.kmb);
+  prefetch(w, rc.evp, rc.kmp);
 
   const Cells c = compute_cells(s, rc.lp_step, rc.lp_stay, rc.lp_skip);
   const int lo = max(max(-K, E - (rc.n_e - 1)), 0);
@@ -580,7 +611,7 @@ __device__ __forceinline__ void restore_slots(Slots &s, Walk &w, const ReadConst
   s.km1s = p1.stdv;
   s.km1r = p1.rstdv;
   s.km1c = p1.c;
-  prefetch(w, rc.evb, rc.kmb);  // event E+1 and k-mer K+128 for the next band
+  prefetch(w, rc.evp, rc.kmp);  // event E+1 and k-mer K+128 for the next band
 }
 
 __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int r,
@@ -593,8 +624,6 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.lp_step = m.lp_step;
   rc.lp_skip = a.lp_skip;
   rc.lp_trim = a.lp_trim;
-  rc.evb = (c_u8 *)(a.ev + m.ev_off);
-  rc.kmb = (c_u8 *)(a.km + m.km_off);
   rc.evp = a.ev + m.ev_off;
   rc.kmp = a.km + m.km_off;
   rc.trace = a.trace + m.trace_off + lane;
@@ -644,7 +673,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   w.mv = 0;
   w.best = ABEA_NEG_INF;
   w.best_e = 0;
-  prefetch(w, rc.evb, rc.kmb);
+  prefetch(w, rc.evp, rc.kmp);
 
   // A whole tile of interior bands takes the fast path (over 8 bands K and E
   // grow by at most 8).  Fast tiles come in pairs that trade the two operand
@@ -1062,7 +1091,10 @@ __global__ __launch_bounds__(64 * WPB) void abea_core_kernel(KernelArgs a, int q
       __builtin_amdgcn_s_setprio(3);
     else
       __builtin_amdgcn_s_setprio(0);
-    if (m.n_k > 0 && m.n_e > 0) core_read(a, m, r, lane);
+    if (m.n_k > 0 && m.n_e > 0) {
+      stage_read(a, m, r, lane);
+      core_read(a, m, r, lane);
+    }
     if constexpr (FUSE) {
       wave_sync();  // this wave's trace, moves and end column are visible to it
       traceback_read(a, m, r, lane, overlap, em_bufs[wib], &seg_infos[wib]);
@@ -1201,10 +1233,8 @@ __global__ void abea_zero_f64_kernel(double *p, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-hipError_t launch_pre(const KernelArgs &a, int q_begin, int q_count, hipStream_t s) {
-  if (q_count <= 0) return hipSuccess;
-  const int grid = q_count < 4096 ? q_count : 4096;
-  hipLaunchKernelGGL(abea_pre_kernel, dim3(grid), dim3(256), 0, s, a, q_begin, q_count);
+hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(abea_queue_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
   return hipGetLastError();
 }
 

@@ -32,6 +32,38 @@ __device__ __forceinline__ uint32_t base_code(char c) {
   return c == 'C' ? 1u : c == 'G' ? 2u : c == 'T' ? 3u : 0u;
 }
 
+// Bases k..k+5 of the read (k in [0, len - 6]): a 4-byte and a 2-byte load
+// at the bases' own (unaligned) address, both inside the read.
+__device__ __forceinline__ void kmer_bytes(const char *S, int k, uint32_t &lo, uint32_t &hi) {
+  uint16_t h;
+  __builtin_memcpy(&lo, S + k, 4);
+  __builtin_memcpy(&h, S + k + 4, 2);
+  hi = h;
+}
+
+// Base codes of four bases packed in a word (A,C,G,T -> 0..3, anything else
+// -> 0, as GPU/align_cuda.cl:22-35), two bits per byte, without branches:
+// ((c >> 1) ^ (c >> 2)) & 3 is the code of A, C, G and T, and an exact
+// per-byte zero test of c ^ base keeps it only for those four.
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t y) {
+  return ~(((y & 0x7f7f7f7fu) + 0x7f7f7f7fu) | y | 0x7f7f7f7fu);  // 0x80 where a byte is 0
+}
+
+__device__ __forceinline__ uint32_t base_codes4(uint32_t x) {
+  const uint32_t code = ((x >> 1) ^ (x >> 2)) & 0x03030303u;
+  const uint32_t ok = zero_bytes(x ^ 0x41414141u) | zero_bytes(x ^ 0x43434343u) |
+                      zero_bytes(x ^ 0x47474747u) | zero_bytes(x ^ 0x54545454u);
+  return code & ((ok >> 7) * 3u);
+}
+
+// 6-mer rank of bases 0-3 (lo) and 4-5 (hi), first base most significant
+// (GPU/align_cuda.cl:39-51); the multiply gathers the four 2-bit codes of lo
+// into bits 24-31 in reverse byte order.
+__device__ __forceinline__ uint32_t rank_lohi(uint32_t lo, uint32_t hi) {
+  const uint32_t c4 = base_codes4(lo), c2 = base_codes4(hi);
+  return (((c4 * 0x40100401u) >> 24) << 4) | ((c2 & 3u) << 2) | ((c2 >> 8) & 3u);
+}
+
 // Wave-wide scans and reductions on DPP (row shifts within 16-lane rows, then
 // the row broadcasts of lanes 15 and 31), no LDS round trips.
 
