@@ -734,17 +734,18 @@ abea_status_t abea_plan_autotune(abea_plan_t *p, const abea_output_t *out, int r
   if (st) return st;
   if (reps < 1) reps = 1;
   const abea_tuning_t saved = p->tune;
-  auto time_forward = [&](const abea_tuning_t &t, double *ms) -> abea_status_t {
+  // the fastest of n forwards: launch noise only ever adds time
+  auto time_forward = [&](const abea_tuning_t &t, int n, double *ms) -> abea_status_t {
     p->tune = t;
-    double acc = 0;
-    for (int r = 0; r < reps; ++r) {
+    double best_run = 1e30;
+    for (int r = 0; r < n; ++r) {
       abea_status_t s2 = abea_plan_forward(p, out);
       if (s2) return s2;
       double tm[4];
       if ((s2 = abea_plan_last_times(p, tm))) return s2;
-      acc += tm[0] + tm[1] + tm[2];
+      best_run = std::min(best_run, tm[0] + tm[1] + tm[2]);
     }
-    *ms = acc / reps;
+    *ms = best_run;
     return ABEA_OK;
   };
   struct Cand { int wpb, wpc, fuse, overlap; };
@@ -752,8 +753,12 @@ abea_status_t abea_plan_autotune(abea_plan_t *p, const abea_output_t *out, int r
                         {4, 32, 0, 128}, {1, 16, 1, 128}, {1, 24, 1, 128}, {1, 32, 1, 128},
                         {2, 32, 1, 128}, {4, 32, 1, 128}, {1, 24, 0, 64},  {1, 24, 0, 256},
                         {1, 24, 1, 64},  {1, 24, 1, 256}};
-  double best = 1e30;
-  abea_tuning_t best_t = saved;
+  // every candidate `reps` times, then the three fastest `reps` + 3 times more
+  struct Timed {
+    abea_tuning_t t;
+    double ms;
+  };
+  std::vector<Timed> timed;
   for (const Cand &c : cands) {
     abea_tuning_t t = saved;
     t.use_graph = 0;
@@ -762,15 +767,29 @@ abea_status_t abea_plan_autotune(abea_plan_t *p, const abea_output_t *out, int r
     t.fuse_traceback = c.fuse;
     t.traceback_overlap = c.overlap;
     double ms = 0;
-    if ((st = time_forward(t, &ms))) {
+    if ((st = time_forward(t, reps, &ms))) {
       p->tune = saved;
       return st;
     }
-    if (ms < best) {
-      best = ms;
-      best_t = t;
-    }
+    timed.push_back({t, ms});
   }
+  std::stable_sort(timed.begin(), timed.end(),
+                   [](const Timed &x, const Timed &y) { return x.ms < y.ms; });
+  for (size_t i = 0; i < std::min<size_t>(3, timed.size()); ++i) {
+    double ms = 0;
+    if ((st = time_forward(timed[i].t, reps + 3, &ms))) {
+      p->tune = saved;
+      return st;
+    }
+    timed[i].ms = std::min(timed[i].ms, ms);
+  }
+  double best = 1e30;
+  abea_tuning_t best_t = saved;
+  for (size_t i = 0; i < std::min<size_t>(3, timed.size()); ++i)
+    if (timed[i].ms < best) {
+      best = timed[i].ms;
+      best_t = timed[i].t;
+    }
   best_t.use_graph = saved.use_graph;
   p->tune = best_t;
   if (best_ms) *best_ms = best;

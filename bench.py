@@ -1,7 +1,8 @@
 """ABEA benchmark on one MI355X.
 
-Headline: batched persistent ABEA (length-sorted queue, persistent core,
-pre->core->post replayed as one hipGraph, autotuned launch shapes) over the
+Headline: batched persistent ABEA (length-sorted queue, persistent core that
+stages, walks and traces back each read, replayed as a hipGraph, autotuned
+launch shapes) over the
 16,384-read long-tailed workload (log-normal median 8 kb, clipped 500 b-200 kb,
 ~150 Mbases, ~40 G nominal cells), inputs resident in HBM as f5c event_t
 records.  A "step" is one full forward over that batch.
@@ -31,9 +32,14 @@ W = 100
 # HBM bytes per launch of the core kernel on the headline workload, from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings
 # (profiles/abea_kernels_batched16384_ln8kb.md gives the runs), for the
-# fused-traceback launch shapes the autotuner picks here.
-TRAFFIC_BYTES = {"abea_core_kernel<4, true>": 44133278310,
-                 "abea_core_kernel<1, true>": 44455266202}
+# fused-traceback launch shape the autotuner picks here.
+TRAFFIC_BYTES = {"abea_core_kernel<1, true>": 58959486874}
+# VALU wave-instructions per launch of the same kernel on the same workload
+# (rocprofv3 SQ_INSTS_VALU, same file).  The core is bound by VALU issue: a
+# SIMD issues one wave64 VALU instruction per 4 cycles.
+VALU_INSTS = {"abea_core_kernel<1, true>": 2.189e10}
+N_SIMD = 1024          # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9       # peak engine clock
 
 
 def log(*a):
@@ -219,7 +225,7 @@ def main():
         plan.forward()
         kt += np.array(plan.times()[:3])
     kt /= args.steps
-    log(f"[bench] kernel ms pre/core/post: {kt.round(3).tolist()}")
+    log(f"[bench] kernel ms queue_reset/core/post: {kt.round(3).tolist()}")
     res = plan.results()
     qc_pass = int((res["n_pairs"] > 0).sum())
     walk_b, tb_b = alg_bytes(b, res["path_len"])
@@ -243,7 +249,8 @@ def main():
                    "bases": int(b.read_len.sum()), "events": int(b.n_events.sum()),
                    "nominal_cells": cells, "bandwidth": W, "tuning": tune},
         "reads_per_s": round(b.n_reads / sec, 1),
-        "kernel_ms": {"pre": round(kt[0], 3), "core": round(kt[1], 3), "post": round(kt[2], 3)},
+        "kernel_ms": {"queue_reset": round(kt[0], 3), "core": round(kt[1], 3),
+                      "post": round(kt[2], 3)},
         "qc_pass_reads": qc_pass,
         "roofline": {"bound": "hbm", "kernel": kernel_name(tune), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -251,6 +258,16 @@ def main():
                      "traffic": TRAFFIC_BYTES.get(kernel_name(tune)),
                      "alg_bytes_per_launch": core_bytes},
     }
+    vi = VALU_INSTS.get(kernel_name(tune))
+    if vi is not None:
+        # VALU issue rate against the ceiling of one wave instruction per SIMD
+        # per 4 cycles at the peak clock
+        rate = vi / (kt[1] * 1e-3)
+        peak = N_SIMD * CLOCK_HZ / 4
+        out["issue"] = {"bound": "valu", "valu_insts_per_launch": vi,
+                        "valu_insts_per_band": round(vi / int(b.n_bands.sum()), 2),
+                        "achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
+                        "unit": "G wave-instructions/s", "frac": round(rate / peak, 4)}
     del plan, db
     torch.cuda.empty_cache()
 

@@ -273,8 +273,10 @@ typedef struct {
 
 abea_status_t abea_plan_get_tuning(const abea_plan_t *plan, abea_tuning_t *t);
 abea_status_t abea_plan_set_tuning(abea_plan_t *plan, const abea_tuning_t *t);
-/* Time the candidate configurations on this plan (runs forwards) and keep
- * the fastest; best_ms receives its time. */
+/* Time the candidate configurations on this plan (runs forwards: each
+ * candidate reps times, the three fastest reps + 3 times more, a candidate's
+ * time being its fastest run) and keep the fastest; best_ms receives its
+ * time. */
 abea_status_t abea_plan_autotune(abea_plan_t *plan, const abea_output_t *out, int reps,
                                  double *best_ms);
 
@@ -324,7 +326,8 @@ abea_status_t abea_plan_postalign(abea_plan_t *plan, const abea_output_t *fwd,
 abea_status_t abea_plan_estimate_scalings(abea_plan_t *plan, abea_scalings_t *out);
 
 /* Per-kernel device time (ms) of the last forward/adjoint, hipEvent-timed
- * on the plan's stream: [0]=pre [1]=core [2]=post [3]=adjoint. */
+ * on the plan's stream: [0]=queue reset (batched form; the core stages each
+ * read itself) [1]=core [2]=post [3]=adjoint. */
 abea_status_t abea_plan_last_times(const abea_plan_t *plan, double times_ms[4]);
 /* Bytes of device workspace held by the plan. */
 int64_t abea_plan_workspace_bytes(const abea_plan_t *plan);
