@@ -93,6 +93,8 @@ EXPORTS = [
     "abea_plan_refine_step", "abea_plan_refine", "abea_estimate_scalings_using_mom",
     "abea_postalign", "abea_recalibrate_model", "abea_estimate_scalings_db", "abea_postalign_db",
     "abea_kmer_offsets", "abea_plan_postalign", "abea_plan_estimate_scalings",
+    "abea_getevents", "abea_getevents_db", "abea_getevents_device",
+    "abea_ctx_set_getevents_overlap", "abea_ctx_getevents_reruns",
 ]
 
 _lib = None
@@ -173,6 +175,16 @@ def lib():
     L.abea_plan_postalign.argtypes = [P, C.POINTER(OutputT), C.POINTER(PostalignOutputT), C.c_int]
     L.abea_plan_estimate_scalings.restype = C.c_int
     L.abea_plan_estimate_scalings.argtypes = [P, P]
+    L.abea_getevents.restype = EventTable
+    L.abea_getevents.argtypes = [C.c_size_t, P]
+    L.abea_getevents_db.restype = C.c_int
+    L.abea_getevents_db.argtypes = [C.c_int32, P, P, P]
+    L.abea_getevents_device.restype = C.c_int
+    L.abea_getevents_device.argtypes = [P, C.c_int32, P, P, P, P]
+    L.abea_ctx_set_getevents_overlap.restype = C.c_int
+    L.abea_ctx_set_getevents_overlap.argtypes = [P, C.c_int32]
+    L.abea_ctx_getevents_reruns.restype = C.c_int64
+    L.abea_ctx_getevents_reruns.argtypes = [P]
     _lib = L
     return L
 

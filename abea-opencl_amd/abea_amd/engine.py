@@ -392,3 +392,41 @@ def pack_moves(bits_per_read: list, read_len, n_events) -> np.ndarray:
         total += words
     assert len(off) == len(out_parts)
     return np.concatenate(out_parts) if out_parts else np.zeros(0, np.uint32)
+
+
+# f5c event_t as a numpy record (u64 start, f32 length, mean, stdv, 4 B pad)
+EVENT_DTYPE = np.dtype({"names": ["start", "length", "mean", "stdv"],
+                        "formats": ["<u8", "<f4", "<f4", "<f4"],
+                        "offsets": [0, 8, 12, 16], "itemsize": 24})
+
+
+def getevents(ctx: Context, raw: torch.Tensor, raw_off: np.ndarray,
+              events: torch.Tensor | None = None, n_events: torch.Tensor | None = None):
+    """f5c getevents() (GPU/f5cmisc.h:44) on signals resident in device memory:
+    read i is raw[raw_off[i]:raw_off[i+1]] (float32 pA).  Returns (events,
+    n_events): a uint8 tensor of 24-byte event_t records, read i's at record
+    raw_off[i] + i, and an int64 count per read (both on the device)."""
+    raw_off = np.ascontiguousarray(raw_off, np.int64)
+    n = raw_off.shape[0] - 1
+    if raw.dtype != torch.float32 or not raw.is_cuda:
+        raise _lib.AbeaError("getevents: raw must be a float32 device tensor")
+    cap = int(raw_off[-1]) + n
+    if events is None:
+        events = torch.empty(max(cap, 1) * 24, dtype=torch.uint8, device=raw.device)
+    if n_events is None:
+        n_events = torch.empty(max(n, 1), dtype=torch.int64, device=raw.device)
+    ctx.enter()
+    _lib.check(_lib.lib().abea_getevents_device(ctx.handle, n, _nptr(raw_off), _ptr(raw),
+                                                _ptr(events), _ptr(n_events)),
+               "abea_getevents_device")
+    ctx.leave()
+    return events, n_events
+
+
+def getevents_results(events: torch.Tensor, n_events: torch.Tensor, raw_off: np.ndarray) -> list:
+    """Per-read EVENT_DTYPE arrays from getevents' device outputs."""
+    raw_off = np.asarray(raw_off, np.int64)
+    n = raw_off.shape[0] - 1
+    rec = events.cpu().numpy().view(EVENT_DTYPE)
+    cnt = n_events.cpu().numpy()[:n]
+    return [rec[raw_off[i] + i: raw_off[i] + i + cnt[i]].copy() for i in range(n)]

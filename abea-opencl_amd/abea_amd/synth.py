@@ -228,6 +228,29 @@ def workload(name: str, seed: int = 0, n: int | None = None, with_aos: bool = Fa
                     skip=WORKLOADS[name].get("skip", 0.1))
 
 
+def raw_signal(b: Batch, seed: int = 0):
+    """A raw signal (float32 pA) for the batch's events, the input of
+    getevents: each event lasts floor(U(0.002, 0.01) s * 4000 Hz) samples
+    (its recorded length when the batch has them) of its mean plus N(0, sd)
+    noise, sd ~ U(1, 3) (its recorded stdv when present).  Returns (raw,
+    raw_off) with read i at raw[raw_off[i]:raw_off[i+1]]."""
+    rng = np.random.default_rng(seed + 0x5161)
+    ne = b.ev_mean.shape[0]
+    if b.ev_length is not None:
+        lens = b.ev_length.astype(np.int64)
+    else:
+        lens = np.floor(rng.uniform(0.002, 0.01, ne) * SAMPLE_RATE).astype(np.int64)
+    sd = b.ev_stdv if b.ev_stdv is not None else rng.uniform(1.0, 3.0, ne).astype(np.float32)
+    raw = np.repeat(b.ev_mean, lens)
+    raw += np.repeat(sd, lens) * rng.standard_normal(raw.shape[0], dtype=np.float32)
+    per_read = np.zeros(b.n_reads, np.int64)
+    read_of_event = np.repeat(np.arange(b.n_reads), b.n_events)
+    np.add.at(per_read, read_of_event, lens)
+    raw_off = np.zeros(b.n_reads + 1, np.int64)
+    np.cumsum(per_read, out=raw_off[1:])
+    return raw.astype(np.float32), raw_off
+
+
 def perturb_scalings(b: Batch, seed: int = 0, shift_pm: float = 5.0, scale_pm: float = 0.10):
     """The refinement workload's start: each read's shift moved by
     U(-shift_pm, shift_pm) and its scale by a factor 1 + U(-scale_pm,

@@ -86,12 +86,22 @@ void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_
 
 }  // namespace
 
+// Event-detection scratch of a context (grows, never shrinks).
+struct GeScratch {
+  DevBuf raw_off, chunk_off, chunk_read, S, Q, t1, t2, peaks, n_peaks, entry, exit, reruns;
+  std::vector<int64_t> raw_off_h, chunk_off_h;
+  std::vector<int32_t> chunk_read_h;
+  int32_t n_reads = 0;
+};
+
 struct abea_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int n_cu = 256;
   std::string arch;
+  int32_t ge_overlap = abea::kGeOverlap;
+  GeScratch ge;
 };
 
 struct abea_plan {
@@ -187,11 +197,120 @@ abea_status_t abea_ctx_create(abea_ctx_t **out, int device, void *stream) {
 
 void abea_ctx_destroy(abea_ctx_t *c) {
   if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);  // work on the context's scratch
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
 void *abea_ctx_stream(abea_ctx_t *c) { return c ? (void *)c->stream : nullptr; }
+
+}  // extern "C"
+
+namespace {
+
+// getevents over n signals in device memory (see abea_getevents_device).
+abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, const float *raw,
+                         abea_event_t *events, int64_t *n_events) {
+  if (n < 0 || (n > 0 && (!raw_off_h || !events || !n_events)))
+    return fail(ABEA_ERR_INVALID, "getevents: null argument or negative read count");
+  if (n == 0) return ABEA_OK;
+  if (raw_off_h[0] < 0) return fail(ABEA_ERR_INVALID, "getevents: negative raw offset");
+  GeScratch &g = ctx->ge;
+  g.raw_off_h.assign(raw_off_h, raw_off_h + n + 1);
+  g.chunk_off_h.assign((size_t)n + 1, 0);
+  g.chunk_read_h.clear();
+  for (int32_t r = 0; r < n; ++r) {
+    const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
+    if (len < 0 || len >= ((int64_t)1 << 31))
+      return fail(ABEA_ERR_INVALID, "getevents: read %d has %lld samples (need 0 .. 2^31-1)", r,
+                  (long long)len);
+    const int64_t nch = (len + abea::kGeChunk - 1) / abea::kGeChunk;
+    g.chunk_off_h[r + 1] = g.chunk_off_h[r] + nch;
+    for (int64_t c = 0; c < nch; ++c) g.chunk_read_h.push_back(r);
+  }
+  if (!raw && raw_off_h[n] > 0) return fail(ABEA_ERR_INVALID, "getevents: null raw signal");
+  const int64_t nc = g.chunk_off_h[n];
+  const int64_t extent = raw_off_h[n];
+  hipStream_t s = ctx->stream;
+  abea_status_t st;
+  if ((st = g.raw_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
+      (st = g.chunk_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
+      (st = g.chunk_read.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nc, 1))) ||
+      (st = g.S.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
+      (st = g.Q.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
+      (st = g.t1.ensure(sizeof(float) * (size_t)std::max<int64_t>(extent, 1))) ||
+      (st = g.t2.ensure(sizeof(float) * (size_t)std::max<int64_t>(extent, 1))) ||
+      (st = g.peaks.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nc, 1) * abea::kGeChunk)) ||
+      (st = g.n_peaks.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nc, 1))) ||
+      (st = g.entry.ensure(sizeof(abea::GeState) * (size_t)std::max<int64_t>(nc, 1))) ||
+      (st = g.exit.ensure(sizeof(abea::GeState) * (size_t)std::max<int64_t>(nc, 1))) ||
+      (st = g.reruns.ensure(sizeof(int32_t) * (size_t)n)))
+    return st;
+  HIP_TRY(hipMemcpyAsync(g.raw_off.p, g.raw_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(g.chunk_off.p, g.chunk_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
+                         hipMemcpyHostToDevice, s));
+  if (nc)
+    HIP_TRY(hipMemcpyAsync(g.chunk_read.p, g.chunk_read_h.data(), sizeof(int32_t) * (size_t)nc,
+                           hipMemcpyHostToDevice, s));
+  abea::GeArgs a{};
+  a.n_reads = n;
+  a.overlap = ctx->ge_overlap;
+  a.n_chunks = nc;
+  a.raw_off = g.raw_off.as<int64_t>();
+  a.raw = raw;
+  a.S = g.S.as<double>();
+  a.Q = g.Q.as<double>();
+  a.t1 = g.t1.as<float>();
+  a.t2 = g.t2.as<float>();
+  a.chunk_off = g.chunk_off.as<int64_t>();
+  a.chunk_read = g.chunk_read.as<int32_t>();
+  a.peaks = g.peaks.as<int32_t>();
+  a.n_peaks = g.n_peaks.as<int32_t>();
+  a.entry = g.entry.as<abea::GeState>();
+  a.exit = g.exit.as<abea::GeState>();
+  a.reruns = g.reruns.as<int32_t>();
+  a.events = events;
+  a.n_events = n_events;
+  HIP_TRY(abea::launch_getevents(a, s));
+  g.n_reads = n;
+  return ABEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+abea_status_t abea_getevents_device(abea_ctx_t *ctx, int32_t n_reads, const int64_t *raw_off_host,
+                                    const float *raw, abea_event_t *events, int64_t *n_events) {
+  if (!ctx) return fail(ABEA_ERR_INVALID, "abea_getevents_device: null context");
+  HIP_TRY(hipSetDevice(ctx->device));
+  return ge_enqueue(ctx, n_reads, raw_off_host, raw, events, n_events);
+}
+
+abea_status_t abea_ctx_set_getevents_overlap(abea_ctx_t *ctx, int32_t samples) {
+  if (!ctx) return fail(ABEA_ERR_INVALID, "abea_ctx_set_getevents_overlap: null context");
+  if (samples < 0 || samples > abea::kGeChunk)
+    return fail(ABEA_ERR_INVALID, "getevents overlap %d outside 0..%d", samples, abea::kGeChunk);
+  ctx->ge_overlap = samples;
+  return ABEA_OK;
+}
+
+int64_t abea_ctx_getevents_reruns(abea_ctx_t *ctx) {
+  if (!ctx || ctx->ge.n_reads <= 0) return 0;
+  std::vector<int32_t> h((size_t)ctx->ge.n_reads);
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipMemcpyAsync(h.data(), ctx->ge.reruns.p, sizeof(int32_t) * h.size(),
+                     hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    fail(ABEA_ERR_HIP, "abea_ctx_getevents_reruns: readback failed");
+    return -1;
+  }
+  int64_t t = 0;
+  for (int32_t v : h) t += v;
+  return t;
+}
 
 }  // extern "C"
 
@@ -809,6 +928,8 @@ struct HostRun {
       path_len, score;
   // post-align and MOM entries
   DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms, pstat, items;
+  // getevents entries
+  DevBuf ge_raw, ge_ev, ge_nev;
   ~HostRun() {
     delete plan;
     abea_ctx_destroy(ctx);
@@ -1161,6 +1282,69 @@ abea_scalings_t abea_estimate_scalings_using_mom(char *sequence, int32_t sequenc
   char *rd[1] = {sequence};
   if (run_mom(1, rd, &sequence_len, &et, pore_model, &out)) out.scale = out.shift = std::nanf("");
   return out;
+}
+
+abea_status_t abea_getevents_db(int32_t n_reads, const size_t *nsample, float *const *rawptr,
+                                abea_event_table *et) {
+  if (n_reads < 0 || (n_reads > 0 && (!nsample || !rawptr || !et)))
+    return fail(ABEA_ERR_INVALID, "getevents_db: null argument or negative read count");
+  for (int32_t i = 0; i < n_reads; ++i) {
+    et[i] = abea_event_table{0, 0, 0, nullptr};
+    if (nsample[i] && !rawptr[i]) return fail(ABEA_ERR_INVALID, "getevents: read %d: null rawptr", i);
+    if (nsample[i] >= ((size_t)1 << 31))
+      return fail(ABEA_ERR_INVALID, "getevents: read %d has %zu samples (need < 2^31)", i,
+                  nsample[i]);
+  }
+  if (n_reads == 0) return ABEA_OK;
+  HostRun *r = nullptr;
+  abea_status_t st = host_run_get(&r);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(r->ctx->device));
+  hipStream_t s = r->ctx->stream;
+  std::vector<int64_t> off((size_t)n_reads + 1, 0);
+  for (int32_t i = 0; i < n_reads; ++i) off[i + 1] = off[i] + (int64_t)nsample[i];
+  const int64_t total = off[n_reads];
+  if ((st = r->ge_raw.ensure(sizeof(float) * (size_t)std::max<int64_t>(total, 1))) ||
+      (st = r->ge_ev.ensure(sizeof(abea_event_t) * (size_t)(total + n_reads))) ||
+      (st = r->ge_nev.ensure(sizeof(int64_t) * (size_t)n_reads)))
+    return st;
+  for (int32_t i = 0; i < n_reads; ++i)
+    if (nsample[i])
+      HIP_TRY(hipMemcpyAsync(r->ge_raw.as<float>() + off[i], rawptr[i], sizeof(float) * nsample[i],
+                             hipMemcpyHostToDevice, s));
+  if ((st = ge_enqueue(r->ctx, n_reads, off.data(), r->ge_raw.as<float>(),
+                       r->ge_ev.as<abea_event_t>(), r->ge_nev.as<int64_t>())))
+    return st;
+  std::vector<int64_t> cnt((size_t)n_reads);
+  HIP_TRY(hipMemcpyAsync(cnt.data(), r->ge_nev.p, sizeof(int64_t) * (size_t)n_reads,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < n_reads; ++i) {
+    if (cnt[i] <= 0) continue;
+    abea_event_t *e = (abea_event_t *)malloc(sizeof(abea_event_t) * (size_t)cnt[i]);
+    if (!e) {
+      for (int32_t j = 0; j < i; ++j) {
+        free(et[j].event);
+        et[j] = abea_event_table{0, 0, 0, nullptr};
+      }
+      return fail(ABEA_ERR_NOMEM, "getevents: malloc of %lld events failed", (long long)cnt[i]);
+    }
+    HIP_TRY(hipMemcpyAsync(e, r->ge_ev.as<abea_event_t>() + off[i] + i,
+                           sizeof(abea_event_t) * (size_t)cnt[i], hipMemcpyDeviceToHost, s));
+    et[i] = abea_event_table{(size_t)cnt[i], 0, (size_t)cnt[i], e};
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return ABEA_OK;
+}
+
+abea_event_table abea_getevents(size_t nsample, float *rawptr) {
+  abea_event_table et{0, 0, 0, nullptr};
+  if (nsample && !rawptr) {
+    fail(ABEA_ERR_INVALID, "getevents: null rawptr");
+    return et;
+  }
+  if (abea_getevents_db(1, &nsample, &rawptr, &et)) et = abea_event_table{0, 0, 0, nullptr};
+  return et;
 }
 
 abea_status_t abea_estimate_scalings_db(int32_t n_bam_rec, char **read, const int32_t *read_len,

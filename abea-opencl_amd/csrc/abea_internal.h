@@ -123,6 +123,37 @@ struct MomArgs {
   abea_scalings_t *out;
 };
 
+// Event detection (getevents), one read per raw signal.
+constexpr int kGeChunk = 2048;   // samples per detector chunk
+constexpr int kGeOverlap = 256;  // samples a chunk's speculative run starts early
+struct GeDet {
+  int32_t pos;        // -1: no peak yet
+  float val;          // running minimum, or the peak's value
+  int32_t valid;
+  int32_t masked_to;  // samples i <= masked_to are skipped
+};
+struct GeState {
+  GeDet d[2];  // short, long
+};
+struct GeArgs {
+  int32_t n_reads;
+  int32_t overlap;
+  int64_t n_chunks;
+  const int64_t *raw_off;     // n_reads + 1: read r is raw[raw_off[r] .. raw_off[r+1])
+  const float *raw;
+  double *S, *Q;              // prefix sums, read r at raw_off[r] + r (n + 1 entries)
+  float *t1, *t2;             // t-statistics, read r at raw_off[r]
+  const int64_t *chunk_off;   // n_reads + 1: read r's chunks
+  const int32_t *chunk_read;  // the read of each chunk
+  int32_t *peaks;             // kGeChunk boundaries per chunk, in emission order
+  int32_t *n_peaks;           // per chunk
+  GeState *entry, *exit;      // per chunk: state before its first and after its last sample
+  int32_t *reruns;            // per read: chunks re-run from the true state
+  abea_event_t *events;       // read r at raw_off[r] + r (room for n + 1)
+  int64_t *n_events;          // per read
+};
+hipError_t launch_getevents(const GeArgs &a, hipStream_t s);
+
 // flag bit for a read whose pairs are not an alignment path (kernel guard)
 constexpr int32_t kBadPath = 0x100;
 

@@ -175,6 +175,23 @@ abea_status_t abea_postalign_db(int32_t n_bam_rec, char **read, const int32_t *r
                                 abea_index_pair_t **base_to_event_map, int32_t *read_stat_flag);
 
 /* ---------------------------------------------------------------------- */
+/* (1c) f5c getevents(), GPU/f5cmisc.h:44: event detection on one read's raw
+ * signal in picoamperes (f5c's rawptr), by scrappie's DNA detector as f5c
+ * ports it (the event types are scrappie's, GPU/f5c.h:112-130): two windowed
+ * t-statistics (w = 3 and 6) over double prefix sums, the short/long peak
+ * detector (thresholds 1.4 and 9.0, peak height 0.2), one event between
+ * consecutive boundaries.  Returns the event table with a malloc()ed event
+ * array that the caller frees, start = 0 and end = n, as getevents does; a
+ * signal with no boundary is one event (nsample = 0: no events).  On error:
+ * n = 0, event = NULL, abea_last_error() set.  nsample < 2^31. */
+abea_event_table abea_getevents(size_t nsample, float *rawptr);
+
+/* getevents over a batch in host buffers (signal i: rawptr[i], nsample[i]);
+ * et[i] as abea_getevents returns it. */
+abea_status_t abea_getevents_db(int32_t n_reads, const size_t *nsample, float *const *rawptr,
+                                abea_event_table *et);
+
+/* ---------------------------------------------------------------------- */
 /* (3) Device-resident batches.  All pointers in abea_batch_t and
  * abea_output_t are device pointers except the two *_host arrays. */
 
@@ -186,6 +203,23 @@ typedef struct abea_plan abea_plan_t;
 abea_status_t abea_ctx_create(abea_ctx_t **ctx, int device, void *stream);
 void abea_ctx_destroy(abea_ctx_t *ctx);
 void *abea_ctx_stream(abea_ctx_t *ctx);
+
+/* getevents on signals already in device memory (new; GPU/f5cmisc.h:44 per
+ * read).  Read i is raw[raw_off_host[i] .. raw_off_host[i+1]) (raw device,
+ * raw_off_host a host array of n_reads + 1 non-decreasing offsets, each
+ * read < 2^31 samples); its events go to events[raw_off_host[i] + i ...]
+ * (device, room for nsample + 1 each) and their count to n_events[i]
+ * (device).  Enqueued on the context's stream; the context keeps the
+ * scratch (about 40 bytes per sample). */
+abea_status_t abea_getevents_device(abea_ctx_t *ctx, int32_t n_reads,
+                                    const int64_t *raw_off_host, const float *raw,
+                                    abea_event_t *events, int64_t *n_events);
+/* Samples each detector chunk's speculative run starts before the chunk
+ * (default 256, 0..2048).  Results do not depend on it, only how many
+ * chunks are re-run from the true state; abea_ctx_getevents_reruns() gives
+ * that count for the last abea_getevents_device() call (synchronizes). */
+abea_status_t abea_ctx_set_getevents_overlap(abea_ctx_t *ctx, int32_t samples);
+int64_t abea_ctx_getevents_reruns(abea_ctx_t *ctx);
 
 typedef struct {
   int32_t n_reads;

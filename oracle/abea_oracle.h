@@ -134,6 +134,24 @@ orc_scalings_t orc_estimate_scalings_mom(const char *sequence, int32_t sequence_
                                          const orc_model_t *model, const float *ev_mean,
                                          int32_t n_events);
 
+/* ---- the step before: getevents.c ---------------------------------------- */
+typedef struct { /* event_t, GPU/f5c.h:113-121 (24 bytes with tail padding) */
+  uint64_t start;
+  float length, mean, stdv;
+} orc_event_t;
+
+/* Prefix sums of a signal and of its (float) squares, n + 1 entries each. */
+void orc_signal_prefix(const float *x, int64_t n, double *S, double *Q);
+/* The windowed t-statistic of width w over the prefix sums, n entries. */
+void orc_tstat(const double *S, const double *Q, int64_t n, int32_t w, float *t);
+/* The short/long peak detector: boundaries in emission order, returns their
+ * count (peaks has room for n). */
+int64_t orc_detect_peaks(const float *t_short, const float *t_long, int64_t n, int64_t *peaks);
+/* f5c getevents(), GPU/f5cmisc.h:44: the events of n raw samples.  Returns
+ * the event count, minus the count when it exceeds cap, 0 when allocation
+ * fails. */
+int64_t orc_getevents(const float *raw, int64_t n, orc_event_t *out, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,14 @@ def lib():
         L.orc_recalibrate_model.argtypes = [P, P, P, P, C.c_int32, C.c_int]
         L.orc_estimate_scalings_mom.restype = Scalings
         L.orc_estimate_scalings_mom.argtypes = [P, C.c_int32, P, P, C.c_int32]
+        L.orc_signal_prefix.restype = None
+        L.orc_signal_prefix.argtypes = [P, C.c_int64, P, P]
+        L.orc_tstat.restype = None
+        L.orc_tstat.argtypes = [P, P, C.c_int64, C.c_int32, P]
+        L.orc_detect_peaks.restype = C.c_int64
+        L.orc_detect_peaks.argtypes = [P, P, C.c_int64, P]
+        L.orc_getevents.restype = C.c_int64
+        L.orc_getevents.argtypes = [P, C.c_int64, P, C.c_int64]
         L.orc_align_batch.restype = C.c_int
         L.orc_align_batch.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int]
         _lib = L
@@ -282,3 +290,53 @@ def process_read(seq: bytes, ev_mean, model, scalings, scale_var: bool = True):
     if not ok or new[2] > MIN_CALIBRATION_VAR:
         out["flag"] = FAILED_CALIBRATION
     return out
+
+
+# ---- the step before align(): event detection (getevents.c) ----------------
+# f5c event_t, GPU/f5c.h:113-121: u64 start, f32 length, mean, stdv, 4 B pad
+EVENT_DTYPE = np.dtype({"names": ["start", "length", "mean", "stdv"],
+                        "formats": ["<u8", "<f4", "<f4", "<f4"],
+                        "offsets": [0, 8, 12, 16], "itemsize": 24})
+
+
+def signal_prefix(x):
+    """Prefix sums (S, Q) of a raw signal and of its float squares, in
+    double, n + 1 entries each."""
+    x = np.ascontiguousarray(x, np.float32)
+    S = np.zeros(x.shape[0] + 1, np.float64)
+    Q = np.zeros_like(S)
+    lib().orc_signal_prefix(_p(x), int(x.shape[0]), _p(S), _p(Q))
+    return S, Q
+
+
+def tstat(S, Q, w: int):
+    """The windowed t-statistic of width w over the prefix sums."""
+    S = np.ascontiguousarray(S, np.float64)
+    Q = np.ascontiguousarray(Q, np.float64)
+    n = S.shape[0] - 1
+    t = np.zeros(max(n, 0), np.float32)
+    lib().orc_tstat(_p(S), _p(Q), n, int(w), _p(t))
+    return t
+
+
+def detect_peaks(t_short, t_long):
+    """Event boundaries of the short/long peak detector, in emission order."""
+    a = np.ascontiguousarray(t_short, np.float32)
+    b = np.ascontiguousarray(t_long, np.float32)
+    pk = np.zeros(max(a.shape[0], 1), np.int64)
+    n = lib().orc_detect_peaks(_p(a), _p(b), int(a.shape[0]), _p(pk))
+    return pk[:n].copy()
+
+
+def getevents(raw):
+    """f5c getevents() (GPU/f5cmisc.h:44): the event table of a raw signal,
+    as EVENT_DTYPE records."""
+    raw = np.ascontiguousarray(raw, np.float32)
+    n = int(raw.shape[0])
+    if n == 0:
+        return np.zeros(0, EVENT_DTYPE)
+    out = np.zeros(n + 1, EVENT_DTYPE)
+    m = lib().orc_getevents(_p(raw), n, out.ctypes.data_as(C.c_void_p), n + 1)
+    if m <= 0:
+        raise RuntimeError(f"orc_getevents failed ({m})")
+    return out[:m].copy()

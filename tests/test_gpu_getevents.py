@@ -1,0 +1,173 @@
+"""GPU parity of event detection (f5c getevents(), GPU/f5cmisc.h:44) with the
+oracle restatement (oracle/getevents.c): event tables byte for byte.
+
+Covers signal lengths around the window and chunk sizes, empty and tiny
+signals, the serial prefix-sum path (terms too fine for an exact parallel
+sum), forced speculation misses (no lead-in), dyadic step signals with
+known cuts, reads at a non-zero offset, and the host drop-ins."""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "abea-opencl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from abea_amd import _lib, engine, synth  # noqa: E402
+
+
+def _first_diff(a: np.ndarray, b: np.ndarray) -> str:
+    if a.shape != b.shape:
+        return f"shapes {a.shape} vs {b.shape}"
+    ab, bb = a.view(np.uint8).reshape(a.shape[0], -1), b.view(np.uint8).reshape(b.shape[0], -1)
+    bad = np.nonzero((ab != bb).any(1))[0]
+    return "equal" if bad.size == 0 else f"first differing event {bad[0]}: {a[bad[0]]} vs {b[bad[0]]}"
+
+
+def _events_equal(got: np.ndarray, want: np.ndarray) -> bool:
+    # compare the 20 meaningful bytes of each record (the 4 pad bytes are not
+    # part of event_t's value)
+    if got.shape != want.shape:
+        return False
+    g = got.view(np.uint8).reshape(-1, 24)[:, :20]
+    w = want.view(np.uint8).reshape(-1, 24)[:, :20]
+    return bool(np.array_equal(g, w))
+
+
+def _signals(lengths, seed):
+    """Slices of a synthetic raw signal with the given lengths."""
+    b = synth.workload("plain4096_ln5kb", seed=seed, n=16)
+    raw, off = synth.raw_signal(b, seed=seed)
+    rng = np.random.default_rng(seed)
+    out = []
+    for L in lengths:
+        s = int(rng.integers(0, raw.shape[0] - L)) if L < raw.shape[0] else 0
+        out.append(raw[s:s + L].copy())
+    return out
+
+
+def _run_device(ctx, sigs, lead=0, overlap=None):
+    lens = np.array([s.shape[0] for s in sigs], np.int64)
+    raw_off = np.zeros(len(sigs) + 1, np.int64)
+    raw_off[0] = lead
+    raw_off[1:] = lead + np.cumsum(lens)
+    flat = np.concatenate([np.full(lead, 77.0, np.float32)] + sigs) if sigs else np.zeros(1, np.float32)
+    raw = torch.from_numpy(flat).cuda()
+    if overlap is not None:
+        _lib.check(_lib.lib().abea_ctx_set_getevents_overlap(ctx.handle, overlap), "overlap")
+    ev, cnt = engine.getevents(ctx, raw, raw_off)
+    torch.cuda.synchronize()
+    res = engine.getevents_results(ev, cnt, raw_off)
+    if overlap is not None:
+        _lib.check(_lib.lib().abea_ctx_set_getevents_overlap(ctx.handle, 256), "overlap")
+    return res
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = engine.Context(0)
+    yield c
+    c.close()
+
+
+LENGTHS = [0, 1, 5, 6, 11, 12, 13, 100, 2047, 2048, 2049, 4200, 10000, 65536 + 17]
+
+
+def test_lengths_against_oracle(ctx):
+    sigs = _signals(LENGTHS, seed=3)
+    got = _run_device(ctx, sigs)
+    for L, s, g in zip(LENGTHS, sigs, got):
+        want = oracle.getevents(s)
+        print(f"[getevents] length {L}: {want.shape[0]} events", flush=True)
+        assert _events_equal(g, want), f"length {L}: {_first_diff(g, want)}"
+
+
+def test_whole_reads_and_reruns(ctx):
+    b = synth.workload("plain4096_ln5kb", seed=5, n=24)
+    raw, off = synth.raw_signal(b, seed=5)
+    sigs = [raw[off[i]:off[i + 1]] for i in range(b.n_reads)]
+    want = [oracle.getevents(s) for s in sigs]
+    for overlap in (256, 0, 17):
+        got = _run_device(ctx, sigs, overlap=overlap)
+        reruns = _lib.lib().abea_ctx_getevents_reruns(ctx.handle)
+        print(f"[getevents] overlap {overlap}: chunks re-run {reruns}", flush=True)
+        for i, (g, w) in enumerate(zip(got, want)):
+            assert _events_equal(g, w), f"overlap {overlap} read {i}: {_first_diff(g, w)}"
+        if overlap == 0:
+            assert reruns > 0  # the re-run path was exercised
+
+
+def test_serial_prefix_path(ctx):
+    # terms as fine as 2^-60 cannot be summed exactly in double over a pA
+    # signal: the prefix sums take the serial path
+    sigs = _signals([3000, 5000], seed=7)
+    for s in sigs:
+        s[::97] = np.float32(1e-18)
+    got = _run_device(ctx, sigs)
+    for s, g in zip(sigs, got):
+        assert _events_equal(g, oracle.getevents(s)), _first_diff(g, oracle.getevents(s))
+
+
+def test_step_signals_and_offset_reads(ctx):
+    rng = np.random.default_rng(9)
+    sigs, cuts = [], []
+    for n_seg in (2, 40, 300):
+        lengths = rng.integers(7, 41, n_seg)
+        levels = [100.0]
+        while len(levels) < n_seg:
+            v = round(float(rng.uniform(60, 130)) * 4) / 4
+            if abs(v - levels[-1]) >= 5:
+                levels.append(v)
+        sigs.append(np.concatenate([np.full(L, v, np.float32) for L, v in zip(lengths, levels)]))
+        cuts.append(np.cumsum(lengths)[:-1])
+    sigs.insert(1, np.zeros(0, np.float32))  # an empty read between others
+    got = _run_device(ctx, sigs, lead=5)
+    for s, g in zip(sigs, got):
+        assert _events_equal(g, oracle.getevents(s))
+    for c, g in zip(cuts, [got[0], got[2], got[3]]):
+        assert np.array_equal(g["start"][1:].astype(np.int64), c)
+
+
+def test_host_dropins(ctx):
+    sigs = _signals([9000, 0, 31, 4097], seed=11)
+    L = _lib.lib()
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    # per read
+    for s in sigs:
+        et = L.abea_getevents(s.shape[0], s.ctypes.data_as(C.c_void_p) if s.shape[0] else None)
+        want = oracle.getevents(s)
+        assert et.n == want.shape[0] and et.start == 0 and et.end == et.n
+        if et.n:
+            buf = C.string_at(C.cast(et.event, C.c_void_p), 24 * et.n)
+            got = np.frombuffer(buf, engine.EVENT_DTYPE)
+            assert _events_equal(got, want), _first_diff(got, want)
+            libc.free(C.cast(et.event, C.c_void_p))
+        else:
+            assert not et.event
+    # batch
+    n = len(sigs)
+    ns = (C.c_size_t * n)(*[s.shape[0] for s in sigs])
+    ptrs = (C.c_void_p * n)(*[s.ctypes.data if s.shape[0] else None for s in sigs])
+    ets = (_lib.EventTable * n)()
+    _lib.check(L.abea_getevents_db(n, ns, ptrs, ets), "abea_getevents_db")
+    for s, et in zip(sigs, ets):
+        want = oracle.getevents(s)
+        assert et.n == want.shape[0]
+        if et.n:
+            got = np.frombuffer(C.string_at(C.cast(et.event, C.c_void_p), 24 * et.n),
+                                engine.EVENT_DTYPE)
+            assert _events_equal(got, want)
+            libc.free(C.cast(et.event, C.c_void_p))
+    # a null signal with samples is an error: an empty table and a message
+    et = L.abea_getevents(10, None)
+    assert et.n == 0 and not et.event
+    assert b"null" in L.abea_last_error()

@@ -142,7 +142,8 @@ def _mutant(name):
     so = os.path.join(ROOT, "oracle", "_build", f"liborc_{name.lower()}.so")
     flags = ["-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", f"-D{name}"]
     subprocess.run(["gcc"] + flags + ["-o", so, os.path.join(ROOT, "oracle", "abea_oracle.c"),
-                                      os.path.join(ROOT, "oracle", "postalign.c"), "-lm"],
+                                      os.path.join(ROOT, "oracle", "postalign.c"),
+                                      os.path.join(ROOT, "oracle", "getevents.c"), "-lm"],
                    check=True)
     return so
 
