@@ -88,9 +88,10 @@ void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_
 
 // Event-detection scratch of a context (grows, never shrinks).
 struct GeScratch {
-  DevBuf raw_off, chunk_off, chunk_read, S, Q, t1, t2, peaks, n_peaks, entry, exit, reruns;
-  std::vector<int64_t> raw_off_h, chunk_off_h;
-  std::vector<int32_t> chunk_read_h;
+  DevBuf raw_off, chunk_off, chunk_read, det_off, det_read, S, Q, t1, t2, csum, carry, exact,
+      peaks, n_peaks, entry, exit, first_bad, reruns;
+  std::vector<int64_t> raw_off_h, chunk_off_h, det_off_h;
+  std::vector<int32_t> chunk_read_h, det_read_h;
   int32_t n_reads = 0;
 };
 
@@ -219,57 +220,84 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
   GeScratch &g = ctx->ge;
   g.raw_off_h.assign(raw_off_h, raw_off_h + n + 1);
   g.chunk_off_h.assign((size_t)n + 1, 0);
+  g.det_off_h.assign((size_t)n + 1, 0);
   g.chunk_read_h.clear();
+  g.det_read_h.clear();
   for (int32_t r = 0; r < n; ++r) {
     const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
     if (len < 0 || len >= ((int64_t)1 << 31))
       return fail(ABEA_ERR_INVALID, "getevents: read %d has %lld samples (need 0 .. 2^31-1)", r,
                   (long long)len);
     const int64_t nch = (len + abea::kGeChunk - 1) / abea::kGeChunk;
+    const int64_t ndc = (len + abea::kGeDetChunk - 1) / abea::kGeDetChunk;
     g.chunk_off_h[r + 1] = g.chunk_off_h[r] + nch;
-    for (int64_t c = 0; c < nch; ++c) g.chunk_read_h.push_back(r);
+    g.det_off_h[r + 1] = g.det_off_h[r] + ndc;
+    g.chunk_read_h.insert(g.chunk_read_h.end(), (size_t)nch, r);
+    g.det_read_h.insert(g.det_read_h.end(), (size_t)ndc, r);
   }
   if (!raw && raw_off_h[n] > 0) return fail(ABEA_ERR_INVALID, "getevents: null raw signal");
-  const int64_t nc = g.chunk_off_h[n];
+  const int64_t nc = g.chunk_off_h[n], nd = g.det_off_h[n];
   const int64_t extent = raw_off_h[n];
+  const size_t nc1 = (size_t)std::max<int64_t>(nc, 1), nd1 = (size_t)std::max<int64_t>(nd, 1);
   hipStream_t s = ctx->stream;
   abea_status_t st;
   if ((st = g.raw_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
       (st = g.chunk_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
-      (st = g.chunk_read.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nc, 1))) ||
+      (st = g.det_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
+      (st = g.chunk_read.ensure(sizeof(int32_t) * nc1)) ||
+      (st = g.det_read.ensure(sizeof(int32_t) * nd1)) ||
       (st = g.S.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
       (st = g.Q.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
-      (st = g.t1.ensure(sizeof(float) * (size_t)std::max<int64_t>(extent, 1))) ||
-      (st = g.t2.ensure(sizeof(float) * (size_t)std::max<int64_t>(extent, 1))) ||
-      (st = g.peaks.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nc, 1) * abea::kGeChunk)) ||
-      (st = g.n_peaks.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nc, 1))) ||
-      (st = g.entry.ensure(sizeof(abea::GeState) * (size_t)std::max<int64_t>(nc, 1))) ||
-      (st = g.exit.ensure(sizeof(abea::GeState) * (size_t)std::max<int64_t>(nc, 1))) ||
+      // per read a 16-byte aligned start and 8 floats of slack; the
+      // detector's block loads read up to 15 floats past a read's end
+      (st = g.t1.ensure(sizeof(float) * (size_t)(extent + 8 * (int64_t)n + 32))) ||
+      (st = g.t2.ensure(sizeof(float) * (size_t)(extent + 8 * (int64_t)n + 32))) ||
+      (st = g.csum.ensure(sizeof(abea::GeChunkSum) * nc1)) ||
+      (st = g.carry.ensure(2 * sizeof(double) * nc1)) ||
+      (st = g.exact.ensure(sizeof(int32_t) * (size_t)n)) ||
+      (st = g.peaks.ensure(sizeof(int32_t) * nd1 * abea::kGeDetChunk)) ||
+      (st = g.n_peaks.ensure(sizeof(int32_t) * nd1)) ||
+      (st = g.entry.ensure(sizeof(abea::GeState) * nd1)) ||
+      (st = g.exit.ensure(sizeof(abea::GeState) * nd1)) ||
+      (st = g.first_bad.ensure(sizeof(int32_t) * (size_t)n)) ||
       (st = g.reruns.ensure(sizeof(int32_t) * (size_t)n)))
     return st;
   HIP_TRY(hipMemcpyAsync(g.raw_off.p, g.raw_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(g.chunk_off.p, g.chunk_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
                          hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(g.det_off.p, g.det_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
+                         hipMemcpyHostToDevice, s));
   if (nc)
     HIP_TRY(hipMemcpyAsync(g.chunk_read.p, g.chunk_read_h.data(), sizeof(int32_t) * (size_t)nc,
+                           hipMemcpyHostToDevice, s));
+  if (nd)
+    HIP_TRY(hipMemcpyAsync(g.det_read.p, g.det_read_h.data(), sizeof(int32_t) * (size_t)nd,
                            hipMemcpyHostToDevice, s));
   abea::GeArgs a{};
   a.n_reads = n;
   a.overlap = ctx->ge_overlap;
   a.n_chunks = nc;
+  a.n_det = nd;
   a.raw_off = g.raw_off.as<int64_t>();
   a.raw = raw;
   a.S = g.S.as<double>();
   a.Q = g.Q.as<double>();
   a.t1 = g.t1.as<float>();
   a.t2 = g.t2.as<float>();
+  a.bounds = g.t1.as<int32_t>();
   a.chunk_off = g.chunk_off.as<int64_t>();
   a.chunk_read = g.chunk_read.as<int32_t>();
+  a.csum = g.csum.as<abea::GeChunkSum>();
+  a.carry = g.carry.as<double>();
+  a.exact = g.exact.as<int32_t>();
+  a.det_off = g.det_off.as<int64_t>();
+  a.det_read = g.det_read.as<int32_t>();
   a.peaks = g.peaks.as<int32_t>();
   a.n_peaks = g.n_peaks.as<int32_t>();
   a.entry = g.entry.as<abea::GeState>();
   a.exit = g.exit.as<abea::GeState>();
+  a.first_bad = g.first_bad.as<int32_t>();
   a.reruns = g.reruns.as<int32_t>();
   a.events = events;
   a.n_events = n_events;
@@ -291,8 +319,9 @@ abea_status_t abea_getevents_device(abea_ctx_t *ctx, int32_t n_reads, const int6
 
 abea_status_t abea_ctx_set_getevents_overlap(abea_ctx_t *ctx, int32_t samples) {
   if (!ctx) return fail(ABEA_ERR_INVALID, "abea_ctx_set_getevents_overlap: null context");
-  if (samples < 0 || samples > abea::kGeChunk)
-    return fail(ABEA_ERR_INVALID, "getevents overlap %d outside 0..%d", samples, abea::kGeChunk);
+  if (samples < 0 || samples > abea::kGeDetChunk)
+    return fail(ABEA_ERR_INVALID, "getevents overlap %d outside 0..%d", samples,
+                abea::kGeDetChunk);
   ctx->ge_overlap = samples;
   return ABEA_OK;
 }

@@ -6,22 +6,28 @@
 // t-statistics (w = 3 and 6), the short/long peak detector (thresholds 1.4
 // and 9.0, peak height 0.2), and events between consecutive boundaries.
 //
-//   prefix   one wave per read.  The sums are exact in any order when every
-//            term is a multiple of 2^g and the total stays below 2^(52+g)
-//            (true for pA-scale signals); then a DPP scan per 64 samples,
-//            else a serial pass in index order.
+// Every stage runs over chunks of every read at once:
+//   chunk_sum / read_scan / chunk_prefix   the prefix sums.  They are exact
+//            in any order when every term is a multiple of 2^g and the sum
+//            of magnitudes stays below 2^(52+g) (true for pA-scale
+//            signals): chunk sums, a scan of them per read, then a scan
+//            inside each chunk.  A read that fails the test takes one serial
+//            pass in index order.
 //   tstat    one thread per sample, the C source's float/double types.
-//   detect   one lane per 2048-sample chunk, started fresh 256 samples early
-//            (speculation): the chunk's entry state, its boundaries in
+//   detect   one lane per 512-sample chunk, started fresh a few samples
+//            early (speculation): the chunk's entry state, its boundaries in
 //            emission order and its exit state.
-//   verify   one lane per read walks its chunks in order: a chunk whose entry
-//            state equals the true state there is accepted as is (the
-//            detector is a deterministic state machine), otherwise it is
-//            re-run from the true state.  The result is the serial one.
-//   events   one wave per read: boundaries -> event records.
+//   check / fix   a chunk is right when its entry state is its
+//            predecessor's exit state (the detector is a deterministic state
+//            machine and the first chunk starts from the true initial
+//            state); from a read's first chunk where that fails, one lane
+//            walks on, re-running each chunk whose entry is not the true
+//            state.  The result is the serial one.
+//   compact / events   boundaries gathered per read, one thread per event.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 
 #include "abea_internal.h"
 #include "abea_wave.h"
@@ -32,6 +38,14 @@ namespace {
 
 constexpr float kThrShort = 1.4f, kThrLong = 9.0f, kPeakHeight = 0.2f;
 constexpr int kWShort = 3, kWLong = 6;
+
+// The t-statistics of read r start at toff(r): 16-byte aligned, at least 5
+// floats after the previous read's end.  The detector's block loads read up
+// to 15 floats past a read's end: into the next read's statistics, or into
+// the 32 floats allocated past the last read's.
+__device__ __forceinline__ int64_t toff(const GeArgs &a, int r) {
+  return ((a.raw_off[r] + 3) & ~(int64_t)3) + 8 * (int64_t)r;
+}
 
 // --------------------------------------------------------------------------
 // prefix sums
@@ -69,55 +83,92 @@ __device__ __forceinline__ double wave_sum_any_f64(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void abea_ge_prefix_kernel(GeArgs a) {
+// wave reductions over the exactness data of 64 values
+__device__ __forceinline__ void reduce_sum_info(GeChunkSum &c) {
+  c.s = wave_sum_any_f64(c.s);
+  c.q = wave_sum_any_f64(c.q);
+  c.abs_s = wave_sum_any_f64(c.abs_s);
+  c.abs_q = wave_sum_any_f64(c.abs_q);
+  c.gx = wave_min_i32(c.gx);
+  c.gq = wave_min_i32(c.gq);
+  c.bad = wave_max_i32(c.bad);
+}
+
+// One wave per prefix chunk (4 per block): the chunk's sums and exactness
+// data, read 64 consecutive samples per step.
+__global__ __launch_bounds__(256) void abea_ge_chunk_sum_kernel(GeArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= a.n_chunks) return;
+  const int r = a.chunk_read[c];
+  const int64_t off = a.raw_off[r];
+  const int64_t n = a.raw_off[r + 1] - off;
+  const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
+  const float *x = a.raw + off;
+  GeChunkSum cs{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, 0};
+#pragma unroll 4
+  for (int64_t i = c0 + lane; i < c1; i += 64) {
+    const float v = x[i];
+    const float sq = v * v;
+    const int ex = ulp_exp(v), eq = ulp_exp(sq);
+    cs.bad |= (ex == 2000) | (eq == 2000);
+    cs.gx = min(cs.gx, ex);
+    cs.gq = min(cs.gq, eq);
+    cs.s += (double)v;
+    cs.q += (double)sq;
+    cs.abs_s += fabs((double)v);
+    cs.abs_q += (double)sq;
+  }
+  reduce_sum_info(cs);
+  if (lane == 0) a.csum[c] = cs;
+}
+
+// One wave per read: the exactness decision, then the chunks' carries (an
+// exclusive scan of their sums, exact when the read is), or the serial
+// pass.  Also clears the read's detector mismatch marker.
+__global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (r >= a.n_reads) return;
   const int64_t off = a.raw_off[r];
   const int64_t n = a.raw_off[r + 1] - off;
-  const float *x = a.raw + off;
+  const int64_t cb = a.chunk_off[r], ce = a.chunk_off[r + 1];
   double *S = a.S + off + r, *Q = a.Q + off + r;
-  // exactness: every term (and square) a multiple of 2^g, total < 2^(52+g)
-  int gx = 1000, gq = 1000, bad = 0;
-  double ax = 0.0, aq = 0.0;
-  for (int64_t i = lane; i < n; i += 64) {
-    const float v = x[i];
-    const float sq = v * v;
-    const int ex = ulp_exp(v), eq = ulp_exp(sq);
-    bad |= (ex == 2000) | (eq == 2000);
-    gx = min(gx, ex);
-    gq = min(gq, eq);
-    ax += fabs((double)v);
-    aq += (double)sq;
+  GeChunkSum t{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, 0};
+  for (int64_t c = cb + lane; c < ce; c += 64) {
+    const GeChunkSum &k = a.csum[c];
+    t.abs_s += k.abs_s;
+    t.abs_q += k.abs_q;
+    t.gx = min(t.gx, k.gx);
+    t.gq = min(t.gq, k.gq);
+    t.bad |= k.bad;
   }
-  gx = wave_min_i32(gx);
-  gq = wave_min_i32(gq);
-  bad = wave_max_i32(bad);
-  ax = wave_sum_any_f64(ax);
-  aq = wave_sum_any_f64(aq);
+  reduce_sum_info(t);
   // the sums of magnitudes are rounded: leave a factor-2 margin
-  const bool exact = !bad && (gx >= 1000 || ax < ldexp(1.0, 51 + gx)) &&
-                     (gq >= 1000 || aq < ldexp(1.0, 51 + gq));
+  const bool exact = !t.bad && (t.gx >= 1000 || t.abs_s < ldexp(1.0, 51 + t.gx)) &&
+                     (t.gq >= 1000 || t.abs_q < ldexp(1.0, 51 + t.gq));
   if (lane == 0) {
+    a.exact[r] = exact ? 1 : 0;
+    a.first_bad[r] = INT32_MAX;
     S[0] = 0.0;
     Q[0] = 0.0;
   }
   if (exact) {
-    double cs = 0.0, cq = 0.0;  // running totals (exact)
-    for (int64_t b = 0; b < n; b += 64) {
-      const int64_t i = b + lane;
-      const float v = i < n ? x[i] : 0.0f;
-      const float sq = v * v;
-      const double ps = wave_scan_add_f64((double)v), pq = wave_scan_add_f64((double)sq);
-      if (i < n) {
-        S[i + 1] = cs + ps;
-        Q[i + 1] = cq + pq;
+    double cs = 0.0, cq = 0.0;
+    for (int64_t c0 = cb; c0 < ce; c0 += 64) {
+      const int64_t c = c0 + lane;
+      const double vs = c < ce ? a.csum[c].s : 0.0, vq = c < ce ? a.csum[c].q : 0.0;
+      const double ps = wave_scan_add_f64(vs), pq = wave_scan_add_f64(vq);
+      if (c < ce) {
+        a.carry[2 * c] = cs + ps - vs;  // exact: every partial sum is representable
+        a.carry[2 * c + 1] = cq + pq - vq;
       }
       cs += readlane_f64(ps, 63);
       cq += readlane_f64(pq, 63);
     }
   } else if (lane == 0) {
     // the C source's serial loop, in index order
+    const float *x = a.raw + off;
     double s = 0.0, q = 0.0;
     for (int64_t i = 0; i < n; ++i) {
       const float sq = x[i] * x[i];
@@ -126,6 +177,34 @@ __global__ __launch_bounds__(256) void abea_ge_prefix_kernel(GeArgs a) {
       S[i + 1] = s;
       Q[i + 1] = q;
     }
+  }
+}
+
+// One wave per prefix chunk of an exact read: S and Q inside the chunk from
+// its carry and a scan per 64 samples.
+__global__ __launch_bounds__(256) void abea_ge_chunk_prefix_kernel(GeArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= a.n_chunks) return;
+  const int r = a.chunk_read[c];
+  if (!a.exact[r]) return;
+  const int64_t off = a.raw_off[r];
+  const int64_t n = a.raw_off[r + 1] - off;
+  const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
+  const float *x = a.raw + off;
+  double *S = a.S + off + r, *Q = a.Q + off + r;
+  double cs = a.carry[2 * c], cq = a.carry[2 * c + 1];
+  for (int64_t b = c0; b < c1; b += 64) {
+    const int64_t i = b + lane;
+    const float v = i < c1 ? x[i] : 0.0f;
+    const float sq = v * v;
+    const double ps = wave_scan_add_f64((double)v), pq = wave_scan_add_f64((double)sq);
+    if (i < c1) {
+      S[i + 1] = cs + ps;
+      Q[i + 1] = cq + pq;
+    }
+    cs += readlane_f64(ps, 63);
+    cq += readlane_f64(pq, 63);
   }
 }
 
@@ -154,14 +233,16 @@ __device__ __forceinline__ float tstat_at(const double *S, const double *Q, int6
 }
 
 __global__ __launch_bounds__(256) void abea_ge_tstat_kernel(GeArgs a) {
-  for (int r = blockIdx.x; r < a.n_reads; r += gridDim.x) {
-    const int64_t off = a.raw_off[r];
-    const int64_t n = a.raw_off[r + 1] - off;
-    const double *S = a.S + off + r, *Q = a.Q + off + r;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      a.t1[off + i] = tstat_at(S, Q, n, kWShort, i);
-      a.t2[off + i] = tstat_at(S, Q, n, kWLong, i);
-    }
+  const int64_t c = blockIdx.x;
+  const int r = a.chunk_read[c];
+  const int64_t off = a.raw_off[r];
+  const int64_t n = a.raw_off[r + 1] - off;
+  const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
+  const double *S = a.S + off + r, *Q = a.Q + off + r;
+  const int64_t to = toff(a, r);
+  for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+    a.t1[to + i] = tstat_at(S, Q, n, kWShort, i);
+    a.t2[to + i] = tstat_at(S, Q, n, kWLong, i);
   }
 }
 
@@ -237,57 +318,128 @@ __device__ __forceinline__ bool same_state(GeState a, GeState b, int32_t at) {
 }
 
 // Run samples [from, to) of a read from state st; boundaries emitted at
-// samples >= keep_from go to pk (count in *np).
+// samples >= keep_from go to pk (count in *np).  from is a multiple of 4 and
+// t1, t2 are 16-byte aligned: the statistics come in blocks of 16 samples
+// (float4 loads), the next block loading while this one runs.
 __device__ void run_detector(GeState &st, const float *t1, const float *t2, int32_t from,
                              int32_t to, int32_t keep_from, int32_t *pk, int32_t *np,
                              GeState *entry) {
   int32_t cnt = 0;
   if (entry && from == keep_from) *entry = st;
-  for (int32_t i = from; i < to; ++i) {
-    int32_t p0 = 0, p1 = 0;
-    const int ne = detect_step(st, t1[i], t2[i], i, p0, p1);
-    if (i >= keep_from) {
-      if (ne > 0) pk[cnt++] = p0;
-      if (ne > 1) pk[cnt++] = p1;
+  const float4 *q1 = reinterpret_cast<const float4 *>(t1), *q2 = reinterpret_cast<const float4 *>(t2);
+  float4 n1[4], n2[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    n1[k] = q1[from / 4 + k];
+    n2[k] = q2[from / 4 + k];
+  }
+  for (int32_t i0 = from; i0 < to; i0 += 16) {
+    float b1[16], b2[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      b1[4 * k] = n1[k].x, b1[4 * k + 1] = n1[k].y, b1[4 * k + 2] = n1[k].z, b1[4 * k + 3] = n1[k].w;
+      b2[4 * k] = n2[k].x, b2[4 * k + 1] = n2[k].y, b2[4 * k + 2] = n2[k].z, b2[4 * k + 3] = n2[k].w;
     }
-    if (entry && i + 1 == keep_from) *entry = st;
+    if (i0 + 16 < to) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        n1[k] = q1[(i0 + 16) / 4 + k];
+        n2[k] = q2[(i0 + 16) / 4 + k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int32_t i = i0 + k;
+      if (i < to) {
+        int32_t p0 = 0, p1 = 0;
+        const int ne = detect_step(st, b1[k], b2[k], i, p0, p1);
+        if (i >= keep_from) {
+          if (ne > 0) pk[cnt++] = p0;
+          if (ne > 1) pk[cnt++] = p1;
+        }
+        if (entry && i + 1 == keep_from) *entry = st;
+      }
+    }
   }
   *np = cnt;
 }
 
+// One lane per detector chunk: a fresh run from a.overlap samples before the
+// chunk.  At most 2 * (ceil(512 / 3) + 1) = 344 boundaries: a detector that
+// emits is back to no peak and needs 3 more samples to emit again.
 __global__ __launch_bounds__(256) void abea_ge_detect_kernel(GeArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.n_chunks) return;
-  const int r = a.chunk_read[c];
+  if (c >= a.n_det) return;
+  const int r = a.det_read[c];
   const int64_t off = a.raw_off[r];
   const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
-  const int32_t c0 = (int32_t)(c - a.chunk_off[r]) * kGeChunk;
-  const int32_t c1 = min(c0 + kGeChunk, n);
-  const int32_t from = max(c0 - a.overlap, 0);
+  const int32_t c0 = (int32_t)(c - a.det_off[r]) * kGeDetChunk;
+  const int32_t c1 = min(c0 + kGeDetChunk, n);
+  const int32_t from = max(c0 - a.overlap, 0) & ~3;  // block-aligned (a longer lead-in)
   GeState st = fresh_state();
-  run_detector(st, a.t1 + off, a.t2 + off, from, c1, c0, a.peaks + c * kGeChunk, a.n_peaks + c,
-               a.entry + c);
+  const int64_t to = toff(a, r);
+  run_detector(st, a.t1 + to, a.t2 + to, from, c1, c0, a.peaks + c * kGeDetChunk,
+               a.n_peaks + c, a.entry + c);
   a.exit[c] = st;
 }
 
-__global__ __launch_bounds__(256) void abea_ge_verify_kernel(GeArgs a) {
+// One thread per detector chunk: is its entry its predecessor's exit?
+__global__ __launch_bounds__(256) void abea_ge_check_kernel(GeArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n_det) return;
+  const int r = a.det_read[c];
+  const int32_t lc = (int32_t)(c - a.det_off[r]);
+  if (lc == 0) return;  // the first chunk starts from the true initial state
+  if (!same_state(a.entry[c], a.exit[c - 1], lc * kGeDetChunk)) atomicMin(&a.first_bad[r], lc);
+}
+
+// One lane per read.  The chunks before the first mismatch ran from the true
+// state (induction from the first chunk); from there on, each chunk whose
+// entry is not the true state is re-run from it.
+__global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
-  const int64_t off = a.raw_off[r];
-  const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
-  GeState st = fresh_state();
+  const int32_t fb = a.first_bad[r];
   int32_t reruns = 0;
-  for (int64_t c = a.chunk_off[r]; c < a.chunk_off[r + 1]; ++c) {
-    const int32_t c0 = (int32_t)(c - a.chunk_off[r]) * kGeChunk;
-    if (same_state(a.entry[c], st, c0)) {
-      st = a.exit[c];
-    } else {
-      ++reruns;
-      run_detector(st, a.t1 + off, a.t2 + off, c0, min(c0 + kGeChunk, n), c0,
-                   a.peaks + c * kGeChunk, a.n_peaks + c, nullptr);
+  if (fb != INT32_MAX) {
+    const int64_t off = a.raw_off[r];
+    const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
+    const int64_t cb = a.det_off[r];
+    GeState st = a.exit[cb + fb - 1];
+    for (int64_t c = cb + fb; c < a.det_off[r + 1]; ++c) {
+      const int32_t c0 = (int32_t)(c - cb) * kGeDetChunk;
+      if (same_state(a.entry[c], st, c0)) {
+        st = a.exit[c];
+      } else {
+        ++reruns;
+        run_detector(st, a.t1 + toff(a, r), a.t2 + toff(a, r), c0, min(c0 + kGeDetChunk, n), c0,
+                     a.peaks + c * kGeDetChunk, a.n_peaks + c, nullptr);
+      }
     }
   }
   a.reruns[r] = reruns;
+}
+
+// One wave per read (4 per block): the boundaries gathered in emission order
+// (into the read's t1 space, free by now) and the event count.
+__global__ __launch_bounds__(256) void abea_ge_compact_kernel(GeArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (r >= a.n_reads) return;
+  const int64_t off = a.raw_off[r];
+  const int64_t n = a.raw_off[r + 1] - off;
+  int32_t *bd = a.bounds + off;
+  const int64_t cb = a.det_off[r], ce = a.det_off[r + 1];
+  int64_t base = 0;
+  for (int64_t c0 = cb; c0 < ce; c0 += 64) {
+    const int64_t c = c0 + lane;
+    const int32_t cnt = c < ce ? a.n_peaks[c] : 0;
+    const int incl = wave_scan_add(cnt);
+    const int64_t j = base + incl - cnt;
+    for (int32_t k = 0; k < cnt; ++k) bd[j + k] = a.peaks[c * kGeDetChunk + k];
+    base += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) a.n_events[r] = n > 0 ? base + 1 : 0;
 }
 
 // --------------------------------------------------------------------------
@@ -305,41 +457,24 @@ __device__ __forceinline__ abea_event_t make_event(int64_t start, int64_t end, c
   return e;
 }
 
-__global__ __launch_bounds__(64) void abea_ge_events_kernel(GeArgs a) {
-  const int lane = threadIdx.x;
-  for (int r = blockIdx.x; r < a.n_reads; r += gridDim.x) {
-    const int64_t off = a.raw_off[r];
-    const int64_t n = a.raw_off[r + 1] - off;
-    const double *S = a.S + off + r, *Q = a.Q + off + r;
-    abea_event_t *ev = a.events + off + r;
-    const int64_t cb = a.chunk_off[r], ce = a.chunk_off[r + 1];
-    int64_t base = 0;       // events before this round of chunks
-    int32_t last_pk = 0;    // the last boundary of the chunks before it (0: none)
-    for (int64_t c0 = cb; c0 < ce; c0 += 64) {
-      const int64_t c = c0 + lane;
-      const int32_t cnt = c < ce ? a.n_peaks[c] : 0;
-      const int32_t *pk = a.peaks + c * kGeChunk;
-      const int incl = wave_scan_add(cnt);
-      // the last boundary before this chunk: the last one of the nearest
-      // earlier chunk with any, in this round, else the carried one
-      const int own = cnt > 0 ? lane : -1;
-      const int below = __shfl_up(wave_scan_max(own), 1, 64);
-      int32_t prev = last_pk;
-      if (lane > 0 && below >= 0) prev = a.peaks[(c0 + below) * kGeChunk + a.n_peaks[c0 + below] - 1];
-      int64_t j = base + incl - cnt;
-      for (int32_t k = 0; k < cnt; ++k, ++j) {
-        ev[j] = make_event(prev, pk[k], S, Q);
-        prev = pk[k];
-      }
-      const int top = wave_scan_max(own);
-      const int lastc = __shfl(top, 63, 64);
-      if (lastc >= 0) last_pk = a.peaks[(c0 + lastc) * kGeChunk + a.n_peaks[c0 + lastc] - 1];
-      base += __shfl(incl, 63, 64);
-    }
-    if (lane == 0) {
-      if (n > 0) ev[base] = make_event(last_pk, n, S, Q);
-      a.n_events[r] = n > 0 ? base + 1 : 0;
-    }
+// One block per prefix chunk: events j in [2048 lc, 2048 (lc + 1)) of the
+// read (the last chunk takes any rest), one thread each.  Event j runs from
+// boundary j - 1 (0 for the first) to boundary j (n for the last).
+__global__ __launch_bounds__(256) void abea_ge_events_kernel(GeArgs a) {
+  const int64_t c = blockIdx.x;
+  const int r = a.chunk_read[c];
+  const int64_t lc = c - a.chunk_off[r], nch = a.chunk_off[r + 1] - a.chunk_off[r];
+  const int64_t off = a.raw_off[r];
+  const int64_t n = a.raw_off[r + 1] - off;
+  const int64_t m = a.n_events[r];
+  const int64_t j0 = lc * kGeChunk, j1 = lc == nch - 1 ? m : min(j0 + kGeChunk, m);
+  const double *S = a.S + off + r, *Q = a.Q + off + r;
+  const int32_t *bd = a.bounds + off;
+  abea_event_t *ev = a.events + off + r;
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+    const int64_t start = j == 0 ? 0 : bd[j - 1];
+    const int64_t end = j == m - 1 ? n : bd[j];
+    ev[j] = make_event(start, end, S, Q);
   }
 }
 
@@ -347,15 +482,24 @@ __global__ __launch_bounds__(64) void abea_ge_events_kernel(GeArgs a) {
 
 hipError_t launch_getevents(const GeArgs &a, hipStream_t s) {
   if (a.n_reads <= 0) return hipSuccess;
-  hipLaunchKernelGGL(abea_ge_prefix_kernel, dim3((a.n_reads + 3) / 4), dim3(256), 0, s, a);
-  const int grid = a.n_reads < 4096 ? a.n_reads : 4096;
-  hipLaunchKernelGGL(abea_ge_tstat_kernel, dim3(grid), dim3(256), 0, s, a);
+  const unsigned rb = (unsigned)((a.n_reads + 3) / 4);     // a wave per read
+  const unsigned cb = (unsigned)((a.n_chunks + 3) / 4);    // a wave per prefix chunk
+  const unsigned db = (unsigned)((a.n_det + 255) / 256);   // a lane per detector chunk
+  if (a.n_chunks > 0) hipLaunchKernelGGL(abea_ge_chunk_sum_kernel, dim3(cb), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(abea_ge_read_scan_kernel, dim3(rb), dim3(256), 0, s, a);
   if (a.n_chunks > 0) {
-    hipLaunchKernelGGL(abea_ge_detect_kernel, dim3((unsigned)((a.n_chunks + 255) / 256)), dim3(256),
-                       0, s, a);
+    hipLaunchKernelGGL(abea_ge_chunk_prefix_kernel, dim3(cb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(abea_ge_tstat_kernel, dim3((unsigned)a.n_chunks), dim3(256), 0, s, a);
   }
-  hipLaunchKernelGGL(abea_ge_verify_kernel, dim3((a.n_reads + 255) / 256), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(abea_ge_events_kernel, dim3(grid), dim3(64), 0, s, a);
+  if (a.n_det > 0) {
+    hipLaunchKernelGGL(abea_ge_detect_kernel, dim3(db), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(abea_ge_check_kernel, dim3(db), dim3(256), 0, s, a);
+  }
+  hipLaunchKernelGGL(abea_ge_fix_kernel, dim3((unsigned)((a.n_reads + 255) / 256)), dim3(256), 0,
+                     s, a);
+  hipLaunchKernelGGL(abea_ge_compact_kernel, dim3(rb), dim3(256), 0, s, a);
+  if (a.n_chunks > 0)
+    hipLaunchKernelGGL(abea_ge_events_kernel, dim3((unsigned)a.n_chunks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

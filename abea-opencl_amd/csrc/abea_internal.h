@@ -123,9 +123,12 @@ struct MomArgs {
   abea_scalings_t *out;
 };
 
-// Event detection (getevents), one read per raw signal.
-constexpr int kGeChunk = 2048;   // samples per detector chunk
-constexpr int kGeOverlap = 256;  // samples a chunk's speculative run starts early
+// Event detection (getevents), one read per raw signal.  Samples are cut
+// into kGeChunk-sample chunks for the prefix sums, the t-statistics and the
+// events, and into kGeDetChunk-sample chunks for the peak detector.
+constexpr int kGeChunk = 2048;
+constexpr int kGeDetChunk = 512;
+constexpr int kGeOverlap = 64;  // samples a detector chunk's speculative run starts early
 struct GeDet {
   int32_t pos;        // -1: no peak yet
   float val;          // running minimum, or the peak's value
@@ -135,20 +138,38 @@ struct GeDet {
 struct GeState {
   GeDet d[2];  // short, long
 };
+// per prefix chunk: its sums and what decides whether a read's sums are
+// exact in any order
+struct GeChunkSum {
+  double s, q;        // sum of samples, of their float squares
+  double abs_s, abs_q;  // sums of magnitudes (upper bounds of every partial sum)
+  int32_t gx, gq;     // smallest ulp exponent of a sample / square (1000: all zero)
+  int32_t bad;        // an inf or NaN
+  int32_t pad;
+};
 struct GeArgs {
   int32_t n_reads;
   int32_t overlap;
-  int64_t n_chunks;
+  int64_t n_chunks;           // prefix chunks
+  int64_t n_det;              // detector chunks
   const int64_t *raw_off;     // n_reads + 1: read r is raw[raw_off[r] .. raw_off[r+1])
   const float *raw;
   double *S, *Q;              // prefix sums, read r at raw_off[r] + r (n + 1 entries)
-  float *t1, *t2;             // t-statistics, read r at raw_off[r]
-  const int64_t *chunk_off;   // n_reads + 1: read r's chunks
-  const int32_t *chunk_read;  // the read of each chunk
-  int32_t *peaks;             // kGeChunk boundaries per chunk, in emission order
-  int32_t *n_peaks;           // per chunk
-  GeState *entry, *exit;      // per chunk: state before its first and after its last sample
-  int32_t *reruns;            // per read: chunks re-run from the true state
+  float *t1, *t2;             // t-statistics, read r from align4(raw_off[r]) + 8r
+  int32_t *bounds;            // compacted boundaries, read r at raw_off[r] (reuses t1)
+  const int64_t *chunk_off;   // n_reads + 1: read r's prefix chunks
+  const int32_t *chunk_read;  // the read of each prefix chunk
+  GeChunkSum *csum;           // per prefix chunk
+  double *carry;              // per prefix chunk: S and Q before it (2 doubles)
+  int32_t *exact;             // per read: 1 when the parallel sums are exact
+  const int64_t *det_off;     // n_reads + 1: read r's detector chunks
+  const int32_t *det_read;    // the read of each detector chunk
+  int32_t *peaks;             // kGeDetChunk boundaries per detector chunk, in emission order
+  int32_t *n_peaks;           // per detector chunk
+  GeState *entry, *exit;      // per detector chunk: state before its first / after its last sample
+  int32_t *first_bad;         // per read: first detector chunk whose entry is not its
+                              // predecessor's exit (INT32_MAX: none)
+  int32_t *reruns;            // per read: detector chunks re-run from the true state
   abea_event_t *events;       // read r at raw_off[r] + r (room for n + 1)
   int64_t *n_events;          // per read
 };

@@ -137,6 +137,66 @@ def postalign_leg(args, ctx, tune) -> dict:
     return res
 
 
+def getevents_leg(args, ctx) -> dict:
+    """f5c's step before MOM and align(): event detection on the raw signals
+    (pA) of 1,024 reads of the 5 kb log-normal workload, resident in HBM.
+    The oracle's serial time on a bounded sample of the same reads is the CPU
+    baseline."""
+    import numpy as np
+    import torch
+
+    from abea_amd import _lib, engine, synth
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # CPU baseline only
+
+    b = synth.workload("plain4096_ln5kb", seed=args.seed + 7, n=1024)
+    raw_h, raw_off = synth.raw_signal(b, seed=args.seed + 7)
+    raw = torch.from_numpy(raw_h).cuda()
+    n_samples = int(raw_off[-1])
+    ev = torch.empty((n_samples + b.n_reads) * 24, dtype=torch.uint8, device="cuda")
+    cnt = torch.empty(b.n_reads, dtype=torch.int64, device="cuda")
+    engine.getevents(ctx, raw, raw_off, ev, cnt)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(ctx.stream)
+        _lib.check(_lib.lib().abea_getevents_device(ctx.handle, b.n_reads,
+                                                     raw_off.ctypes.data_as(_lib.P),
+                                                     _lib.P(raw.data_ptr()), _lib.P(ev.data_ptr()),
+                                                     _lib.P(cnt.data_ptr())), "getevents")
+        e1.record(ctx.stream)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ms = min(ts)
+    n_ev = int(cnt.sum().item())
+    reruns = int(_lib.lib().abea_ctx_getevents_reruns(ctx.handle))
+    # algorithmic bytes: the signal read once (4 B per sample), the event
+    # records written (24 B each)
+    alg = 4 * n_samples + 24 * n_ev
+    # CPU: the oracle, one thread, on the first reads up to ~2 s of work
+    t0 = time.perf_counter()
+    done = 0
+    for i in range(b.n_reads):
+        oracle.getevents(raw_h[raw_off[i]:raw_off[i + 1]])
+        done = i + 1
+        if time.perf_counter() - t0 > 2.0:
+            break
+    cpu_s = time.perf_counter() - t0
+    cpu_samples = int(raw_off[done])
+    res = {"workload": "getevents1024_ln5kb", "reads": b.n_reads, "samples": n_samples,
+           "events": n_ev, "ms": round(ms, 3),
+           "gsamples_per_s": round(n_samples / (ms * 1e-3) / 1e9, 3),
+           "alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "chunks_rerun": reruns,
+           "cpu": {"gsamples_per_s": round(cpu_samples / cpu_s / 1e9, 4), "cores": 1,
+                   "kind": "port", "sample": f"first {done} reads ({cpu_samples} samples)"}}
+    log(f"[bench] getevents: {res}")
+    del raw, ev, cnt
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +211,7 @@ def main():
                          "1 GiB device copy to calibrate HBM byte counters")
     ap.add_argument("--eager", action="store_true",
                     help="time eager launches instead of the captured graph")
-    ap.add_argument("--leg", choices=["postalign"], default=None,
+    ap.add_argument("--leg", choices=["postalign", "getevents"], default=None,
                     help="run only this extra measurement (its JSON on stdout)")
     ap.add_argument("--tuning", default=None,
                     help="WPB,WPC,FUSE,OVERLAP: use this core launch shape instead of "
@@ -172,6 +232,9 @@ def main():
 
     if args.leg == "postalign":
         print(json.dumps(postalign_leg(args, engine.Context(0), None)), flush=True)
+        return
+    if args.leg == "getevents":
+        print(json.dumps(getevents_leg(args, engine.Context(0))), flush=True)
         return
 
     t = time.perf_counter()
@@ -322,6 +385,7 @@ def main():
         del p3, d3
 
         out["postalign"] = postalign_leg(args, ctx, tune)
+        out["getevents"] = getevents_leg(args, ctx)
         torch.cuda.empty_cache()
 
         # scaling refinement: 8,192 x 20 kb reads, scalings perturbed (shift

@@ -210,12 +210,12 @@ void *abea_ctx_stream(abea_ctx_t *ctx);
  * read < 2^31 samples); its events go to events[raw_off_host[i] + i ...]
  * (device, room for nsample + 1 each) and their count to n_events[i]
  * (device).  Enqueued on the context's stream; the context keeps the
- * scratch (about 40 bytes per sample). */
+ * scratch (about 28 bytes per sample). */
 abea_status_t abea_getevents_device(abea_ctx_t *ctx, int32_t n_reads,
                                     const int64_t *raw_off_host, const float *raw,
                                     abea_event_t *events, int64_t *n_events);
-/* Samples each detector chunk's speculative run starts before the chunk
- * (default 256, 0..2048).  Results do not depend on it, only how many
+/* Samples each 512-sample detector chunk's speculative run starts before
+ * the chunk (default 64, 0..512).  Results do not depend on it, only how many
  * chunks are re-run from the true state; abea_ctx_getevents_reruns() gives
  * that count for the last abea_getevents_device() call (synchronizes). */
 abea_status_t abea_ctx_set_getevents_overlap(abea_ctx_t *ctx, int32_t samples);

@@ -67,7 +67,7 @@ def _run_device(ctx, sigs, lead=0, overlap=None):
     torch.cuda.synchronize()
     res = engine.getevents_results(ev, cnt, raw_off)
     if overlap is not None:
-        _lib.check(_lib.lib().abea_ctx_set_getevents_overlap(ctx.handle, 256), "overlap")
+        _lib.check(_lib.lib().abea_ctx_set_getevents_overlap(ctx.handle, 64), "overlap")
     return res
 
 
@@ -95,7 +95,7 @@ def test_whole_reads_and_reruns(ctx):
     raw, off = synth.raw_signal(b, seed=5)
     sigs = [raw[off[i]:off[i + 1]] for i in range(b.n_reads)]
     want = [oracle.getevents(s) for s in sigs]
-    for overlap in (256, 0, 17):
+    for overlap in (64, 0, 17, 512):
         got = _run_device(ctx, sigs, overlap=overlap)
         reruns = _lib.lib().abea_ctx_getevents_reruns(ctx.handle)
         print(f"[getevents] overlap {overlap}: chunks re-run {reruns}", flush=True)
