@@ -248,10 +248,10 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
       (st = g.det_read.ensure(sizeof(int32_t) * nd1)) ||
       (st = g.S.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
       (st = g.Q.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
-      // per read a 16-byte aligned start and 8 floats of slack; the
-      // detector's block loads read up to 15 floats past a read's end
-      (st = g.t1.ensure(sizeof(float) * (size_t)(extent + 8 * (int64_t)n + 32))) ||
-      (st = g.t2.ensure(sizeof(float) * (size_t)(extent + 8 * (int64_t)n + 32))) ||
+      // per read a cache-line aligned start and 32 floats of slack; the
+      // detector's block loads read up to 31 floats past a read's end
+      (st = g.t1.ensure(sizeof(float) * (size_t)(extent + 32 * (int64_t)n + 64))) ||
+      (st = g.t2.ensure(sizeof(float) * (size_t)(extent + 32 * (int64_t)n + 64))) ||
       (st = g.csum.ensure(sizeof(abea::GeChunkSum) * nc1)) ||
       (st = g.carry.ensure(2 * sizeof(double) * nc1)) ||
       (st = g.exact.ensure(sizeof(int32_t) * (size_t)n)) ||

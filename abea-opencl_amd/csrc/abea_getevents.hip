@@ -7,13 +7,17 @@
 // and 9.0, peak height 0.2), and events between consecutive boundaries.
 //
 // Every stage runs over chunks of every read at once:
-//   chunk_sum / read_scan / chunk_prefix   the prefix sums.  They are exact
-//            in any order when every term is a multiple of 2^g and the sum
-//            of magnitudes stays below 2^(52+g) (true for pA-scale
-//            signals): chunk sums, a scan of them per read, then a scan
-//            inside each chunk.  A read that fails the test takes one serial
-//            pass in index order.
-//   tstat    one thread per sample, the C source's float/double types.
+//   chunk_sum / read_scan   the double sums are exact in any order when
+//            every term is a multiple of 2^g and the sum of magnitudes stays
+//            below 2^(52+g) (true for pA-scale signals).  Then every
+//            difference of prefix sums the C source takes (window sums,
+//            event sums) is the exact sum of its samples, whatever the
+//            order: those sums come straight from the samples, and only the
+//            chunks' carries are kept (for long events).  A read that fails
+//            the test takes one serial pass in index order that writes the
+//            prefix sums, and its statistics and events read them.
+//   tstat    one thread per sample (window sums from the chunk's samples in
+//            LDS), the C source's float/double types.
 //   detect   one lane per 512-sample chunk, started fresh a few samples
 //            early (speculation): the chunk's entry state, its boundaries in
 //            emission order and its exit state.
@@ -39,12 +43,13 @@ namespace {
 constexpr float kThrShort = 1.4f, kThrLong = 9.0f, kPeakHeight = 0.2f;
 constexpr int kWShort = 3, kWLong = 6;
 
-// The t-statistics of read r start at toff(r): 16-byte aligned, at least 5
-// floats after the previous read's end.  The detector's block loads read up
-// to 15 floats past a read's end: into the next read's statistics, or into
-// the 32 floats allocated past the last read's.
+// The t-statistics of read r start at toff(r): 128-byte (cache line)
+// aligned, after the previous read's end (align32(b) - align32(a) + 32 >
+// b - a).  The detector's block loads read up to 15 floats past a read's
+// end: into the next read's statistics, or into the 64 floats allocated past
+// the last read's.
 __device__ __forceinline__ int64_t toff(const GeArgs &a, int r) {
-  return ((a.raw_off[r] + 3) & ~(int64_t)3) + 8 * (int64_t)r;
+  return ((a.raw_off[r] + 31) & ~(int64_t)31) + 32 * (int64_t)r;
 }
 
 // --------------------------------------------------------------------------
@@ -124,8 +129,9 @@ __global__ __launch_bounds__(256) void abea_ge_chunk_sum_kernel(GeArgs a) {
 }
 
 // One wave per read: the exactness decision, then the chunks' carries (an
-// exclusive scan of their sums, exact when the read is), or the serial
-// pass.  Also clears the read's detector mismatch marker.
+// exclusive scan of their sums, exact when the read is), or the serial pass
+// that writes the prefix sums.  Also clears the read's detector mismatch
+// marker.
 __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -150,8 +156,10 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
   if (lane == 0) {
     a.exact[r] = exact ? 1 : 0;
     a.first_bad[r] = INT32_MAX;
-    S[0] = 0.0;
-    Q[0] = 0.0;
+    if (!exact) {
+      S[0] = 0.0;
+      Q[0] = 0.0;
+    }
   }
   if (exact) {
     double cs = 0.0, cq = 0.0;
@@ -180,34 +188,6 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
   }
 }
 
-// One wave per prefix chunk of an exact read: S and Q inside the chunk from
-// its carry and a scan per 64 samples.
-__global__ __launch_bounds__(256) void abea_ge_chunk_prefix_kernel(GeArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= a.n_chunks) return;
-  const int r = a.chunk_read[c];
-  if (!a.exact[r]) return;
-  const int64_t off = a.raw_off[r];
-  const int64_t n = a.raw_off[r + 1] - off;
-  const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
-  const float *x = a.raw + off;
-  double *S = a.S + off + r, *Q = a.Q + off + r;
-  double cs = a.carry[2 * c], cq = a.carry[2 * c + 1];
-  for (int64_t b = c0; b < c1; b += 64) {
-    const int64_t i = b + lane;
-    const float v = i < c1 ? x[i] : 0.0f;
-    const float sq = v * v;
-    const double ps = wave_scan_add_f64((double)v), pq = wave_scan_add_f64((double)sq);
-    if (i < c1) {
-      S[i + 1] = cs + ps;
-      Q[i + 1] = cq + pq;
-    }
-    cs += readlane_f64(ps, 63);
-    cq += readlane_f64(pq, 63);
-  }
-}
-
 // --------------------------------------------------------------------------
 // t-statistics, one thread per sample; the C source's float/double types
 __device__ __forceinline__ float tstat_at(const double *S, const double *Q, int64_t n, int w,
@@ -232,17 +212,59 @@ __device__ __forceinline__ float tstat_at(const double *S, const double *Q, int6
   return (float)(fabs((double)dm) / sqrt((double)(var / wf)));
 }
 
+// The statistic from exact window sums (a read whose sums are exact): the
+// same values as tstat_at's prefix-sum differences.  xs[k] is sample k - 6
+// of the chunk's halo-extended block.
+__device__ __forceinline__ float tstat_win(const float *xs, int64_t n, int w, int64_t i, int k) {
+  if (n < 2 * (int64_t)w || i < w || i > n - w) return 0.0f;
+  const float wf = (float)w;
+  double s1 = 0.0, q1 = 0.0, s2d = 0.0, q2d = 0.0;
+  for (int j = 0; j < w; ++j) {
+    const float a = xs[k - w + j], b = xs[k + j];
+    s1 += (double)a;
+    q1 += (double)(a * a);
+    s2d += (double)b;
+    q2d += (double)(b * b);
+  }
+  const float s2 = (float)s2d, q2 = (float)q2d;
+  const float m1 = (float)(s1 / (double)wf);
+  const float m2 = s2 / wf;
+  double cv = q1 / (double)wf;
+  cv -= (double)(m1 * m1);
+  cv += (double)(q2 / wf);
+  cv -= (double)(m2 * m2);
+  const float var = fmaxf((float)cv, FLT_MIN);
+  const float dm = m2 - m1;
+  return (float)(fabs((double)dm) / sqrt((double)(var / wf)));
+}
+
+// One block per prefix chunk.
 __global__ __launch_bounds__(256) void abea_ge_tstat_kernel(GeArgs a) {
+  __shared__ float xs[kGeChunk + 2 * kWLong];
   const int64_t c = blockIdx.x;
   const int r = a.chunk_read[c];
   const int64_t off = a.raw_off[r];
   const int64_t n = a.raw_off[r + 1] - off;
   const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
-  const double *S = a.S + off + r, *Q = a.Q + off + r;
   const int64_t to = toff(a, r);
-  for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-    a.t1[to + i] = tstat_at(S, Q, n, kWShort, i);
-    a.t2[to + i] = tstat_at(S, Q, n, kWLong, i);
+  if (a.exact[r]) {
+    const float *x = a.raw + off;
+    for (int k = threadIdx.x; k < kGeChunk + 2 * kWLong; k += blockDim.x) {
+      const int64_t i = c0 - kWLong + k;
+      xs[k] = (i >= 0 && i < n) ? x[i] : 0.0f;
+    }
+    __syncthreads();
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+      const int k = (int)(i - c0) + kWLong;
+      a.t1[to + i] = tstat_win(xs, n, kWShort, i, k);
+      a.t2[to + i] = tstat_win(xs, n, kWLong, i, k);
+    }
+  } else {
+    const double *S = a.S + off + r, *Q = a.Q + off + r;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+      a.t1[to + i] = tstat_at(S, Q, n, kWShort, i);
+      a.t2[to + i] = tstat_at(S, Q, n, kWLong, i);
+    }
   }
 }
 
@@ -318,8 +340,8 @@ __device__ __forceinline__ bool same_state(GeState a, GeState b, int32_t at) {
 }
 
 // Run samples [from, to) of a read from state st; boundaries emitted at
-// samples >= keep_from go to pk (count in *np).  from is a multiple of 4 and
-// t1, t2 are 16-byte aligned: the statistics come in blocks of 16 samples
+// samples >= keep_from go to pk (count in *np).  from is a multiple of 16 and
+// t1, t2 are cache-line aligned: the statistics come in blocks of 16 samples
 // (float4 loads), the next block loading while this one runs.
 __device__ void run_detector(GeState &st, const float *t1, const float *t2, int32_t from,
                              int32_t to, int32_t keep_from, int32_t *pk, int32_t *np,
@@ -375,7 +397,7 @@ __global__ __launch_bounds__(256) void abea_ge_detect_kernel(GeArgs a) {
   const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
   const int32_t c0 = (int32_t)(c - a.det_off[r]) * kGeDetChunk;
   const int32_t c1 = min(c0 + kGeDetChunk, n);
-  const int32_t from = max(c0 - a.overlap, 0) & ~3;  // block-aligned (a longer lead-in)
+  const int32_t from = max(c0 - a.overlap, 0) & ~15;  // block-aligned (a longer lead-in)
   GeState st = fresh_state();
   const int64_t to = toff(a, r);
   run_detector(st, a.t1 + to, a.t2 + to, from, c1, c0, a.peaks + c * kGeDetChunk,
@@ -457,24 +479,75 @@ __device__ __forceinline__ abea_event_t make_event(int64_t start, int64_t end, c
   return e;
 }
 
+// Exact sums of samples [0, p) of an exact read (sum of samples, of their
+// float squares): the carry of p's chunk plus the samples before p in it.
+__device__ __forceinline__ void exact_prefix(const GeArgs &a, const float *x, int64_t cbase,
+                                             int64_t p, double &s, double &q) {
+  s = 0.0;
+  q = 0.0;
+  if (p <= 0) return;
+  const int64_t lc = (p - 1) / kGeChunk, k0 = lc * kGeChunk;
+  s = a.carry[2 * (cbase + lc)];
+  q = a.carry[2 * (cbase + lc) + 1];
+  for (int64_t k = k0; k < p; ++k) {
+    const float v = x[k];
+    s += (double)v;
+    q += (double)(v * v);
+  }
+}
+
+// An event from the exact sums of its samples (read exact) or from the
+// prefix sums (serial reads); the same values either way.
+__device__ __forceinline__ abea_event_t make_event_exact(const GeArgs &a, const float *x,
+                                                         int64_t cbase, int64_t start,
+                                                         int64_t end) {
+  double ds, dq;
+  if (start <= end && end - start <= 2 * kGeChunk) {
+    ds = 0.0;
+    dq = 0.0;
+    for (int64_t k = start; k < end; ++k) {
+      const float v = x[k];
+      ds += (double)v;
+      dq += (double)(v * v);
+    }
+  } else {
+    double s0, q0, s1, q1;
+    exact_prefix(a, x, cbase, start, s0, q0);
+    exact_prefix(a, x, cbase, end, s1, q1);
+    ds = s1 - s0;
+    dq = q1 - q0;
+  }
+  abea_event_t e{};
+  e.start = (uint64_t)start;
+  e.length = (float)((uint64_t)end - (uint64_t)start);
+  e.mean = (float)ds / e.length;
+  const float dsq = (float)dq;
+  const float var = dsq / e.length - e.mean * e.mean;
+  e.stdv = sqrtf(fmaxf(var, 0.0f));
+  return e;
+}
+
 // One block per prefix chunk: events j in [2048 lc, 2048 (lc + 1)) of the
 // read (the last chunk takes any rest), one thread each.  Event j runs from
 // boundary j - 1 (0 for the first) to boundary j (n for the last).
 __global__ __launch_bounds__(256) void abea_ge_events_kernel(GeArgs a) {
   const int64_t c = blockIdx.x;
   const int r = a.chunk_read[c];
-  const int64_t lc = c - a.chunk_off[r], nch = a.chunk_off[r + 1] - a.chunk_off[r];
+  const int64_t cbase = a.chunk_off[r];
+  const int64_t lc = c - cbase, nch = a.chunk_off[r + 1] - cbase;
   const int64_t off = a.raw_off[r];
   const int64_t n = a.raw_off[r + 1] - off;
   const int64_t m = a.n_events[r];
   const int64_t j0 = lc * kGeChunk, j1 = lc == nch - 1 ? m : min(j0 + kGeChunk, m);
-  const double *S = a.S + off + r, *Q = a.Q + off + r;
   const int32_t *bd = a.bounds + off;
   abea_event_t *ev = a.events + off + r;
+  const bool exact = a.exact[r] != 0;
+  const double *S = a.S + off + r, *Q = a.Q + off + r;
+  const float *x = a.raw + off;
   for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
     const int64_t start = j == 0 ? 0 : bd[j - 1];
     const int64_t end = j == m - 1 ? n : bd[j];
-    ev[j] = make_event(start, end, S, Q);
+    ev[j] = exact ? make_event_exact(a, x, cbase, start, end) : make_event(start, end, S, Q);
   }
 }
 
@@ -487,10 +560,8 @@ hipError_t launch_getevents(const GeArgs &a, hipStream_t s) {
   const unsigned db = (unsigned)((a.n_det + 255) / 256);   // a lane per detector chunk
   if (a.n_chunks > 0) hipLaunchKernelGGL(abea_ge_chunk_sum_kernel, dim3(cb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(abea_ge_read_scan_kernel, dim3(rb), dim3(256), 0, s, a);
-  if (a.n_chunks > 0) {
-    hipLaunchKernelGGL(abea_ge_chunk_prefix_kernel, dim3(cb), dim3(256), 0, s, a);
+  if (a.n_chunks > 0)
     hipLaunchKernelGGL(abea_ge_tstat_kernel, dim3((unsigned)a.n_chunks), dim3(256), 0, s, a);
-  }
   if (a.n_det > 0) {
     hipLaunchKernelGGL(abea_ge_detect_kernel, dim3(db), dim3(256), 0, s, a);
     hipLaunchKernelGGL(abea_ge_check_kernel, dim3(db), dim3(256), 0, s, a);

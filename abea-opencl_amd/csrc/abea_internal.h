@@ -154,8 +154,8 @@ struct GeArgs {
   int64_t n_det;              // detector chunks
   const int64_t *raw_off;     // n_reads + 1: read r is raw[raw_off[r] .. raw_off[r+1])
   const float *raw;
-  double *S, *Q;              // prefix sums, read r at raw_off[r] + r (n + 1 entries)
-  float *t1, *t2;             // t-statistics, read r from align4(raw_off[r]) + 8r
+  double *S, *Q;              // prefix sums of serial reads, read r at raw_off[r] + r (n + 1)
+  float *t1, *t2;             // t-statistics, read r from align32(raw_off[r]) + 32r
   int32_t *bounds;            // compacted boundaries, read r at raw_off[r] (reuses t1)
   const int64_t *chunk_off;   // n_reads + 1: read r's prefix chunks
   const int32_t *chunk_read;  // the read of each prefix chunk
