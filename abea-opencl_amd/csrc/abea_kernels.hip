@@ -751,7 +751,9 @@ template <int C>
 __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, int S, int b_end,
                                              const int (&sid)[C], int (&k)[C], int (&e)[C],
                                              SegInfo *si) {
-  int lo[C], hi[C], cnt[C], ent_k[C], ent_e[C], ex_k[C], ex_e[C], ct[C], cq[C];
+  // a segment's entry and exit cells go to its LDS record when they are
+  // found (-1 until then), not held in registers
+  int lo[C], hi[C], cnt[C], ct[C], cq[C];
   uint32_t word[C];
   uint2 cw[C];
   bool act[C], have[C];
@@ -761,11 +763,16 @@ __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, i
     lo[c] = 2 + sid[c] * S;
     hi[c] = min(lo[c] + S - 1, b_end);
     cnt[c] = 0;
-    ent_k[c] = ent_e[c] = ex_k[c] = ex_e[c] = -1;
     ct[c] = cq[c] = -1;
     word[c] = 0;
     cw[c] = make_uint2(0, 0);
     have[c] = false;
+    if (act[c]) {
+      si->ent_k[sid[c]] = -1;
+      si->ent_e[sid[c]] = -1;
+      si->ex_k[sid[c]] = -1;
+      si->ex_e[sid[c]] = -1;
+    }
   }
   for (;;) {
     bool any = false;
@@ -779,12 +786,12 @@ __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, i
       if (act[c]) {
         if (!have[c] && bb[c] <= hi[c]) {
           have[c] = true;
-          ent_k[c] = k[c];
-          ent_e[c] = e[c];
+          si->ent_k[sid[c]] = k[c];
+          si->ent_e[sid[c]] = e[c];
         }
         if (bb[c] < lo[c]) {
-          ex_k[c] = k[c];
-          ex_e[c] = e[c];
+          si->ex_k[sid[c]] = k[c];
+          si->ex_e[sid[c]] = e[c];
           act[c] = false;
         }
       }
@@ -819,11 +826,7 @@ __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, i
   for (int c = 0; c < C; ++c) {
     if (sid[c] >= 0) {
       if (cnt[c] & 3) ((uint32_t *)(seg + sid[c] * S))[cnt[c] >> 2] = word[c];
-      si->ent_k[sid[c]] = ent_k[c];
-      si->ent_e[sid[c]] = ent_e[c];
       si->cnt[sid[c]] = cnt[c];
-      si->ex_k[sid[c]] = ex_k[c];
-      si->ex_e[sid[c]] = ex_e[c];
     }
   }
 }
