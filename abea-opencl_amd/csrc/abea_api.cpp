@@ -357,7 +357,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   p->n = n;
   p->form = form;
   p->meta_h.assign(n, ReadMeta{});
-  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0, sg = 0, maxlen = 0, bm = 0;
+  int64_t tr = 0, km = 0, pr = 0, mv = 0, evmax = 0, nb = 0, sg = 0, maxlen = 0, bm = 0, sev = 0;
   for (int32_t i = 0; i < n; ++i) {
     ReadMeta &m = p->meta_h[i];
     const int32_t n_e = b->n_events_host[i];
@@ -369,7 +369,8 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     evmax = std::max<int64_t>(evmax, m.ev_off + n_e);
     m.pair_off = pr;
     pr += (int64_t)n_e + std::max<int32_t>(n_k, 0);
-    m.km_off = km;
+    m.km_off = km + abea::kStagePad;
+    m.sev_off = sev + abea::kStagePad;
     m.bmap_off = bm;
     bm += std::max<int32_t>(n_k, 0);
     m.trace_off = tr;
@@ -377,7 +378,8 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
     m.seg_off = sg;
     if (n_k > 0 && n_e > 0) {
       const int64_t bands = (int64_t)n_e + n_k + 2;
-      km += n_k;
+      km += n_k + 2 * abea::kStagePad;
+      sev += n_e + 2 * abea::kStagePad;
       tr += (bands + abea::kTileBands - 1) / abea::kTileBands * abea::kTileWords;
       mv += (bands + 31) / 32;
       // traceback segments: one code byte per row they cover; the last
@@ -395,7 +397,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
   p->seg_bytes = sg;
   p->move_words = mv;
   p->km_count = km;
-  p->ev_count = evmax;
+  p->ev_count = sev;
   p->n_bands_total = nb;
   // The core prefetches event E+1 and k-mer K+128 of the band walk without
   // clamping: E <= n_e + n_k + 50 and K + 128 <= n_e + n_k + 77 for any read,
@@ -429,7 +431,7 @@ abea_status_t plan_bind(abea_plan *p, const abea_batch_t *b, int form) {
 
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
-  const int64_t kmc = km + p->tail_pad, evc = evmax + p->tail_pad;
+  const int64_t kmc = km + p->tail_pad, evc = sev + p->tail_pad;  // staged, padded per read
   const size_t need = (size_t)tr * 4 + (size_t)kmc * (sizeof(KmerParam) + 2) + (size_t)evc * 4 +
                       (size_t)mv * 12 + (size_t)sg + (size_t)n * (sizeof(ReadMeta) + 12);
   if (free_b && need > free_b + p->trace.n + p->km.n + p->km_rank.n + p->ev.n + p->ends.n +

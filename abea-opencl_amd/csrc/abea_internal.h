@@ -26,8 +26,20 @@ struct alignas(16) ReadMeta {
   double lp_stay;     // per-read transition log-probabilities (host glibc log/exp)
   double lp_step;
   int64_t bmap_off;   // offset of the read's base-to-event map entries (prefix of max(n_k, 0))
+  int64_t sev_off;    // offset of the read's staged event means (kStagePad entries of padding
+                      // before and after them; km_off likewise for the k-mer records)
+  int64_t reserved;
 };
-static_assert(sizeof(ReadMeta) == 80, "ReadMeta must stay 80 bytes");
+static_assert(sizeof(ReadMeta) == 96, "ReadMeta must stay 96 bytes");
+
+// Padding around each read's staged event means and k-mer records.  Padding
+// events are kPadEvent and padding k-mers {0, 1, 1, -inf}: every emission
+// that uses one is -inf, so band cells outside the matrix compute to -inf in
+// the walk without masks.  A pad of 128 covers the walk's gathers (events down
+// to E - 126 and up to E + 16 with E <= n_e + 111; k-mers from K >= -120 to
+// K + 127 with K <= n_k).
+constexpr int kStagePad = 128;
+constexpr float kPadEvent = 1e30f;  // (x - m)/s squared overflows float for any stdv s < 5e10
 
 // Staged per-k-mer emission parameters, written by the core wave of the read.
 struct alignas(16) KmerParam {

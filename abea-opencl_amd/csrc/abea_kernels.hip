@@ -227,7 +227,19 @@ __device__ __forceinline__ void stage_read(const KernelArgs &a, const ReadMeta &
       }
     }
   }
-  float *ev = a.ev + m.ev_off;
+  // the padding on both sides of the read's k-mer records and events
+  for (int j = lane; j < kStagePad; j += 64) {
+    const KmerParam pad{0.0f, 1.0f, 1.0f, -__builtin_inff()};
+    km[-kStagePad + j] = pad;
+    km[m.n_k + j] = pad;
+    kr[-kStagePad + j] = 0;
+    kr[m.n_k + j] = 0;
+  }
+  float *ev = a.ev + m.sev_off;
+  for (int j = lane; j < kStagePad; j += 64) {
+    ev[-kStagePad + j] = kPadEvent;
+    ev[m.n_e + j] = kPadEvent;
+  }
   for (int e0 = 0; e0 < m.n_e; e0 += 256) {
     float v[4];
 #pragma unroll
@@ -332,15 +344,17 @@ struct Cells {
   double left0;
   uint64_t u0, l0, u1, l1;
 };
+// skip0 / skip1: the skip transition of each parity's slots (the same value
+// but for one slot at the right edge of the matrix, see tile_fast).
 __device__ __forceinline__ Cells compute_cells(const Slots &s, double lp_step, double lp_stay,
-                                               double lp_skip) {
+                                               double skip0, double skip1) {
   Cells c;
   c.left0 = rotate_up_f64(s.D1);
   const float em0 = emission(s.ev0, s.km0m, s.km0s, s.km0r, s.km0c);
   const float em1 = emission(s.ev1, s.km1m, s.km1s, s.km1r, s.km1c);
   bool u0, l0, u1, l1;
-  c.M0 = dp_cell(s.Lq, s.D0, c.left0, em0, lp_step, lp_stay, lp_skip, u0, l0);
-  c.M1 = dp_cell(s.Dq, s.D1, s.D0, em1, lp_step, lp_stay, lp_skip, u1, l1);
+  c.M0 = dp_cell(s.Lq, s.D0, c.left0, em0, lp_step, lp_stay, skip0, u0, l0);
+  c.M1 = dp_cell(s.Dq, s.D1, s.D0, em1, lp_step, lp_stay, skip1, u1, l1);
   c.u0 = __builtin_amdgcn_ballot_w64(u0);
   c.l0 = __builtin_amdgcn_ballot_w64(l0);
   c.u1 = __builtin_amdgcn_ballot_w64(u1);
@@ -381,10 +395,18 @@ __device__ __forceinline__ int slot_lane(int k) { return (k >> 1) & 63; }
 // bands or the two slots just outside the window (K-1 and K+100), so those
 // two are set to -inf and the other out-of-window slots are left as they are
 // (never read by an in-window cell).
-template <bool ODD>
-__device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i, float ev0,
+// Per-read values an edge tile needs.
+struct EdgeConst {
+  int n_e, n_k, js;   // js: slot of the last k-mer (the end-search column)
+  double lp_trim;
+};
+
+template <bool ODD, bool EDGE>
+__device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i, int b, float ev0,
                                           float ev1, const KmerParam &k0, const KmerParam &k1,
-                                          double lp_step, double lp_stay, double lp_skip) {
+                                          double lp_step, double lp_stay, double skip0,
+                                          double skip1, const EdgeConst &ec, float &best,
+                                          int &best_e) {
   // Suzuki's rule (GPU/align_cuda.cl:398-410) on the scalar unit: slot K
   // (lane h, parity p) and slot K+99 (lane lu, the other parity) are read
   // out of the two registers with lane indices chosen by p, then compared
@@ -411,13 +433,37 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i,
   t.km1s = k1.stdv;
   t.km1r = k1.rstdv;
   t.km1c = k1.c;
-  const Cells c = compute_cells(t, lp_step, lp_stay, lp_skip);
+  const Cells c = compute_cells(t, lp_step, lp_stay, skip0, skip1);
   const int p2 = K & 1;
   const int lb = slot_lane(K - 1), lt = slot_lane(K + 100);
   const float ninf = ABEA_NEG_INF;
   const float M0 = put_lane(c.M0, ninf, p2 ? lb : lt);
-  const float M1 = put_lane(c.M1, ninf, p2 ? lt : lb);
-  commit_cells(s, c, M0, M1, c.u1, c.l1);
+  float M1 = put_lane(c.M1, ninf, p2 ? lt : lb);
+  uint64_t u1 = c.u1, l1 = c.l1;
+  if (EDGE) {
+    if (K < 0) {
+      // the trim cell (b - 1, -1) in slot 127 (lane 63, parity 1), from-code
+      // U, GPU/align_cuda.cl:432-443
+      const int et = b - 1;
+      const float tv = (et >= 0 && et < ec.n_e) ? (float)(ec.lp_trim * (double)(et + 1))
+                                                 : ABEA_NEG_INF;
+      M1 = put_lane(M1, tv, 63);
+      u1 |= 1ull << 63;
+      l1 &= ~(1ull << 63);
+    }
+    // the end search (GPU/align_cuda.cl:722-746) on the last k-mer's cell
+    // (b - 1 - n_k, n_k - 1) while it is in the band's fill range
+    const int os = (ec.n_k - 1) - K, e = b - 1 - ec.n_k;
+    if (os >= 0 && os < ABEA_BANDWIDTH && e >= 0 && e <= ec.n_e - 1) {
+      const float v = __builtin_bit_cast(float, lane_bits((ec.js & 1) ? M1 : M0, ec.js >> 1));
+      const float sc = (float)((double)v + (double)(ec.n_e - e) * ec.lp_trim);
+      if (sc > best) {
+        best = sc;
+        best_e = e;
+      }
+    }
+  }
+  commit_cells(s, c, M0, M1, u1, l1);
 }
 
 struct ReadConst {
@@ -455,7 +501,7 @@ __device__ __forceinline__ void band_general(Slots &s, Walk &w, const ReadConst 
   w.mv |= (uint32_t)right << (b & 31);
   prefetch(w, rc.evp, rc.kmp);
 
-  const Cells c = compute_cells(s, rc.lp_step, rc.lp_stay, rc.lp_skip);
+  const Cells c = compute_cells(s, rc.lp_step, rc.lp_stay, rc.lp_skip, rc.lp_skip);
   const int lo = max(max(-K, E - (rc.n_e - 1)), 0);
   const int hi = min(min(rc.n_k - K, E + 1), ABEA_BANDWIDTH);
   const int span = hi - lo;
@@ -512,10 +558,6 @@ struct TileOps {
   KmerParam k0, k1;
 };
 
-// Smallest lower-left event E0 of a fast tile: the gathers of the tile and
-// of the next one then read events at indices >= E0 - 126 >= 1.
-constexpr int kFastMinE = 127;
-
 // Bands b0 .. b0+7 (b0 a multiple of 8) when all of them are interior.
 //
 // Slot j holds k-mer K0 + ((j - K0) mod 128) for every k-mer the window can
@@ -528,19 +570,22 @@ constexpr int kFastMinE = 127;
 // tile_gather).
 //
 // Index bounds: events b0 - 2 - K0 - off + i = E0 + 1 - off + i lie in
-// [E0 - 126, E0 + 16] = [1, n_e + 7] for E0 >= kFastMinE and E0 + 8 <= n_e - 1,
-// and k-mers K0 + off <= K0 + 127 < n_k + 19; the staged arrays carry that
-// much slack past every read.
+// [E0 - 126, E0 + 16] and k-mers K0 + off in [K0, K0 + 127].  Each read's
+// staged arrays carry kStagePad = 128 padding entries on both sides, so fast
+// tiles may start at -120 <= K0 <= n_k and 0 <= E0 <= n_e + 111.
 struct TileAddr {
   const float *ep0, *ep1;        // event of slot 2l / 2l+1 on the tile's first band
   const KmerParam *kq0, *kq1;    // k-mer of slot 2l / 2l+1
 };
+// Byte offsets are taken from the start of the read's front padding, so that
+// they stay non-negative: event b0 - 2 - K0 - off >= E0 - 126 >= -126.
 __device__ __forceinline__ TileAddr tile_addr(const ReadConst &rc, int b0, int K0, int lane) {
   const uint32_t off0 = (uint32_t)(2 * lane - K0) & 127u, off1 = (uint32_t)(2 * lane + 1 - K0) & 127u;
-  const uint32_t eb = (uint32_t)(b0 - 2 - K0);  // >= off + 1, see above
+  const uint32_t eb = (uint32_t)(b0 - 2 - K0 + kStagePad);  // >= off + 2
+  const char *evq = (const char *)(rc.evp - kStagePad);
   TileAddr t;
-  t.ep0 = (const float *)((const char *)rc.evp + 4u * (eb - off0));
-  t.ep1 = (const float *)((const char *)rc.evp + 4u * (eb - off1));
+  t.ep0 = (const float *)(evq + 4u * (eb - off0));
+  t.ep1 = (const float *)(evq + 4u * (eb - off1));
   t.kq0 = (const KmerParam *)((const char *)(rc.kmp + K0) + 16u * off0);
   t.kq1 = (const KmerParam *)((const char *)(rc.kmp + K0) + 16u * off1);
   return t;
@@ -559,6 +604,17 @@ __device__ __forceinline__ void tile_gather(TileOps &T, const Walk &w, const Rea
   T.k1 = *t.kq1;
 }
 
+// Edge tiles (EDGE) also cover bands whose window reaches outside the matrix
+// on the k side: the trim column while K < 0, and k-mers n_k.. at the right
+// edge.  Cells outside the matrix compute to -inf without masks: their
+// emission is -inf (padding events and k-mers), and each one's left
+// neighbour is such a cell too, except for slot n_k, whose left neighbour
+// (e, n_k - 1) is in the matrix: while that slot holds k-mer n_k (K0 >=
+// n_k - 127) its skip transition is -inf.  The trim cell is set in slot 127
+// while K < 0, and the end search reads the last k-mer's column.  Cells
+// outside the matrix on the e side (above the first event, below the last)
+// compute to -inf the same way in both kinds of tile.
+template <bool EDGE>
 __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc, int b0,
                                           int lane, const TileOps &T, TileOps &N) {
   const int K0 = walk_k(w);
@@ -568,16 +624,33 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
   const float *ep0 = t.ep0, *ep1 = t.ep1;
   N.k0 = *t.kq0;
   N.k1 = *t.kq1;
+  double skip0 = rc.lp_skip, skip1 = rc.lp_skip;
+  EdgeConst ec{rc.n_e, rc.n_k, rc.js, rc.lp_trim};
+  if (EDGE && K0 >= rc.n_k - 127) {
+    const int jn = rc.n_k & 127;
+    if (lane == (jn >> 1)) {
+      if (jn & 1)
+        skip1 = -__builtin_inf();
+      else
+        skip0 = -__builtin_inf();
+    }
+  }
+  float best = w.best;
+  int best_e = w.best_e;
 #pragma unroll
   for (int i = 0; i < 8; i += 2) {
-    band_fast<false>(s, K, gm, i, T.e0[i], T.e1[i], T.k0, T.k1, rc.lp_step, rc.lp_stay,
-                     rc.lp_skip);
+    band_fast<false, EDGE>(s, K, gm, i, b0 + i, T.e0[i], T.e1[i], T.k0, T.k1, rc.lp_step,
+                           rc.lp_stay, skip0, skip1, ec, best, best_e);
     N.e0[i] = ep0[8 + i];
     N.e1[i] = ep1[8 + i];
-    band_fast<true>(s, K, gm, i + 1, T.e0[i + 1], T.e1[i + 1], T.k0, T.k1, rc.lp_step,
-                    rc.lp_stay, rc.lp_skip);
+    band_fast<true, EDGE>(s, K, gm, i + 1, b0 + i + 1, T.e0[i + 1], T.e1[i + 1], T.k0, T.k1,
+                          rc.lp_step, rc.lp_stay, skip0, skip1, ec, best, best_e);
     N.e0[i + 1] = ep0[9 + i];
     N.e1[i + 1] = ep1[9 + i];
+  }
+  if (EDGE) {
+    w.best = best;
+    w.best_e = best_e;
   }
   w.koff = 16u * (uint32_t)(K + 128);
   w.eoff = 4u * (uint32_t)(b0 + 6 - K);  // E + 1 = b - 1 - K after band b = b0+7
@@ -592,16 +665,19 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
 }
 
 // Fast tiles leave the slot registers' events and k-mers, and the walk's
-// scalar prefetches, behind; the general band expects what band b-1 left there: slot j with k-mer K + o and event
-// E - o, o = (j - K) mod 128 (E + K = b - 3).  A negative event index
-// belongs to a slot outside the window and is clamped.  After a fast tile
-// K >= 0, E <= n_e - 1 and K + 127 < n_k + 27, inside the staged slack.
+// scalar prefetches, behind; the general band expects what band b-1 left
+// there: slot j with k-mer K + o and event E - o, o = (j - K) mod 128
+// (E + K = b - 3).  After a fast tile E - o >= -127 and E <= n_e + 119 stay
+// inside the read's event padding; k-mers up to K + 127 <= n_k + 135 may
+// reach 8 entries past its k-mer padding, into the next read's front padding
+// (or the slack after the last read), for slots outside the matrix whose
+// cells the general band masks.
 __device__ __forceinline__ void restore_slots(Slots &s, Walk &w, const ReadConst &rc,
                                               int lane) {
   const int K = walk_k(w), E = walk_e(w);
   const int o0 = (2 * lane - K) & 127, o1 = (2 * lane + 1 - K) & 127;
-  s.ev0 = rc.evp[max(E - o0, 0)];
-  s.ev1 = rc.evp[max(E - o1, 0)];
+  s.ev0 = rc.evp[E - o0];
+  s.ev1 = rc.evp[E - o1];
   const KmerParam p0 = rc.kmp[K + o0], p1 = rc.kmp[K + o1];
   s.km0m = p0.mean;
   s.km0s = p0.stdv;
@@ -624,7 +700,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.lp_step = m.lp_step;
   rc.lp_skip = a.lp_skip;
   rc.lp_trim = a.lp_trim;
-  rc.evp = a.ev + m.ev_off;
+  rc.evp = a.ev + m.sev_off;
   rc.kmp = a.km + m.km_off;
   rc.trace = a.trace + m.trace_off + lane;
   rc.moves = a.moves + m.mv_off;
@@ -632,7 +708,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.replay = a.replay ? (c_u32 *)(a.replay + m.mv_off) : nullptr;
   const int n_e = m.n_e, n_k = m.n_k;
   const int n_bands = n_e + n_k + 2;
-  const float *evp = a.ev + m.ev_off;
+  const float *evp = a.ev + m.sev_off;
   const KmerParam *kmp = a.km + m.km_off;
 
   // band 1: lower-left (E, K) = (50, -51), GPU/align_cuda.cl:251-255; slots
@@ -675,27 +751,42 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   w.best_e = 0;
   prefetch(w, rc.evp, rc.kmp);
 
-  // A whole tile of interior bands takes the fast path (over 8 bands K and E
-  // grow by at most 8).  Fast tiles come in pairs that trade the two operand
-  // buffers, so a tile's gathers for the next one land where that tile reads.
-  auto tile_ok = [&](int b) {
+  // Whole tiles of 8 bands take the fast path (over 8 bands K and E grow by
+  // at most 8): interior tiles (kind 1) when the window stays clear of the
+  // trim column and of the last k-mers, edge tiles (kind 2) otherwise.  The
+  // gathers stay inside the staged padding for -120 <= K0 <= n_k and
+  // 0 <= E0 <= n_e + 111.  Fast tiles come in pairs that trade the two
+  // operand buffers, so a tile's gathers for the next one land where that
+  // tile reads.
+  auto tile_kind = [&](int b) {
     const int K0 = walk_k(w), E0 = walk_e(w);
-    return (b & 7) == 0 && b + 8 <= n_bands && !rc.replay && K0 >= 0 && E0 >= kFastMinE &&
-           K0 + 8 < n_k - ABEA_BANDWIDTH && E0 + 8 <= n_e - 1;
+    if ((b & 7) != 0 || b + 8 > n_bands || rc.replay || K0 < -120 || K0 > n_k || E0 < 0 ||
+        E0 > n_e + 111)
+      return 0;
+    return (K0 >= 0 && K0 + 8 < n_k - ABEA_BANDWIDTH) ? 1 : 2;
   };
   // The slot registers' events and k-mers are dead inside a run of fast
   // tiles and rebuilt when it ends.
   for (int b = 2; b < n_bands;) {
-    if (tile_ok(b)) {
+    int kind = tile_kind(b);
+    if (kind) {
       TileOps T0, T1;
       tile_gather(T0, w, rc, b, lane);
       do {
-        tile_fast(s, w, rc, b, lane, T0, T1);
+        if (kind == 1)
+          tile_fast<false>(s, w, rc, b, lane, T0, T1);
+        else
+          tile_fast<true>(s, w, rc, b, lane, T0, T1);
         b += 8;
-        if (!tile_ok(b)) break;
-        tile_fast(s, w, rc, b, lane, T1, T0);
+        kind = tile_kind(b);
+        if (!kind) break;
+        if (kind == 1)
+          tile_fast<false>(s, w, rc, b, lane, T1, T0);
+        else
+          tile_fast<true>(s, w, rc, b, lane, T1, T0);
         b += 8;
-      } while (tile_ok(b));
+        kind = tile_kind(b);
+      } while (kind);
       if (b >= n_bands) break;
       restore_slots(s, w, rc, lane);
     }
@@ -937,7 +1028,7 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
 
   // ---- phase C: ascending pairs, QC statistics.  Codes are taken 256 at a
   // time (4 runs of 64 consecutive ones) so that their gathers overlap.
-  const float *evp = a.ev + m.ev_off;
+  const float *evp = a.ev + m.sev_off;
   const KmerParam *kmp = a.km + m.km_off;
   abea_aligned_pair_t *pairs = o.pairs + m.pair_off;
   uint8_t *from_out = o.path_from ? o.path_from + m.pair_off : nullptr;
@@ -1142,7 +1233,7 @@ __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const f
     const double scale = (double)sc.scale, shift = (double)sc.shift;
     const abea_aligned_pair_t *pairs = a.out.pairs + m.pair_off;
     const uint8_t *from = a.out.path_from + m.pair_off;
-    const float *evp = a.ev + m.ev_off;
+    const float *evp = a.ev + m.sev_off;
     const uint16_t *kr = a.km_rank + m.km_off;
     double gs = 0.0, gc = 0.0;
     for (int i = lane; i < P; i += 64) {
@@ -1191,7 +1282,7 @@ __global__ __launch_bounds__(256) void abea_refine_kernel(KernelArgs a, abea_sca
     const double scale = (double)sc.scale, shift = (double)sc.shift;
     const abea_aligned_pair_t *pairs = a.out.pairs + m.pair_off;
     const uint8_t *from = a.out.path_from + m.pair_off;
-    const float *evp = a.ev + m.ev_off;
+    const float *evp = a.ev + m.sev_off;
     const uint16_t *kr = a.km_rank + m.km_off;
     double gs = 0.0, gc = 0.0, hss = 0.0, hsc = 0.0, hcc = 0.0;
     for (int i = lane; i < P; i += 64) {
