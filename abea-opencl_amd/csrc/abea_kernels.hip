@@ -344,17 +344,15 @@ struct Cells {
   double left0;
   uint64_t u0, l0, u1, l1;
 };
-// skip0 / skip1: the skip transition of each parity's slots (the same value
-// but for one slot at the right edge of the matrix, see tile_fast).
 __device__ __forceinline__ Cells compute_cells(const Slots &s, double lp_step, double lp_stay,
-                                               double skip0, double skip1) {
+                                               double lp_skip) {
   Cells c;
   c.left0 = rotate_up_f64(s.D1);
   const float em0 = emission(s.ev0, s.km0m, s.km0s, s.km0r, s.km0c);
   const float em1 = emission(s.ev1, s.km1m, s.km1s, s.km1r, s.km1c);
   bool u0, l0, u1, l1;
-  c.M0 = dp_cell(s.Lq, s.D0, c.left0, em0, lp_step, lp_stay, skip0, u0, l0);
-  c.M1 = dp_cell(s.Dq, s.D1, s.D0, em1, lp_step, lp_stay, skip1, u1, l1);
+  c.M0 = dp_cell(s.Lq, s.D0, c.left0, em0, lp_step, lp_stay, lp_skip, u0, l0);
+  c.M1 = dp_cell(s.Dq, s.D1, s.D0, em1, lp_step, lp_stay, lp_skip, u1, l1);
   c.u0 = __builtin_amdgcn_ballot_w64(u0);
   c.l0 = __builtin_amdgcn_ballot_w64(l0);
   c.u1 = __builtin_amdgcn_ballot_w64(u1);
@@ -398,14 +396,15 @@ __device__ __forceinline__ int slot_lane(int k) { return (k >> 1) & 63; }
 // Per-read values an edge tile needs.
 struct EdgeConst {
   int n_e, n_k, js;   // js: slot of the last k-mer (the end-search column)
+  int jn;             // slot of k-mer n_k while it is in the tile's slot range, else -1
   double lp_trim;
 };
 
 template <bool ODD, bool EDGE>
 __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i, int b, float ev0,
                                           float ev1, const KmerParam &k0, const KmerParam &k1,
-                                          double lp_step, double lp_stay, double skip0,
-                                          double skip1, const EdgeConst &ec, float &best,
+                                          double lp_step, double lp_stay, double lp_skip,
+                                          const EdgeConst &ec, float &best,
                                           int &best_e) {
   // Suzuki's rule (GPU/align_cuda.cl:398-410) on the scalar unit: slot K
   // (lane h, parity p) and slot K+99 (lane lu, the other parity) are read
@@ -433,14 +432,25 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i,
   t.km1s = k1.stdv;
   t.km1r = k1.rstdv;
   t.km1c = k1.c;
-  const Cells c = compute_cells(t, lp_step, lp_stay, skip0, skip1);
+  const Cells c = compute_cells(t, lp_step, lp_stay, lp_skip);
   const int p2 = K & 1;
   const int lb = slot_lane(K - 1), lt = slot_lane(K + 100);
   const float ninf = ABEA_NEG_INF;
-  const float M0 = put_lane(c.M0, ninf, p2 ? lb : lt);
+  float M0v = put_lane(c.M0, ninf, p2 ? lb : lt);
   float M1 = put_lane(c.M1, ninf, p2 ? lt : lb);
   uint64_t u1 = c.u1, l1 = c.l1;
   if (EDGE) {
+    // cell (e, n_k) lies outside the matrix, but its skip transition reads
+    // (e, n_k - 1) inside it and no emission: it is set to -inf, the value
+    // a -inf skip would give (with the same decision bits, U and L, since
+    // its other two candidates are -inf).  Cells beyond it then compute to
+    // -inf from -inf neighbours and -inf emissions.
+    if (ec.jn >= 0) {
+      if (ec.jn & 1)
+        M1 = put_lane(M1, ninf, ec.jn >> 1);
+      else
+        M0v = put_lane(M0v, ninf, ec.jn >> 1);
+    }
     if (K < 0) {
       // the trim cell (b - 1, -1) in slot 127 (lane 63, parity 1), from-code
       // U, GPU/align_cuda.cl:432-443
@@ -455,7 +465,7 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i,
     // (b - 1 - n_k, n_k - 1) while it is in the band's fill range
     const int os = (ec.n_k - 1) - K, e = b - 1 - ec.n_k;
     if (os >= 0 && os < ABEA_BANDWIDTH && e >= 0 && e <= ec.n_e - 1) {
-      const float v = __builtin_bit_cast(float, lane_bits((ec.js & 1) ? M1 : M0, ec.js >> 1));
+      const float v = __builtin_bit_cast(float, lane_bits((ec.js & 1) ? M1 : M0v, ec.js >> 1));
       const float sc = (float)((double)v + (double)(ec.n_e - e) * ec.lp_trim);
       if (sc > best) {
         best = sc;
@@ -463,7 +473,7 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i,
       }
     }
   }
-  commit_cells(s, c, M0, M1, u1, l1);
+  commit_cells(s, c, M0v, M1, u1, l1);
 }
 
 struct ReadConst {
@@ -501,7 +511,7 @@ __device__ __forceinline__ void band_general(Slots &s, Walk &w, const ReadConst 
   w.mv |= (uint32_t)right << (b & 31);
   prefetch(w, rc.evp, rc.kmp);
 
-  const Cells c = compute_cells(s, rc.lp_step, rc.lp_stay, rc.lp_skip, rc.lp_skip);
+  const Cells c = compute_cells(s, rc.lp_step, rc.lp_stay, rc.lp_skip);
   const int lo = max(max(-K, E - (rc.n_e - 1)), 0);
   const int hi = min(min(rc.n_k - K, E + 1), ABEA_BANDWIDTH);
   const int span = hi - lo;
@@ -610,7 +620,7 @@ __device__ __forceinline__ void tile_gather(TileOps &T, const Walk &w, const Rea
 // emission is -inf (padding events and k-mers), and each one's left
 // neighbour is such a cell too, except for slot n_k, whose left neighbour
 // (e, n_k - 1) is in the matrix: while that slot holds k-mer n_k (K0 >=
-// n_k - 127) its skip transition is -inf.  The trim cell is set in slot 127
+// n_k - 127) its cell is set to -inf.  The trim cell is set in slot 127
 // while K < 0, and the end search reads the last k-mer's column.  Cells
 // outside the matrix on the e side (above the first event, below the last)
 // compute to -inf the same way in both kinds of tile.
@@ -624,27 +634,19 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
   const float *ep0 = t.ep0, *ep1 = t.ep1;
   N.k0 = *t.kq0;
   N.k1 = *t.kq1;
-  double skip0 = rc.lp_skip, skip1 = rc.lp_skip;
-  EdgeConst ec{rc.n_e, rc.n_k, rc.js, rc.lp_trim};
-  if (EDGE && K0 >= rc.n_k - 127) {
-    const int jn = rc.n_k & 127;
-    if (lane == (jn >> 1)) {
-      if (jn & 1)
-        skip1 = -__builtin_inf();
-      else
-        skip0 = -__builtin_inf();
-    }
-  }
+  // slot n_k & 127 holds k-mer n_k through the tile iff K0 >= n_k - 127
+  const EdgeConst ec{rc.n_e, rc.n_k, rc.js, K0 >= rc.n_k - 127 ? (rc.n_k & 127) : -1,
+                     rc.lp_trim};
   float best = w.best;
   int best_e = w.best_e;
 #pragma unroll
   for (int i = 0; i < 8; i += 2) {
     band_fast<false, EDGE>(s, K, gm, i, b0 + i, T.e0[i], T.e1[i], T.k0, T.k1, rc.lp_step,
-                           rc.lp_stay, skip0, skip1, ec, best, best_e);
+                           rc.lp_stay, rc.lp_skip, ec, best, best_e);
     N.e0[i] = ep0[8 + i];
     N.e1[i] = ep1[8 + i];
     band_fast<true, EDGE>(s, K, gm, i + 1, b0 + i + 1, T.e0[i + 1], T.e1[i + 1], T.k0, T.k1,
-                          rc.lp_step, rc.lp_stay, skip0, skip1, ec, best, best_e);
+                          rc.lp_step, rc.lp_stay, rc.lp_skip, ec, best, best_e);
     N.e0[i + 1] = ep0[9 + i];
     N.e1[i + 1] = ep1[9 + i];
   }
@@ -662,6 +664,34 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
     rc.kck[b0 >> 5] = K;
     w.mv = 0;
   }
+}
+
+// A run of fast tiles of one kind from band b (a multiple of 8 whose tile is
+// of that kind); returns the band after the run.  Tiles come in pairs that
+// trade the two operand buffers, so a tile's gathers for the next one land
+// where that tile reads.  The walk's event index E never falls below its
+// start (50), and K never decreases: an interior run ends for good once the window
+// nears the last k-mer, an edge run at the start when the window has left
+// the trim column.
+template <bool EDGE>
+__device__ __forceinline__ int fast_run(Slots &s, Walk &w, const ReadConst &rc, int b, int lane,
+                                        int n_bands) {
+  auto more = [&](int bb) {
+    const int K0 = walk_k(w), E0 = walk_e(w);
+    if (bb + 8 > n_bands || E0 > rc.n_e + 111) return false;
+    const bool interior = K0 >= 0 && K0 + 8 < rc.n_k - ABEA_BANDWIDTH;
+    return EDGE ? (K0 <= rc.n_k && !interior) : interior;
+  };
+  TileOps T0, T1;
+  tile_gather(T0, w, rc, b, lane);
+  do {
+    tile_fast<EDGE>(s, w, rc, b, lane, T0, T1);
+    b += 8;
+    if (!more(b)) break;
+    tile_fast<EDGE>(s, w, rc, b, lane, T1, T0);
+    b += 8;
+  } while (more(b));
+  return b;
 }
 
 // Fast tiles leave the slot registers' events and k-mers, and the walk's
@@ -755,9 +785,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   // at most 8): interior tiles (kind 1) when the window stays clear of the
   // trim column and of the last k-mers, edge tiles (kind 2) otherwise.  The
   // gathers stay inside the staged padding for -120 <= K0 <= n_k and
-  // 0 <= E0 <= n_e + 111.  Fast tiles come in pairs that trade the two
-  // operand buffers, so a tile's gathers for the next one land where that
-  // tile reads.
+  // 0 <= E0 <= n_e + 111.
   auto tile_kind = [&](int b) {
     const int K0 = walk_k(w), E0 = walk_e(w);
     if ((b & 7) != 0 || b + 8 > n_bands || rc.replay || K0 < -120 || K0 > n_k || E0 < 0 ||
@@ -765,26 +793,15 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
       return 0;
     return (K0 >= 0 && K0 + 8 < n_k - ABEA_BANDWIDTH) ? 1 : 2;
   };
-  // The slot registers' events and k-mers are dead inside a run of fast
-  // tiles and rebuilt when it ends.
+  // The slot registers' events and k-mers are dead inside runs of fast
+  // tiles and rebuilt when they end.  A read's runs go edge (start), interior,
+  // edge (end), each with its own tile loop.
   for (int b = 2; b < n_bands;) {
     int kind = tile_kind(b);
     if (kind) {
-      TileOps T0, T1;
-      tile_gather(T0, w, rc, b, lane);
       do {
-        if (kind == 1)
-          tile_fast<false>(s, w, rc, b, lane, T0, T1);
-        else
-          tile_fast<true>(s, w, rc, b, lane, T0, T1);
-        b += 8;
-        kind = tile_kind(b);
-        if (!kind) break;
-        if (kind == 1)
-          tile_fast<false>(s, w, rc, b, lane, T1, T0);
-        else
-          tile_fast<true>(s, w, rc, b, lane, T1, T0);
-        b += 8;
+        b = kind == 1 ? fast_run<false>(s, w, rc, b, lane, n_bands)
+                      : fast_run<true>(s, w, rc, b, lane, n_bands);
         kind = tile_kind(b);
       } while (kind);
       if (b >= n_bands) break;
