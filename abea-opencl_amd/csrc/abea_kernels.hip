@@ -837,15 +837,21 @@ __device__ __forceinline__ int band_k(const uint32_t *mv, const int32_t *kck, in
   return base + __builtin_popcount(mv[w] & mask);
 }
 
-// Traceback segments: up to kChains per lane, 64 lanes.
-constexpr int kChains = 4;
-constexpr int kMaxSeg = 64 * kChains;
+// Traceback segments: C per lane (chains), 64 lanes.  The fused core walks
+// one chain per lane: every chain starts OVERLAP rows above its segment, so
+// fewer, longer segments cost fewer instructions, and the other waves of
+// the SIMD hide the chain's load latency (36.5 against 37.8 ms per forward
+// of the 16,384-read workload with four).  The separate post kernel, alone
+// on the GPU, keeps four chains per lane to overlap their loads.
+constexpr int kChainsFused = 1, kChainsPost = 4;
 
 // Per-wave LDS record of a read's traceback segments: where the path
 // following each segment's codes entered it (first cell at or below its top
 // row), how many cells it recorded, and where it left (first cell below its
 // bottom row; (-1, -1) when the path ended inside).
+template <int C>
 struct SegInfo {
+  static constexpr int kMaxSeg = 64 * C;
   int ent_k[kMaxSeg], ent_e[kMaxSeg], cnt[kMaxSeg], ex_k[kMaxSeg], ex_e[kMaxSeg];
 };
 
@@ -855,10 +861,10 @@ struct SegInfo {
 // count and exit in si.  The chains are independent, so their trace loads
 // overlap; each keeps the last 8-byte trace chunk it read (4 slots of one
 // 8-band tile), which the next steps usually reuse.
-template <int C>
+template <int C, class SI>
 __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, int S, int b_end,
                                              const int (&sid)[C], int (&k)[C], int (&e)[C],
-                                             SegInfo *si) {
+                                             SI *si) {
   // a segment's entry and exit cells go to its LDS record when they are
   // found (-1 until then), not held in registers
   int lo[C], hi[C], cnt[C], ct[C], cq[C];
@@ -957,8 +963,10 @@ __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, i
 //     with a wave prefix sum over the moves; the QC emission sum runs in the
 //     reference's (traceback) order, in double; max gap from ballots.
 //  4. QC (GPU/align_cuda.cl:856-882).
+template <int C>
 __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, int lane,
-                               int overlap, float *em_buf, SegInfo *si) {
+                               int overlap, float *em_buf, SegInfo<C> *si) {
+  constexpr int kMaxSeg = SegInfo<C>::kMaxSeg;
   const int n_e = m.n_e, n_k = m.n_k;
   const abea_output_t &o = a.out;
   // the end search ran with the band walk (band_general)
@@ -989,8 +997,8 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
   uint8_t *seg = a.seg + m.seg_off;
   const int e_end = uniform(best_e), k_end = n_k - 1;
   const int b_end = e_end + k_end + 2;
-  // rows 2..b_end in up to 256 segments of S rows (S a multiple of 4:
-  // word-aligned code storage per segment), kChains consecutive ones per lane
+  // rows 2..b_end in up to 64 C segments of S rows (S a multiple of 4:
+  // word-aligned code storage per segment), C consecutive ones per lane
   const int rows = b_end - 1;
   const int S = max(4, (((rows + kMaxSeg - 1) / kMaxSeg) + 3) & ~3);
   const int nseg = (rows + S - 1) / S;
@@ -998,10 +1006,10 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
   // ---- phase A: every lane follows its segments, starting overlap rows
   // above each (from the band centre there), or at the end cell
   {
-    int sid[kChains], k[kChains], e[kChains];
+    int sid[C], k[C], e[C];
 #pragma unroll
-    for (int c = 0; c < kChains; ++c) {
-      const int i = lane * kChains + c;
+    for (int c = 0; c < C; ++c) {
+      const int i = lane * C + c;
       sid[c] = i < nseg ? i : -1;
       k[c] = k_end;
       e[c] = e_end;
@@ -1018,7 +1026,7 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
         }
       }
     }
-    chase_chains<kChains>(tw, seg, S, b_end, sid, k, e, si);
+    chase_chains<C>(tw, seg, S, b_end, sid, k, e, si);
   }
   wave_sync();
 
@@ -1184,7 +1192,7 @@ template <int WPB, bool FUSE>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5))) void abea_core_kernel(KernelArgs a, int q_begin, int q_count,
                                                              int persistent, int overlap) {
   __shared__ float em_bufs[WPB][256];
-  __shared__ SegInfo seg_infos[FUSE ? WPB : 1];
+  __shared__ SegInfo<kChainsFused> seg_infos[FUSE ? WPB : 1];
   const int lane = threadIdx.x & 63;
   const int wib = uniform(threadIdx.x >> 6);
   int qi;
@@ -1223,7 +1231,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5))) v
 __global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin, int q_count,
                                                       int overlap) {
   __shared__ float em_buf[256];
-  __shared__ SegInfo seg_info;
+  __shared__ SegInfo<kChainsPost> seg_info;
   const int lane = threadIdx.x;
   for (int qi = blockIdx.x; qi < q_count; qi += gridDim.x) {
     const int r = ((c_i32 *)a.queue)[q_begin + qi];
