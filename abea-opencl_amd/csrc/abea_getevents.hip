@@ -16,11 +16,12 @@
 //            chunks' carries are kept (for long events).  A read that fails
 //            the test takes one serial pass in index order that writes the
 //            prefix sums, and its statistics and events read them.
-//   tstat    one thread per sample (window sums from the chunk's samples in
-//            LDS), the C source's float/double types.
 //   detect   one lane per 512-sample chunk, started fresh a few samples
 //            early (speculation): the chunk's entry state, its boundaries in
-//            emission order and its exit state.
+//            emission order and its exit state.  The lane computes the two
+//            t-statistics of each sample as it goes (the C source's
+//            float/double types): from window sums that slide along the
+//            samples (exact reads), or from the prefix sums.
 //   check / fix   a chunk is right when its entry state is its
 //            predecessor's exit state (the detector is a deterministic state
 //            machine and the first chunk starts from the true initial
@@ -42,15 +43,6 @@ namespace {
 
 constexpr float kThrShort = 1.4f, kThrLong = 9.0f, kPeakHeight = 0.2f;
 constexpr int kWShort = 3, kWLong = 6;
-
-// The t-statistics of read r start at toff(r): 128-byte (cache line)
-// aligned, after the previous read's end (align32(b) - align32(a) + 32 >
-// b - a).  The detector's block loads read up to 15 floats past a read's
-// end: into the next read's statistics, or into the 64 floats allocated past
-// the last read's.
-__device__ __forceinline__ int64_t toff(const GeArgs &a, int r) {
-  return ((a.raw_off[r] + 31) & ~(int64_t)31) + 32 * (int64_t)r;
-}
 
 // --------------------------------------------------------------------------
 // prefix sums
@@ -97,6 +89,7 @@ __device__ __forceinline__ void reduce_sum_info(GeChunkSum &c) {
   c.gx = wave_min_i32(c.gx);
   c.gq = wave_min_i32(c.gq);
   c.bad = wave_max_i32(c.bad);
+  c.hx = wave_max_i32(c.hx);
 }
 
 // One wave per prefix chunk (4 per block): the chunk's sums and exactness
@@ -110,7 +103,7 @@ __global__ __launch_bounds__(256) void abea_ge_chunk_sum_kernel(GeArgs a) {
   const int64_t n = a.raw_off[r + 1] - off;
   const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
   const float *x = a.raw + off;
-  GeChunkSum cs{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, 0};
+  GeChunkSum cs{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, -1000};
 #pragma unroll 4
   for (int64_t i = c0 + lane; i < c1; i += 64) {
     const float v = x[i];
@@ -118,6 +111,7 @@ __global__ __launch_bounds__(256) void abea_ge_chunk_sum_kernel(GeArgs a) {
     const int ex = ulp_exp(v), eq = ulp_exp(sq);
     cs.bad |= (ex == 2000) | (eq == 2000);
     cs.gx = min(cs.gx, ex);
+    cs.hx = max(cs.hx, ex == 1000 ? -1000 : ex);
     cs.gq = min(cs.gq, eq);
     cs.s += (double)v;
     cs.q += (double)sq;
@@ -140,7 +134,7 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
   const int64_t n = a.raw_off[r + 1] - off;
   const int64_t cb = a.chunk_off[r], ce = a.chunk_off[r + 1];
   double *S = a.S + off + r, *Q = a.Q + off + r;
-  GeChunkSum t{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, 0};
+  GeChunkSum t{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, -1000};
   for (int64_t c = cb + lane; c < ce; c += 64) {
     const GeChunkSum &k = a.csum[c];
     t.abs_s += k.abs_s;
@@ -148,13 +142,17 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
     t.gx = min(t.gx, k.gx);
     t.gq = min(t.gq, k.gq);
     t.bad |= k.bad;
+    t.hx = max(t.hx, k.hx);
   }
   reduce_sum_info(t);
   // the sums of magnitudes are rounded: leave a factor-2 margin
   const bool exact = !t.bad && (t.gx >= 1000 || t.abs_s < ldexp(1.0, 51 + t.gx)) &&
                      (t.gq >= 1000 || t.abs_q < ldexp(1.0, 51 + t.gq));
+  // tame: an exact read whose samples are all below 2^60 in magnitude (ulp
+  // below 2^37), so that no window sum of squares overflows float
+  const bool tame = exact && t.hx <= 36;
   if (lane == 0) {
-    a.exact[r] = exact ? 1 : 0;
+    a.exact[r] = exact ? (tame ? 2 : 1) : 0;
     a.first_bad[r] = INT32_MAX;
     if (!exact) {
       S[0] = 0.0;
@@ -189,84 +187,87 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
 }
 
 // --------------------------------------------------------------------------
-// t-statistics, one thread per sample; the C source's float/double types
+// t-statistics, the C source's float/double types
+
+// Division by d = 3 or 6 (3 times a power of two) for the statistics of tame
+// reads (below): a product by y = RN(1/d) and one fused residual correction,
+// Markstein's final step.  RN(1/3) = (1/3)(1 - 2^-54) in double, so x*y
+// rounds to within an ulp of x/d, and the corrected quotient is the correctly
+// rounded one when it is a normal number.  A float quotient is taken the
+// same way in double and then rounded to float: the double quotient is
+// exact enough that the second rounding gives the float quotient.  Both are
+// checked against IEEE division (tests/native/division_check.c): the float
+// route exhaustively over all finite floats (equal but for the sign of a
+// zero quotient, which no statistic sees: it enters only squares and |m2 -
+// m1|), the double one on the inputs a tame read gives.
+constexpr double kRcp3 = 1.0 / 3.0, kRcp6 = 1.0 / 6.0;
+
+__device__ __forceinline__ double div3k(double x, double d, double y) {
+  const double q0 = x * y;
+  return __builtin_fma(__builtin_fma(-q0, d, x), y, q0);
+}
+
+// Correctly rounded square root of a double in [2^-767, 2^1023): the
+// sequence LLVM lowers sqrt(double) to on AMDGPU (a hardware reciprocal
+// square root estimate, then Goldschmidt and two Newton corrections) without
+// the rescaling of smaller inputs and the pass-through of 0 and inf, which
+// these inputs never take.
+__device__ __forceinline__ double sqrt_mid(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+// The statistic from the window before the sample (s1, q1: sums of the
+// samples and of their float squares, double) and the window from it (s2, q2,
+// rounded to float as the C source does).  TAME: the inputs of a tame read,
+// whose divisions by w take the route above (the sums are finite, and zero
+// or at least 2^-149 in magnitude); otherwise IEEE division.
+template <bool TAME>
+__device__ __forceinline__ float tstat_tail(double s1, double q1, float s2, float q2, int w) {
+  const float wf = (float)w;
+  const double wd = (double)w;
+  const double yd = w == kWShort ? kRcp3 : kRcp6;
+  auto d64 = [&](double x) { return TAME ? div3k(x, wd, yd) : x / wd; };
+  auto d32 = [&](float x) { return TAME ? (float)div3k((double)x, wd, yd) : x / wf; };
+  const float m1 = (float)d64(s1);
+  const float m2 = d32(s2);
+  double cv = d64(q1);
+  cv -= (double)(m1 * m1);
+  cv += (double)d32(q2);
+  cv -= (double)(m2 * m2);
+  const float var = fmaxf((float)cv, FLT_MIN);
+  const float dm = m2 - m1;
+  const double v = (double)d32(var);
+  return (float)(fabs((double)dm) / (TAME ? sqrt_mid(v) : sqrt(v)));
+}
+
+// The statistic from the prefix sums (a read whose sums are not exact).
 __device__ __forceinline__ float tstat_at(const double *S, const double *Q, int64_t n, int w,
                                           int64_t i) {
   if (n < 2 * (int64_t)w || i < w || i > n - w) return 0.0f;
-  const float wf = (float)w;
   double s1 = S[i], q1 = Q[i];
   if (i > w) {
     s1 -= S[i - w];
     q1 -= Q[i - w];
   }
-  const float s2 = (float)(S[i + w] - S[i]);
-  const float q2 = (float)(Q[i + w] - Q[i]);
-  const float m1 = (float)(s1 / (double)wf);
-  const float m2 = s2 / wf;
-  double cv = q1 / (double)wf;
-  cv -= (double)(m1 * m1);
-  cv += (double)(q2 / wf);
-  cv -= (double)(m2 * m2);
-  const float var = fmaxf((float)cv, FLT_MIN);
-  const float dm = m2 - m1;
-  return (float)(fabs((double)dm) / sqrt((double)(var / wf)));
+  return tstat_tail<false>(s1, q1, (float)(S[i + w] - S[i]), (float)(Q[i + w] - Q[i]), w);
 }
 
-// The statistic from exact window sums (a read whose sums are exact): the
-// same values as tstat_at's prefix-sum differences.  xs[k] is sample k - 6
-// of the chunk's halo-extended block.
-__device__ __forceinline__ float tstat_win(const float *xs, int64_t n, int w, int64_t i, int k) {
-  if (n < 2 * (int64_t)w || i < w || i > n - w) return 0.0f;
-  const float wf = (float)w;
-  double s1 = 0.0, q1 = 0.0, s2d = 0.0, q2d = 0.0;
-  for (int j = 0; j < w; ++j) {
-    const float a = xs[k - w + j], b = xs[k + j];
-    s1 += (double)a;
-    q1 += (double)(a * a);
-    s2d += (double)b;
-    q2d += (double)(b * b);
-  }
-  const float s2 = (float)s2d, q2 = (float)q2d;
-  const float m1 = (float)(s1 / (double)wf);
-  const float m2 = s2 / wf;
-  double cv = q1 / (double)wf;
-  cv -= (double)(m1 * m1);
-  cv += (double)(q2 / wf);
-  cv -= (double)(m2 * m2);
-  const float var = fmaxf((float)cv, FLT_MIN);
-  const float dm = m2 - m1;
-  return (float)(fabs((double)dm) / sqrt((double)(var / wf)));
-}
-
-// One block per prefix chunk.
-__global__ __launch_bounds__(256) void abea_ge_tstat_kernel(GeArgs a) {
-  __shared__ float xs[kGeChunk + 2 * kWLong];
-  const int64_t c = blockIdx.x;
-  const int r = a.chunk_read[c];
-  const int64_t off = a.raw_off[r];
-  const int64_t n = a.raw_off[r + 1] - off;
-  const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
-  const int64_t to = toff(a, r);
-  if (a.exact[r]) {
-    const float *x = a.raw + off;
-    for (int k = threadIdx.x; k < kGeChunk + 2 * kWLong; k += blockDim.x) {
-      const int64_t i = c0 - kWLong + k;
-      xs[k] = (i >= 0 && i < n) ? x[i] : 0.0f;
-    }
-    __syncthreads();
-    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-      const int k = (int)(i - c0) + kWLong;
-      a.t1[to + i] = tstat_win(xs, n, kWShort, i, k);
-      a.t2[to + i] = tstat_win(xs, n, kWLong, i, k);
-    }
-  } else {
-    const double *S = a.S + off + r, *Q = a.Q + off + r;
-    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-      a.t1[to + i] = tstat_at(S, Q, n, kWShort, i);
-      a.t2[to + i] = tstat_at(S, Q, n, kWLong, i);
-    }
-  }
-}
+// One read's signal as the detector sees it.
+struct GeRead {
+  const float *x;        // the samples
+  const double *S, *Q;   // prefix sums (reads whose sums are not exact)
+  int32_t n;
+  bool exact;            // window sums of the samples are exact in any order
+  bool tame;             // exact, and every sample below 2^60 in magnitude
+};
 
 // --------------------------------------------------------------------------
 // the peak detector
@@ -339,48 +340,140 @@ __device__ __forceinline__ bool same_state(GeState a, GeState b, int32_t at) {
   return eq;
 }
 
+// Sample j of the read; with CHECK, samples outside the read count as 0
+// (where the statistics are 0 anyway).
+template <bool CHECK>
+__device__ __forceinline__ float ge_x(const GeRead &rd, int32_t j) {
+  if (!CHECK) return rd.x[j];
+  const float v = rd.x[min(max(j, 0), rd.n - 1)];
+  return (uint32_t)j < (uint32_t)rd.n ? v : 0.0f;
+}
+
+// One sample of the detector; boundaries at samples >= keep_from go to pk.
+__device__ __forceinline__ void ge_step(GeState &st, float t1, float t2, int32_t i,
+                                        int32_t keep_from, int32_t *pk, int32_t &cnt,
+                                        GeState *entry) {
+  int32_t p0 = 0, p1 = 0;
+  const int ne = detect_step(st, t1, t2, i, p0, p1);
+  if (i >= keep_from) {
+    if (ne > 0) pk[cnt++] = p0;
+    if (ne > 1) pk[cnt++] = p1;
+  }
+  if (entry && i + 1 == keep_from) *entry = st;
+}
+
+// v where c holds, else 0, computed where it stands.  The empty asm
+// statements hide the mask's value and pin the result in place: otherwise
+// the compiler moves each statistic's computation under a branch of its
+// own (the select, or the detector step that uses it), which serialises
+// the statistics of an iteration.
+__device__ __forceinline__ float keep_if(float v, bool c) {
+  uint32_t m = c ? 0xffffffffu : 0u;
+  asm volatile("" : "+v"(m));
+  float r = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & m);
+  asm volatile("" : "+v"(r));
+  return r;
+}
+
+// An exact read: the statistics from window sums that slide along with the
+// samples.  B3 and B6 are the sums (and sums of float squares) of the windows
+// from the sample, w = 3 and 6; each step adds the sample that enters them
+// and takes off the one that leaves, exactly (every partial sum of an exact
+// read is exact).  The window before sample i is the one from i - w, kept in
+// a short history (h3, h6).  Six samples per iteration: their twelve
+// statistics first, as independent straight-line code that the compiler
+// interleaves (each is a long chain of dependent double operations), then
+// the six detector steps.  CHECK: an iteration may load past the read's end
+// (within 18 samples of it).  TAME: the read is tame (tstat_tail).
+template <bool CHECK, bool TAME>
+__device__ void run_exact(GeState &st, const GeRead &rd, int32_t from, int32_t to,
+                          int32_t keep_from, int32_t *pk, int32_t &cnt, GeState *entry) {
+  const int32_t n = rd.n;
+  auto win = [&](int32_t i, int w, double &s, double &q) {
+    s = 0.0;
+    q = 0.0;
+    for (int j = 0; j < w; ++j) {
+      const float v = ge_x<true>(rd, i + j);
+      s += (double)v;
+      q += (double)(v * v);
+    }
+  };
+  // h3[j] = B3 at from + j - 3, h6[j] = B6 at from + j - 6
+  double h3s[kWShort], h3q[kWShort], h6s[kWLong], h6q[kWLong];
+  double b3s, b3q, b6s, b6q;
+#pragma unroll
+  for (int j = 0; j < kWShort; ++j) win(from + j - kWShort, kWShort, h3s[j], h3q[j]);
+#pragma unroll
+  for (int j = 0; j < kWLong; ++j) win(from + j - kWLong, kWLong, h6s[j], h6q[j]);
+  win(from, kWShort, b3s, b3q);
+  win(from, kWLong, b6s, b6q);
+  // xb: samples i0 .. i0 + 11 of the iteration; the next iteration's six
+  // new ones load while this one computes
+  float xb[2 * kWLong];
+#pragma unroll
+  for (int j = 0; j < 2 * kWLong; ++j) xb[j] = ge_x<CHECK>(rd, from + j);
+  for (int32_t i0 = from; i0 < to; i0 += kWLong) {
+    float nx[kWLong];
+#pragma unroll
+    for (int j = 0; j < kWLong; ++j) nx[j] = ge_x<CHECK>(rd, i0 + 2 * kWLong + j);
+    float t1[kWLong], t2[kWLong];
+#pragma unroll
+    for (int k = 0; k < kWLong; ++k) {
+      const int32_t i = i0 + k;
+      const int k3 = k % kWShort;
+      const float u1 = tstat_tail<TAME>(h3s[k3], h3q[k3], (float)b3s, (float)b3q, kWShort);
+      const float u2 = tstat_tail<TAME>(h6s[k], h6q[k], (float)b6s, (float)b6q, kWLong);
+      t1[k] = keep_if(u1, n >= 2 * kWShort && i >= kWShort && i <= n - kWShort);
+      t2[k] = keep_if(u2, n >= 2 * kWLong && i >= kWLong && i <= n - kWLong);
+      h3s[k3] = b3s;
+      h3q[k3] = b3q;
+      h6s[k] = b6s;
+      h6q[k] = b6q;
+      const float x0 = xb[k], x3 = xb[k + kWShort], x6 = xb[k + kWLong];
+      const double d0 = (double)x0, e0 = (double)(x0 * x0);
+      b3s = (b3s + (double)x3) - d0;
+      b3q = (b3q + (double)(x3 * x3)) - e0;
+      b6s = (b6s + (double)x6) - d0;
+      b6q = (b6q + (double)(x6 * x6)) - e0;
+    }
+#pragma unroll
+    for (int k = 0; k < kWLong; ++k)
+      if (i0 + k < to) ge_step(st, t1[k], t2[k], i0 + k, keep_from, pk, cnt, entry);
+#pragma unroll
+    for (int j = 0; j < kWLong; ++j) {
+      xb[j] = xb[j + kWLong];
+      xb[j + kWLong] = nx[j];
+    }
+  }
+}
+
 // Run samples [from, to) of a read from state st; boundaries emitted at
-// samples >= keep_from go to pk (count in *np).  from is a multiple of 16 and
-// t1, t2 are cache-line aligned: the statistics come in blocks of 16 samples
-// (float4 loads), the next block loading while this one runs.
-__device__ void run_detector(GeState &st, const float *t1, const float *t2, int32_t from,
-                             int32_t to, int32_t keep_from, int32_t *pk, int32_t *np,
-                             GeState *entry) {
+// samples >= keep_from go to pk (count in *np), the state before keep_from
+// to *entry.  The statistics come with the samples: from sliding window sums
+// (exact reads) or from the prefix sums.
+__device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_t to,
+                             int32_t keep_from, int32_t *pk, int32_t *np, GeState *entry) {
   int32_t cnt = 0;
   if (entry && from == keep_from) *entry = st;
-  const float4 *q1 = reinterpret_cast<const float4 *>(t1), *q2 = reinterpret_cast<const float4 *>(t2);
-  float4 n1[4], n2[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    n1[k] = q1[from / 4 + k];
-    n2[k] = q2[from / 4 + k];
-  }
-  for (int32_t i0 = from; i0 < to; i0 += 16) {
-    float b1[16], b2[16];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      b1[4 * k] = n1[k].x, b1[4 * k + 1] = n1[k].y, b1[4 * k + 2] = n1[k].z, b1[4 * k + 3] = n1[k].w;
-      b2[4 * k] = n2[k].x, b2[4 * k + 1] = n2[k].y, b2[4 * k + 2] = n2[k].z, b2[4 * k + 3] = n2[k].w;
+  if (rd.exact) {
+    // an iteration loads samples up to i0 + 17 < to + 3 * 6
+    const bool check = to + 3 * kWLong > rd.n;
+    if (rd.tame) {
+      if (check)
+        run_exact<true, true>(st, rd, from, to, keep_from, pk, cnt, entry);
+      else
+        run_exact<false, true>(st, rd, from, to, keep_from, pk, cnt, entry);
+    } else {
+      if (check)
+        run_exact<true, false>(st, rd, from, to, keep_from, pk, cnt, entry);
+      else
+        run_exact<false, false>(st, rd, from, to, keep_from, pk, cnt, entry);
     }
-    if (i0 + 16 < to) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        n1[k] = q1[(i0 + 16) / 4 + k];
-        n2[k] = q2[(i0 + 16) / 4 + k];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int32_t i = i0 + k;
-      if (i < to) {
-        int32_t p0 = 0, p1 = 0;
-        const int ne = detect_step(st, b1[k], b2[k], i, p0, p1);
-        if (i >= keep_from) {
-          if (ne > 0) pk[cnt++] = p0;
-          if (ne > 1) pk[cnt++] = p1;
-        }
-        if (entry && i + 1 == keep_from) *entry = st;
-      }
+  } else {
+    for (int32_t i = from; i < to; ++i) {
+      const float t1 = tstat_at(rd.S, rd.Q, rd.n, kWShort, i);
+      const float t2 = tstat_at(rd.S, rd.Q, rd.n, kWLong, i);
+      ge_step(st, t1, t2, i, keep_from, pk, cnt, entry);
     }
   }
   *np = cnt;
@@ -389,7 +482,7 @@ __device__ void run_detector(GeState &st, const float *t1, const float *t2, int3
 // One lane per detector chunk: a fresh run from a.overlap samples before the
 // chunk.  At most 2 * (ceil(512 / 3) + 1) = 344 boundaries: a detector that
 // emits is back to no peak and needs 3 more samples to emit again.
-__global__ __launch_bounds__(256) void abea_ge_detect_kernel(GeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void abea_ge_detect_kernel(GeArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_det) return;
   const int r = a.det_read[c];
@@ -397,11 +490,11 @@ __global__ __launch_bounds__(256) void abea_ge_detect_kernel(GeArgs a) {
   const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
   const int32_t c0 = (int32_t)(c - a.det_off[r]) * kGeDetChunk;
   const int32_t c1 = min(c0 + kGeDetChunk, n);
-  const int32_t from = max(c0 - a.overlap, 0) & ~15;  // block-aligned (a longer lead-in)
+  const int32_t from = max(c0 - a.overlap, 0);
   GeState st = fresh_state();
-  const int64_t to = toff(a, r);
-  run_detector(st, a.t1 + to, a.t2 + to, from, c1, c0, a.peaks + c * kGeDetChunk,
-               a.n_peaks + c, a.entry + c);
+  const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
+                  a.exact[r] == 2};
+  run_detector(st, rd, from, c1, c0, a.peaks + c * kGeDetChunk, a.n_peaks + c, a.entry + c);
   a.exit[c] = st;
 }
 
@@ -428,14 +521,16 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
     const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
     const int64_t cb = a.det_off[r];
     GeState st = a.exit[cb + fb - 1];
+    const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
+                    a.exact[r] == 2};
     for (int64_t c = cb + fb; c < a.det_off[r + 1]; ++c) {
       const int32_t c0 = (int32_t)(c - cb) * kGeDetChunk;
       if (same_state(a.entry[c], st, c0)) {
         st = a.exit[c];
       } else {
         ++reruns;
-        run_detector(st, a.t1 + toff(a, r), a.t2 + toff(a, r), c0, min(c0 + kGeDetChunk, n), c0,
-                     a.peaks + c * kGeDetChunk, a.n_peaks + c, nullptr);
+        run_detector(st, rd, c0, min(c0 + kGeDetChunk, n), c0, a.peaks + c * kGeDetChunk,
+                     a.n_peaks + c, nullptr);
       }
     }
   }
@@ -443,7 +538,7 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
 }
 
 // One wave per read (4 per block): the boundaries gathered in emission order
-// (into the read's t1 space, free by now) and the event count.
+// and the event count.
 __global__ __launch_bounds__(256) void abea_ge_compact_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -560,8 +655,6 @@ hipError_t launch_getevents(const GeArgs &a, hipStream_t s) {
   const unsigned db = (unsigned)((a.n_det + 255) / 256);   // a lane per detector chunk
   if (a.n_chunks > 0) hipLaunchKernelGGL(abea_ge_chunk_sum_kernel, dim3(cb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(abea_ge_read_scan_kernel, dim3(rb), dim3(256), 0, s, a);
-  if (a.n_chunks > 0)
-    hipLaunchKernelGGL(abea_ge_tstat_kernel, dim3((unsigned)a.n_chunks), dim3(256), 0, s, a);
   if (a.n_det > 0) {
     hipLaunchKernelGGL(abea_ge_detect_kernel, dim3(db), dim3(256), 0, s, a);
     hipLaunchKernelGGL(abea_ge_check_kernel, dim3(db), dim3(256), 0, s, a);

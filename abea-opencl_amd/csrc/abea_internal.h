@@ -136,8 +136,8 @@ struct MomArgs {
 };
 
 // Event detection (getevents), one read per raw signal.  Samples are cut
-// into kGeChunk-sample chunks for the prefix sums, the t-statistics and the
-// events, and into kGeDetChunk-sample chunks for the peak detector.
+// into kGeChunk-sample chunks for the prefix sums and the events, and into
+// kGeDetChunk-sample chunks for the peak detector (with the t-statistics).
 constexpr int kGeChunk = 2048;
 constexpr int kGeDetChunk = 512;
 constexpr int kGeOverlap = 64;  // samples a detector chunk's speculative run starts early
@@ -157,7 +157,7 @@ struct GeChunkSum {
   double abs_s, abs_q;  // sums of magnitudes (upper bounds of every partial sum)
   int32_t gx, gq;     // smallest ulp exponent of a sample / square (1000: all zero)
   int32_t bad;        // an inf or NaN
-  int32_t pad;
+  int32_t hx;         // largest ulp exponent of a nonzero sample (-1000: none)
 };
 struct GeArgs {
   int32_t n_reads;
@@ -167,13 +167,13 @@ struct GeArgs {
   const int64_t *raw_off;     // n_reads + 1: read r is raw[raw_off[r] .. raw_off[r+1])
   const float *raw;
   double *S, *Q;              // prefix sums of serial reads, read r at raw_off[r] + r (n + 1)
-  float *t1, *t2;             // t-statistics, read r from align32(raw_off[r]) + 32r
-  int32_t *bounds;            // compacted boundaries, read r at raw_off[r] (reuses t1)
+  int32_t *bounds;            // compacted boundaries, read r at raw_off[r]
   const int64_t *chunk_off;   // n_reads + 1: read r's prefix chunks
   const int32_t *chunk_read;  // the read of each prefix chunk
   GeChunkSum *csum;           // per prefix chunk
   double *carry;              // per prefix chunk: S and Q before it (2 doubles)
-  int32_t *exact;             // per read: 1 when the parallel sums are exact
+  int32_t *exact;             // per read: 1 when the parallel sums are exact, 2 when also
+                              // every sample is below 2^60 in magnitude (tame)
   const int64_t *det_off;     // n_reads + 1: read r's detector chunks
   const int32_t *det_read;    // the read of each detector chunk
   int32_t *peaks;             // kGeDetChunk boundaries per detector chunk, in emission order
