@@ -88,8 +88,8 @@ void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_
 
 // Event-detection scratch of a context (grows, never shrinks).
 struct GeScratch {
-  DevBuf raw_off, chunk_off, chunk_read, det_off, det_read, S, Q, bounds, csum, carry, exact,
-      peaks, n_peaks, entry, exit, first_bad, reruns;
+  DevBuf raw_off, chunk_off, chunk_read, det_off, det_read, S, Q, ev_base, prev_last, csum, carry,
+      exact, peaks, n_peaks, entry, exit, first_bad, reruns;
   std::vector<int64_t> raw_off_h, chunk_off_h, det_off_h;
   std::vector<int32_t> chunk_read_h, det_read_h;
   int32_t n_reads = 0;
@@ -248,7 +248,8 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
       (st = g.det_read.ensure(sizeof(int32_t) * nd1)) ||
       (st = g.S.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
       (st = g.Q.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
-      (st = g.bounds.ensure(sizeof(int32_t) * (size_t)(extent + 1))) ||
+      (st = g.ev_base.ensure(sizeof(int32_t) * nd1)) ||
+      (st = g.prev_last.ensure(sizeof(int32_t) * nd1)) ||
       (st = g.csum.ensure(sizeof(abea::GeChunkSum) * nc1)) ||
       (st = g.carry.ensure(2 * sizeof(double) * nc1)) ||
       (st = g.exact.ensure(sizeof(int32_t) * (size_t)n)) ||
@@ -280,7 +281,8 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
   a.raw = raw;
   a.S = g.S.as<double>();
   a.Q = g.Q.as<double>();
-  a.bounds = g.bounds.as<int32_t>();
+  a.ev_base = g.ev_base.as<int32_t>();
+  a.prev_last = g.prev_last.as<int32_t>();
   a.chunk_off = g.chunk_off.as<int64_t>();
   a.chunk_read = g.chunk_read.as<int32_t>();
   a.csum = g.csum.as<abea::GeChunkSum>();

@@ -28,7 +28,8 @@
 //            state); from a read's first chunk where that fails, one lane
 //            walks on, re-running each chunk whose entry is not the true
 //            state.  The result is the serial one.
-//   compact / events   boundaries gathered per read, one thread per event.
+//   scan / events   per detector chunk, the index of its first event and the
+//            boundary before it; one thread per event.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -537,24 +538,37 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
   a.reruns[r] = reruns;
 }
 
-// One wave per read (4 per block): the boundaries gathered in emission order
-// and the event count.
-__global__ __launch_bounds__(256) void abea_ge_compact_kernel(GeArgs a) {
+// One wave per read (4 per block): where each detector chunk's events go.
+// Chunk c's boundaries end events ev_base[c] .. ev_base[c] + n_peaks[c] - 1
+// of the read (an exclusive scan of the counts); the event that ends at its
+// first boundary starts at prev_last[c], the last boundary in the chunks
+// before it (an exclusive maximum scan: boundaries increase along the read;
+// 0 where there is none).  Also the event count.
+__global__ __launch_bounds__(256) void abea_ge_scan_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (r >= a.n_reads) return;
-  const int64_t off = a.raw_off[r];
-  const int64_t n = a.raw_off[r + 1] - off;
-  int32_t *bd = a.bounds + off;
+  const int64_t n = a.raw_off[r + 1] - a.raw_off[r];
   const int64_t cb = a.det_off[r], ce = a.det_off[r + 1];
-  int64_t base = 0;
+  int32_t base = 0, last = 0;
   for (int64_t c0 = cb; c0 < ce; c0 += 64) {
     const int64_t c = c0 + lane;
     const int32_t cnt = c < ce ? a.n_peaks[c] : 0;
+    const int32_t lst = cnt > 0 ? a.peaks[c * kGeDetChunk + cnt - 1] : 0;
     const int incl = wave_scan_add(cnt);
-    const int64_t j = base + incl - cnt;
-    for (int32_t k = 0; k < cnt; ++k) bd[j + k] = a.peaks[c * kGeDetChunk + k];
+    int32_t mx = lst;  // inclusive maximum scan
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(mx, o, 64);
+      if (lane >= o) mx = max(mx, v);
+    }
+    const int32_t before = __shfl_up(mx, 1, 64);
+    if (c < ce) {
+      a.ev_base[c] = base + incl - cnt;
+      a.prev_last[c] = max(last, lane > 0 ? before : 0);
+    }
     base += __shfl(incl, 63, 64);
+    last = max(last, __shfl(mx, 63, 64));
   }
   if (lane == 0) a.n_events[r] = n > 0 ? base + 1 : 0;
 }
@@ -598,13 +612,25 @@ __device__ __forceinline__ abea_event_t make_event_exact(const GeArgs &a, const 
                                                          int64_t end) {
   double ds, dq;
   if (start <= end && end - start <= 2 * kGeChunk) {
-    ds = 0.0;
-    dq = 0.0;
-    for (int64_t k = start; k < end; ++k) {
-      const float v = x[k];
-      ds += (double)v;
-      dq += (double)(v * v);
+    // four samples per step into four partial sums (exact in any order),
+    // their loads in flight together
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, q[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t k = start;
+    for (; k + 4 <= end; k += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float v = x[k + u];
+        s[u] += (double)v;
+        q[u] += (double)(v * v);
+      }
     }
+    for (; k < end; ++k) {
+      const float v = x[k];
+      s[0] += (double)v;
+      q[0] += (double)(v * v);
+    }
+    ds = (s[0] + s[1]) + (s[2] + s[3]);
+    dq = (q[0] + q[1]) + (q[2] + q[3]);
   } else {
     double s0, q0, s1, q1;
     exact_prefix(a, x, cbase, start, s0, q0);
@@ -622,27 +648,31 @@ __device__ __forceinline__ abea_event_t make_event_exact(const GeArgs &a, const 
   return e;
 }
 
-// One block per prefix chunk: events j in [2048 lc, 2048 (lc + 1)) of the
-// read (the last chunk takes any rest), one thread each.  Event j runs from
-// boundary j - 1 (0 for the first) to boundary j (n for the last).
+// One wave per detector chunk (4 per block): the events that end at its
+// boundaries, and for a read's last chunk also the event after the read's
+// last boundary (to the end of the signal).  Event j runs from boundary
+// j - 1 (0 for the first) to boundary j (n for the last).
 __global__ __launch_bounds__(256) void abea_ge_events_kernel(GeArgs a) {
-  const int64_t c = blockIdx.x;
-  const int r = a.chunk_read[c];
-  const int64_t cbase = a.chunk_off[r];
-  const int64_t lc = c - cbase, nch = a.chunk_off[r + 1] - cbase;
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= a.n_det) return;
+  const int r = a.det_read[c];
   const int64_t off = a.raw_off[r];
   const int64_t n = a.raw_off[r + 1] - off;
-  const int64_t m = a.n_events[r];
-  const int64_t j0 = lc * kGeChunk, j1 = lc == nch - 1 ? m : min(j0 + kGeChunk, m);
-  const int32_t *bd = a.bounds + off;
+  const int32_t cnt = a.n_peaks[c];
+  const int32_t m = cnt + (c == a.det_off[r + 1] - 1 ? 1 : 0);
+  const int32_t *pk = a.peaks + c * kGeDetChunk;
+  const int64_t jb = a.ev_base[c];
+  const int32_t pl = a.prev_last[c];
   abea_event_t *ev = a.events + off + r;
   const bool exact = a.exact[r] != 0;
+  const int64_t cbase = a.chunk_off[r];
   const double *S = a.S + off + r, *Q = a.Q + off + r;
   const float *x = a.raw + off;
-  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
-    const int64_t start = j == 0 ? 0 : bd[j - 1];
-    const int64_t end = j == m - 1 ? n : bd[j];
-    ev[j] = exact ? make_event_exact(a, x, cbase, start, end) : make_event(start, end, S, Q);
+  for (int32_t k = lane; k < m; k += 64) {
+    const int64_t start = k == 0 ? pl : pk[k - 1];
+    const int64_t end = k < cnt ? pk[k] : n;
+    ev[jb + k] = exact ? make_event_exact(a, x, cbase, start, end) : make_event(start, end, S, Q);
   }
 }
 
@@ -661,9 +691,10 @@ hipError_t launch_getevents(const GeArgs &a, hipStream_t s) {
   }
   hipLaunchKernelGGL(abea_ge_fix_kernel, dim3((unsigned)((a.n_reads + 255) / 256)), dim3(256), 0,
                      s, a);
-  hipLaunchKernelGGL(abea_ge_compact_kernel, dim3(rb), dim3(256), 0, s, a);
-  if (a.n_chunks > 0)
-    hipLaunchKernelGGL(abea_ge_events_kernel, dim3((unsigned)a.n_chunks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(abea_ge_scan_kernel, dim3(rb), dim3(256), 0, s, a);
+  if (a.n_det > 0)
+    hipLaunchKernelGGL(abea_ge_events_kernel, dim3((unsigned)((a.n_det + 3) / 4)), dim3(256), 0,
+                       s, a);
   return hipGetLastError();
 }
 

@@ -167,7 +167,6 @@ struct GeArgs {
   const int64_t *raw_off;     // n_reads + 1: read r is raw[raw_off[r] .. raw_off[r+1])
   const float *raw;
   double *S, *Q;              // prefix sums of serial reads, read r at raw_off[r] + r (n + 1)
-  int32_t *bounds;            // compacted boundaries, read r at raw_off[r]
   const int64_t *chunk_off;   // n_reads + 1: read r's prefix chunks
   const int32_t *chunk_read;  // the read of each prefix chunk
   GeChunkSum *csum;           // per prefix chunk
@@ -179,6 +178,8 @@ struct GeArgs {
   int32_t *peaks;             // kGeDetChunk boundaries per detector chunk, in emission order
   int32_t *n_peaks;           // per detector chunk
   GeState *entry, *exit;      // per detector chunk: state before its first / after its last sample
+  int32_t *ev_base;           // per detector chunk: the read's event ending at its first boundary
+  int32_t *prev_last;         // per detector chunk: the read's last boundary before it (0: none)
   int32_t *first_bad;         // per read: first detector chunk whose entry is not its
                               // predecessor's exit (INT32_MAX: none)
   int32_t *reruns;            // per read: detector chunks re-run from the true state
