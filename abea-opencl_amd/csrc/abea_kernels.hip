@@ -945,6 +945,79 @@ __device__ __forceinline__ void chase_chains(const uint32_t *tw, uint8_t *seg, i
   }
 }
 
+// Follow the from-codes of one segment with the whole wave, from cell (k, e)
+// (wave-uniform), recording as chase_chains does.  The trace comes in blocks
+// of kBlockTiles 8-band tiles: each lane loads its word of every tile of the
+// block at once (coalesced across the wave), and the path's codes are read
+// out of the registers with v_readlane, so the walk through a block waits on
+// no load.  The codes collect in a register, a word per lane, and are
+// stored 256 at a time.  Where speculation failed this replaces one lane's
+// chain, which waits on a load every few steps.
+constexpr int kBlockTiles = 16;
+template <class SI>
+__device__ __forceinline__ void chase_wave(const uint32_t *tw, uint8_t *seg, int S, int b_end,
+                                           int sid, int k, int e, SI *si, int lane) {
+  const int lo = 2 + sid * S, hi = min(lo + S - 1, b_end);
+  int ent_k = -1, ent_e = -1, ex_k = -1, ex_e = -1, cnt = 0;
+  uint32_t word = 0, outv = 0;
+  bool have = false, done = false;
+  uint32_t *out = (uint32_t *)(seg + sid * S);
+  int t0 = (e + k + 2) >> 3;  // the block's top tile
+  while (!done) {
+    uint32_t blk[kBlockTiles];
+#pragma unroll
+    for (int j = 0; j < kBlockTiles; ++j)
+      blk[j] = t0 - j >= 0 ? tw[(t0 - j) * kTileWords + lane] : 0u;
+    for (;;) {
+      const int b = e + k + 2;
+      if (!have && b <= hi) {
+        have = true;
+        ent_k = k;
+        ent_e = e;
+      }
+      if (b < lo) {
+        ex_k = k;
+        ex_e = e;
+        done = true;
+        break;
+      }
+      const int j = t0 - (b >> 3);
+      if (j >= kBlockTiles) break;  // the next block
+      const uint32_t wd = (uint32_t)__builtin_amdgcn_readlane((int)blk[j], (k & 127) >> 1);
+      const uint32_t bits = (wd >> (4 * (7 - (b & 7)) + 2 * (1 - (k & 1)))) & 3u;
+      const uint32_t f = (bits & 1u) ? FROM_L : (bits >> 1);
+      if (b <= hi) {
+        word |= f << (8 * (cnt & 3));
+        if ((cnt & 3) == 3) {
+          const int wi = cnt >> 2;
+          outv = lane == (wi & 63) ? word : outv;
+          if ((wi & 63) == 63) out[wi - 63 + lane] = outv;
+          word = 0;
+        }
+        ++cnt;
+      }
+      k -= (f != FROM_U) ? 1 : 0;
+      e -= (f != FROM_L) ? 1 : 0;
+      if (k < 0 || e < 0) {  // the path ended: exit stays (-1, -1)
+        done = true;
+        break;
+      }
+    }
+    t0 -= kBlockTiles;
+  }
+  // the words not yet stored: those of the last, partial group of 64
+  const int nw = cnt >> 2;  // complete words
+  if (lane < (nw & 63)) out[(nw & ~63) + lane] = outv;
+  if (lane == 0) {
+    if (cnt & 3) out[nw] = word;
+    si->ent_k[sid] = ent_k;
+    si->ent_e[sid] = ent_e;
+    si->ex_k[sid] = ex_k;
+    si->ex_e[sid] = ex_e;
+    si->cnt[sid] = cnt;
+  }
+}
+
 // Traceback of one read by one wavefront (the body of the post kernel, and
 // the tail of the fused core).
 //  1. best end cell on the last k-mer (GPU/align_cuda.cl:722-746), wave-wide
@@ -1039,10 +1112,7 @@ __device__ void traceback_read(const KernelArgs &a, const ReadMeta &m, int r, in
       continue;
     }
     if (si->ent_k[i] != tk || si->ent_e[i] != te) {
-      if (lane == 0) {
-        int sid1[1] = {i}, k1[1] = {tk}, e1[1] = {te};
-        chase_chains<1>(tw, seg, S, b_end, sid1, k1, e1, si);
-      }
+      chase_wave(tw, seg, S, b_end, i, tk, te, si, lane);
       wave_sync();
     }
     P += si->cnt[i];
