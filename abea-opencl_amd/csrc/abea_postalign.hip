@@ -121,21 +121,6 @@ __device__ __forceinline__ void add_in_order(Stage &st, bool m, double t, double
   wave_lds_sync();
 }
 
-// Two running sums over the lanes with m set, in lane order.
-__device__ __forceinline__ void add_in_order2(Stage &st, bool m, double t0, double t1,
-                                              double &s0, double &s1) {
-  const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
-  if (m) *reinterpret_cast<double2 *>(st.t + 2 * lanes_below(mask)) = make_double2(t0, t1);
-  wave_lds_sync();
-  const int cnt = __builtin_popcountll(mask);
-  for (int j = 0; j < cnt; ++j) {
-    const double2 p = *reinterpret_cast<const double2 *>(st.t + 2 * j);
-    s0 += p.x;
-    s1 += p.y;
-  }
-  wave_lds_sync();
-}
-
 __device__ __forceinline__ double var_term(float xf, float muf, float sdf, double shift,
                                            double scale) {
   const double x = xf, mu = muf, sd = sdf;
