@@ -3,8 +3,9 @@ oracle restatement (oracle/getevents.c): event tables byte for byte.
 
 Covers signal lengths around the window and chunk sizes, empty and tiny
 signals, the serial prefix-sum path (terms too fine for an exact parallel
-sum), forced speculation misses (no lead-in), dyadic step signals with
-known cuts, reads at a non-zero offset, and the host drop-ins."""
+sum), exact signals too large for the statistics' fast divisions, forced
+speculation misses (no lead-in), dyadic step signals with known cuts, reads
+at a non-zero offset, and the host drop-ins."""
 import ctypes as C
 import os
 import sys
@@ -114,6 +115,23 @@ def test_serial_prefix_path(ctx):
     got = _run_device(ctx, sigs)
     for s, g in zip(sigs, got):
         assert _events_equal(g, oracle.getevents(s)), _first_diff(g, oracle.getevents(s))
+
+
+def test_large_samples_take_ieee_statistics(ctx):
+    # pA signals scaled by 2^54 (largest samples above 2^60, their squares
+    # still finite) keep their sums exact but leave the tame range, so their
+    # statistics divide by IEEE division.  A power of two scales every sum,
+    # mean and square exactly, and these signals are cut where the unscaled
+    # ones are
+    sigs = _signals([5000, 777, 12000], seed=13)
+    big = [(s * np.float32(2.0 ** 54)).astype(np.float32) for s in sigs]
+    assert all(float(np.abs(b).max()) >= 2.0 ** 60 for b in big)
+    got = _run_device(ctx, big)
+    small = _run_device(ctx, sigs)
+    for b, g, h in zip(big, got, small):
+        want = oracle.getevents(b)
+        assert _events_equal(g, want), _first_diff(g, want)
+        assert np.array_equal(g["start"], h["start"]) and np.array_equal(g["length"], h["length"])
 
 
 def test_step_signals_and_offset_reads(ctx):
