@@ -672,6 +672,21 @@ __device__ __forceinline__ void mom_load(const MomRead &d, int i, MomLoad &L) {
   if (d.nk > 0) kmer_bytes(d.S, min(i, d.nk - 1), L.lo, L.hi);
 }
 
+// acc += v[0] + v[1] + ... + v[63], one add at a time in index order, the
+// LDS loads of 16 terms issued ahead of their adds.  The entries past a
+// chunk's last term are +0.0, which leaves acc unchanged: acc starts at +0
+// and is never -0 (a sum that cancels exactly rounds to +0).
+__device__ __forceinline__ void serial_sum64(const double *v, double &acc) {
+#pragma unroll
+  for (int j0 = 0; j0 < 64; j0 += 16) {
+    double t[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t[j] = v[j0 + j];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc += t[j];
+  }
+}
+
 __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
   __shared__ double stages[kMomWaves][128];
   __shared__ float levels[ABEA_NUM_KMER];
@@ -715,9 +730,7 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
       st[lane] = i < n_e ? (double)cur.x : 0.0;
       st[64 + lane] = i < nk ? (double)levels[rank_lohi(cur.lo, cur.hi)] : 0.0;
       wave_lds_sync();
-      const int c = (lane & 1) ? 64 : 0;
-      const int cnt = min(64, n_max - base);
-      for (int j = 0; j < cnt; ++j) acc += st[c + j];
+      serial_sum64(st + ((lane & 1) ? 64 : 0), acc);
       wave_lds_sync();
     }
     if (!ex_ok) ev_sum = readlane_f64(acc, 0);
@@ -743,9 +756,7 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
     st[lane] = l2;
     st[64 + lane] = d2;
     wave_lds_sync();
-    const int c = (lane & 1) ? 64 : 0;
-    const int cnt = min(64, n_max - base);
-    for (int j = 0; j < cnt; ++j) acc += st[c + j];
+    serial_sum64(st + ((lane & 1) ? 64 : 0), acc);
     wave_lds_sync();
   }
   const double km_sq = readlane_f64(acc, 0), ev_sq = readlane_f64(acc, 1);
