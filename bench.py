@@ -33,11 +33,11 @@ W = 100
 # rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings
 # (profiles/abea_kernels_batched16384_ln8kb.md gives the runs), for the
 # fused-traceback launch shape the autotuner picks here.
-TRAFFIC_BYTES = {"abea_core_kernel<1, true>": 49067303628}
+TRAFFIC_BYTES = {"abea_core_kernel<1, true>": 49442908774}
 # VALU wave-instructions per launch of the same kernel on the same workload
 # (rocprofv3 SQ_INSTS_VALU, same file).  The core is bound by VALU issue: a
 # SIMD issues one wave64 VALU instruction per 4 cycles.
-VALU_INSTS = {"abea_core_kernel<1, true>": 2.088e10}
+VALU_INSTS = {"abea_core_kernel<1, true>": 2.086e10}
 N_SIMD = 1024          # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # peak engine clock
 
