@@ -268,6 +268,7 @@ struct GeRead {
   int32_t n;
   bool exact;            // window sums of the samples are exact in any order
   bool tame;             // exact, and every sample below 2^60 in magnitude
+  int64_t avail;         // samples readable from x on: this read's and the later reads'
 };
 
 // --------------------------------------------------------------------------
@@ -457,8 +458,12 @@ __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_
   int32_t cnt = 0;
   if (entry && from == keep_from) *entry = st;
   if (rd.exact) {
-    // an iteration loads samples up to i0 + 17 < to + 3 * 6
-    const bool check = to + 3 * kWLong > rd.n;
+    // An iteration loads samples up to i0 + 17 < to + 3 * 6.  Past the
+    // read's end those are the next read's samples: they enter only window
+    // sums whose statistics keep_if zeroes (samples above n - w), so only
+    // the end of the last read needs the checked loads, and a wave whose
+    // lanes reach a read's end runs one variant, not two.
+    const bool check = (int64_t)to + 3 * kWLong > rd.avail;
     if (rd.tame) {
       if (check)
         run_exact<true, true>(st, rd, from, to, keep_from, pk, cnt, entry);
@@ -494,7 +499,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void a
   const int32_t from = max(c0 - a.overlap, 0);
   GeState st = fresh_state();
   const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
-                  a.exact[r] == 2};
+                  a.exact[r] == 2, a.raw_off[a.n_reads] - off};
   run_detector(st, rd, from, c1, c0, a.peaks + c * kGeDetChunk, a.n_peaks + c, a.entry + c);
   a.exit[c] = st;
 }
@@ -523,7 +528,7 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
     const int64_t cb = a.det_off[r];
     GeState st = a.exit[cb + fb - 1];
     const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
-                    a.exact[r] == 2};
+                    a.exact[r] == 2, a.raw_off[a.n_reads] - off};
     for (int64_t c = cb + fb; c < a.det_off[r + 1]; ++c) {
       const int32_t c0 = (int32_t)(c - cb) * kGeDetChunk;
       if (same_state(a.entry[c], st, c0)) {
