@@ -619,20 +619,15 @@ __device__ __forceinline__ abea_event_t make_event_exact(const GeArgs &a, const 
   if (start <= end && end - start <= 2 * kGeChunk) {
     // four samples per step into four partial sums (exact in any order),
     // their loads in flight together
+    // (the last step's missing samples count as +0.0, which changes no sum)
     double s[4] = {0.0, 0.0, 0.0, 0.0}, q[4] = {0.0, 0.0, 0.0, 0.0};
-    int64_t k = start;
-    for (; k + 4 <= end; k += 4) {
+    for (int64_t k = start; k < end; k += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const float v = x[k + u];
+        const float v = k + u < end ? x[k + u] : 0.0f;
         s[u] += (double)v;
         q[u] += (double)(v * v);
       }
-    }
-    for (; k < end; ++k) {
-      const float v = x[k];
-      s[0] += (double)v;
-      q[0] += (double)(v * v);
     }
     ds = (s[0] + s[1]) + (s[2] + s[3]);
     dq = (q[0] + q[1]) + (q[2] + q[3]);
