@@ -75,6 +75,7 @@ class PostalignOutputT(C.Structure):
 
 FAILED_CALIBRATION = 0x001
 FAILED_ALIGNMENT = 0x002
+FAILED_QUALITY_CHK = 0x004
 
 
 class TuningT(C.Structure):
@@ -94,7 +95,8 @@ EXPORTS = [
     "abea_postalign", "abea_recalibrate_model", "abea_estimate_scalings_db", "abea_postalign_db",
     "abea_kmer_offsets", "abea_plan_postalign", "abea_plan_estimate_scalings",
     "abea_getevents", "abea_getevents_db", "abea_getevents_device",
-    "abea_ctx_set_getevents_overlap", "abea_ctx_getevents_reruns",
+    "abea_ctx_set_getevents_overlap", "abea_ctx_getevents_reruns", "abea_host_last_times",
+    "abea_host_contexts",
 ]
 
 _lib = None
@@ -185,6 +187,10 @@ def lib():
     L.abea_ctx_set_getevents_overlap.argtypes = [P, C.c_int32]
     L.abea_ctx_getevents_reruns.restype = C.c_int64
     L.abea_ctx_getevents_reruns.argtypes = [P]
+    L.abea_host_last_times.restype = None
+    L.abea_host_last_times.argtypes = [C.POINTER(C.c_double)]
+    L.abea_host_contexts.restype = C.c_int32
+    L.abea_host_contexts.argtypes = []
     _lib = L
     return L
 

@@ -13,9 +13,13 @@ The gradient is the one of the score along the path the forward traced
   scalings [n, 4] = (scale, shift, var, log_var): d/dscale, d/dshift; var
       and log_var get 0, since f5c's emission ignores them
       (GPU/align_cuda.cl:95, 515).
-  model [4096, 3] = (level_mean, level_stdv, level_log_stdv): d/dlevel_mean
-      and the total derivative d/dlevel_stdv with level_log_stdv taken as
-      log(level_stdv); the log column gets 0.
+  model [4096, 3] = (level_mean, level_stdv, level_log_stdv): the partial
+      derivatives of the function the forward computes, which reads the
+      cached level_log_stdv as an input of its own (GPU/align_cuda.cl:
+      517-518): d/dlevel_stdv holds the log column fixed, and the log column
+      gets d/dlevel_log_stdv (-1 per D or U cell of the k-mer, weighted).  A
+      caller that ties level_log_stdv = log(level_stdv) in torch gets the total
+      derivative from the chain rule.
 """
 from __future__ import annotations
 
@@ -47,9 +51,7 @@ class ABEAScore(torch.autograd.Function):
         gs[:, 0] = g_read[:, 1]  # scale
         gs[:, 1] = g_read[:, 0]  # shift
         gs[:, 2] = g_read[:, 2]  # var (0)
-        gm = torch.zeros((4096, 3), dtype=torch.float64, device=grad_score.device)
-        gm[:, :2] = g_model
-        return gs.to(ctx.dtypes[0]), gm.to(ctx.dtypes[1]), None
+        return gs.to(ctx.dtypes[0]), g_model.to(ctx.dtypes[1]), None
 
 
 def abea_score(plan: Plan, scalings: torch.Tensor, model: torch.Tensor) -> torch.Tensor:

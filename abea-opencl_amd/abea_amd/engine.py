@@ -197,11 +197,14 @@ class Plan:
         self.generation += 1
 
     def adjoint(self, grad_score: torch.Tensor):
+        """Gradient of sum_r grad_score[r] * score[r] along the paths of the
+        last forward: ([n, 3] f32 d/d(shift, scale, var), [4096, 3] f64
+        partial derivatives in (level_mean, level_stdv, level_log_stdv))."""
         n = self.batch.n_reads
         dev = grad_score.device
         gs = grad_score.contiguous().to(torch.float32)
         g_scal = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
-        g_model = torch.empty((4096, 2), dtype=torch.float64, device=dev)
+        g_model = torch.empty((4096, 3), dtype=torch.float64, device=dev)
         self.ctx.enter()
         _lib.check(_lib.lib().abea_plan_adjoint(self.handle, C.byref(self.out), _ptr(gs),
                                                 _ptr(g_scal), _ptr(g_model)), "abea_plan_adjoint")

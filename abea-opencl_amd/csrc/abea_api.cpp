@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -117,6 +119,8 @@ struct abea_plan {
   int64_t n_bands_total = 0, seg_bytes = 0, tail_pad = 0;
   int32_t n_prio = 0;  // leading queue entries run at raised wave priority
   DevBuf meta, queue, counter, km, km_rank, ev, ends, trace, moves, kck, replay, seg;
+  // adjoint: fixed-point k-mer gradient bins (allocated by the first adjoint)
+  DevBuf gacc, gspecial;
   // postalign scratch, allocated by the first postalign call: per-k-mer
   // recalibration terms, per-read walk status (zeroed once, then kept zero by
   // the finish kernel) and the walk's work items
@@ -654,7 +658,7 @@ int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
   if (!p) return 0;
   return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->km_rank.n + p->ev.n +
                    p->ends.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n +
-                   p->terms.n + p->pstat.n + p->items.n);
+                   p->terms.n + p->pstat.n + p->items.n + p->gacc.n + p->gspecial.n);
 }
 
 int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }
@@ -695,9 +699,12 @@ abea_status_t abea_plan_adjoint(abea_plan_t *p, const abea_output_t *fwd, const 
   if (st) return st;
   hipStream_t s = p->ctx->stream;
   const KernelArgs a = make_args(p, fwd);
+  if ((st = p->gacc.ensure(sizeof(unsigned long long) * 6 * ABEA_NUM_KMER)) ||
+      (st = p->gspecial.ensure(sizeof(double) * 3 * ABEA_NUM_KMER)))
+    return st;
   HIP_TRY(hipEventRecord(p->evt[4], s));
-  HIP_TRY(abea::launch_zero_f64(grad_model, 2 * ABEA_NUM_KMER, s));
-  HIP_TRY(abea::launch_adjoint(a, grad_score, grad_scalings, grad_model, s));
+  HIP_TRY(abea::launch_adjoint(a, grad_score, grad_scalings, grad_model,
+                               p->gacc.as<unsigned long long>(), p->gspecial.as<double>(), s));
   HIP_TRY(hipEventRecord(p->evt[3], s));
   HIP_TRY(hipEventSynchronize(p->evt[3]));
   float ms = 0;
@@ -958,30 +965,74 @@ struct HostRun {
   DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms, pstat, items;
   // getevents entries
   DevBuf ge_raw, ge_ev, ge_nev;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // drop-in phase timing
   ~HostRun() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
     delete plan;
     abea_ctx_destroy(ctx);
   }
 };
 
-thread_local HostRun *t_run = nullptr;
+// Host-buffer entry points share a pool of contexts (stream, plan, device
+// buffers): a call takes one for its device and returns it when it ends, so
+// the contexts held are bounded by the peak number of concurrent calls, not
+// by the number of threads that ever called (f5c starts fresh worker threads
+// for every batch).  The pool lives until the process ends.
+std::mutex g_pool_mu;
+std::vector<HostRun *> *g_pool = new std::vector<HostRun *>();
+int32_t g_pool_created = 0;
 
-abea_status_t host_run_get(HostRun **out) {
-  if (!t_run) {
-    HostRun *r = new HostRun();
+struct HostLease {
+  HostRun *r = nullptr;
+  HostLease() = default;
+  HostLease(const HostLease &) = delete;
+  HostLease &operator=(const HostLease &) = delete;
+  ~HostLease() {
+    if (!r) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool->push_back(r);
+  }
+  abea_status_t acquire() {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    abea_status_t st = abea_ctx_create(&r->ctx, dev, nullptr);
+    {
+      std::lock_guard<std::mutex> lk(g_pool_mu);
+      for (size_t i = g_pool->size(); i-- > 0;)
+        if ((*g_pool)[i]->ctx->device == dev) {
+          r = (*g_pool)[i];
+          g_pool->erase(g_pool->begin() + (std::ptrdiff_t)i);
+          return ABEA_OK;
+        }
+    }
+    HostRun *h = new HostRun();
+    abea_status_t st = abea_ctx_create(&h->ctx, dev, nullptr);
     if (st) {
-      delete r;
+      delete h;
       return st;
     }
-    r->plan = new abea_plan();
-    r->plan->ctx = r->ctx;
-    t_run = r;
+    h->plan = new abea_plan();
+    h->plan->ctx = h->ctx;
+    for (auto &e : h->ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        delete h;
+        return fail(ABEA_ERR_HIP, "hipEventCreate failed");
+      }
+    r = h;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    ++g_pool_created;
+    return ABEA_OK;
   }
-  *out = t_run;
-  return ABEA_OK;
+};
+
+// Phases of the calling thread's last abea_align / abea_align_db call (ms):
+// flatten, host-to-device copies of the inputs, planning (per-read metadata
+// and its upload), kernels, device-to-host copies, unflatten, whole call.
+// The copies, planning and kernels are hipEvent-timed on the call's stream.
+thread_local double t_host_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+
+double ms_since(const std::chrono::steady_clock::time_point &t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 template <class T>
@@ -1000,11 +1051,13 @@ abea_status_t run_host_batch(int form, int32_t n, const std::vector<char> &seq,
                              const abea_scalings_t *scal, const abea_model_t *model,
                              std::vector<abea_aligned_pair_t> &pairs_h,
                              std::vector<int32_t> &n_pairs_h, std::vector<int64_t> &pair_off) {
-  HostRun *r = nullptr;
-  abea_status_t st = host_run_get(&r);
+  HostLease lease;
+  abea_status_t st = lease.acquire();
   if (st) return st;
+  HostRun *r = lease.r;
   HIP_TRY(hipSetDevice(r->ctx->device));
   hipStream_t s = r->ctx->stream;
+  HIP_TRY(hipEventRecord(r->ev[0], s));
   if ((st = upload(r->seq, seq.data(), seq.size(), s))) return st;
   if ((st = upload(r->seq_ptr, seq_ptr.data(), seq_ptr.size(), s))) return st;
   if ((st = upload(r->read_len, read_len.data(), read_len.size(), s))) return st;
@@ -1013,6 +1066,7 @@ abea_status_t run_host_batch(int form, int32_t n, const std::vector<char> &seq,
   if ((st = upload(r->n_events, n_events.data(), n_events.size(), s))) return st;
   if ((st = upload(r->scal, scal, (size_t)n, s))) return st;
   if ((st = upload(r->model, model, (size_t)ABEA_NUM_KMER, s))) return st;
+  HIP_TRY(hipEventRecord(r->ev[1], s));
   abea_batch_t b{};
   b.n_reads = n;
   b.seq = r->seq.as<char>();
@@ -1034,7 +1088,9 @@ abea_status_t run_host_batch(int form, int32_t n, const std::vector<char> &seq,
   abea_output_t out{};
   out.pairs = r->pairs.as<abea_aligned_pair_t>();
   out.n_pairs = r->n_pairs.as<int32_t>();
+  HIP_TRY(hipEventRecord(r->ev[2], s));
   if ((st = abea_plan_forward(r->plan, &out))) return st;
+  HIP_TRY(hipEventRecord(r->ev[3], s));
   pairs_h.resize((size_t)cap);
   n_pairs_h.resize(n);
   pair_off.resize(n);
@@ -1043,7 +1099,13 @@ abea_status_t run_host_batch(int form, int32_t n, const std::vector<char> &seq,
     HIP_TRY(hipMemcpyAsync(pairs_h.data(), out.pairs, sizeof(abea_aligned_pair_t) * cap,
                            hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(n_pairs_h.data(), out.n_pairs, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipEventRecord(r->ev[4], s));
   HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  for (int i = 0; i < 4; ++i) {
+    HIP_TRY(hipEventElapsedTime(&ms, r->ev[i], r->ev[i + 1]));
+    t_host_ms[1 + i] = ms;
+  }
   return ABEA_OK;
 }
 
@@ -1059,6 +1121,7 @@ int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_
     fail(ABEA_ERR_INVALID, "abea_align: null argument");
     return -1;
   }
+  const auto t0 = std::chrono::steady_clock::now();
   const int32_t n_e = (int32_t)events.n;
   std::vector<char> seq(sequence, sequence + std::max(sequence_len, 0));
   std::vector<int64_t> seq_ptr{0}, ev_ptr{0};
@@ -1067,12 +1130,16 @@ int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_
   std::vector<abea_aligned_pair_t> pairs;
   std::vector<int32_t> n_pairs;
   std::vector<int64_t> off;
+  t_host_ms[0] = ms_since(t0);
   if (run_host_batch(ABEA_PLAN_PLAIN, 1, seq, seq_ptr, read_len, ev, ev_ptr, n_events, &scaling,
                      models, pairs, n_pairs, off))
     return -1;
+  const auto t4 = std::chrono::steady_clock::now();
   int32_t np = n_pairs[0];
-  if ((int64_t)np > 2 * (int64_t)n_e) return 0;  // would overrun f5c's 2*n_events buffer
+  if ((int64_t)np > 2 * (int64_t)n_e) np = 0;  // would overrun f5c's 2*n_events buffer
   std::memcpy(out_2, pairs.data(), sizeof(abea_aligned_pair_t) * np);
+  t_host_ms[5] = ms_since(t4);
+  t_host_ms[6] = ms_since(t0);
   return np;
 }
 
@@ -1084,6 +1151,7 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
   if (n_bam_rec < 0 || (n_bam_rec > 0 && (!read || !read_len || !et || !model || !scalings ||
                                           !event_align_pairs || !n_event_align_pairs)))
     return fail(ABEA_ERR_INVALID, "abea_align_db: null argument");
+  const auto t0 = std::chrono::steady_clock::now();
   // flatten, as GPU/align_cuda.c:432-492 does (no NUL terminators needed)
   std::vector<int64_t> seq_ptr(n_bam_rec), ev_ptr(n_bam_rec);
   std::vector<int32_t> rl(read_len, read_len + n_bam_rec), ne(n_bam_rec);
@@ -1104,19 +1172,41 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
   std::vector<abea_aligned_pair_t> pairs;
   std::vector<int32_t> n_pairs;
   std::vector<int64_t> off;
+  t_host_ms[0] = ms_since(t0);
   abea_status_t st = run_host_batch(ABEA_PLAN_BATCHED, n_bam_rec, seq, seq_ptr, rl, ev, ev_ptr, ne,
                                     scalings, model, pairs, n_pairs, off);
   if (st) return st;
+  const auto t4 = std::chrono::steady_clock::now();
   // unflatten into callee-malloc'd per-read arrays, GPU/align_cuda.c:926-931
   for (int32_t i = 0; i < n_bam_rec; ++i) {
     const int32_t np = n_pairs[i];
-    n_event_align_pairs[i] = np;
     event_align_pairs[i] =
         (abea_aligned_pair_t *)malloc(sizeof(abea_aligned_pair_t) * (size_t)std::max(np, 1));
-    if (!event_align_pairs[i]) return fail(ABEA_ERR_NOMEM, "abea_align_db: malloc failed");
+    if (!event_align_pairs[i]) {
+      // the caller owns nothing from a failed call
+      for (int32_t j = 0; j < i; ++j) free(event_align_pairs[j]);
+      for (int32_t j = 0; j < n_bam_rec; ++j) {
+        event_align_pairs[j] = nullptr;
+        n_event_align_pairs[j] = 0;
+      }
+      return fail(ABEA_ERR_NOMEM, "abea_align_db: malloc failed");
+    }
+    n_event_align_pairs[i] = np;
     if (np) std::memcpy(event_align_pairs[i], pairs.data() + off[i], sizeof(abea_aligned_pair_t) * np);
   }
+  t_host_ms[5] = ms_since(t4);
+  t_host_ms[6] = ms_since(t0);
   return ABEA_OK;
+}
+
+void abea_host_last_times(double times_ms[7]) {
+  if (!times_ms) return;
+  for (int i = 0; i < 7; ++i) times_ms[i] = t_host_ms[i];
+}
+
+int32_t abea_host_contexts(void) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  return g_pool_created;
 }
 
 }  // extern "C"
@@ -1186,9 +1276,10 @@ abea_status_t download(T *h, const DevBuf &d, size_t count, hipStream_t s) {
 abea_status_t run_mom(int32_t n, char *const *read, const int32_t *read_len,
                       const abea_event_table *et, const abea_model_t *model,
                       abea_scalings_t *scal_h) {
-  HostRun *r = nullptr;
-  abea_status_t st = host_run_get(&r);
+  HostLease lease;
+  abea_status_t st = lease.acquire();
   if (st) return st;
+  HostRun *r = lease.r;
   HIP_TRY(hipSetDevice(r->ctx->device));
   hipStream_t s = r->ctx->stream;
   HostPost h;
@@ -1324,9 +1415,10 @@ abea_status_t abea_getevents_db(int32_t n_reads, const size_t *nsample, float *c
                   nsample[i]);
   }
   if (n_reads == 0) return ABEA_OK;
-  HostRun *r = nullptr;
-  abea_status_t st = host_run_get(&r);
+  HostLease lease;
+  abea_status_t st = lease.acquire();
   if (st) return st;
+  HostRun *r = lease.r;
   HIP_TRY(hipSetDevice(r->ctx->device));
   hipStream_t s = r->ctx->stream;
   std::vector<int64_t> off((size_t)n_reads + 1, 0);
@@ -1396,8 +1488,9 @@ int32_t abea_postalign(abea_event_alignment_t *alignment, abea_index_pair_t *bas
          n_kmers, n_events);
     return -1;
   }
-  HostRun *r = nullptr;
-  if (host_run_get(&r)) return -1;
+  HostLease lease;
+  if (lease.acquire()) return -1;
+  HostRun *r = lease.r;
   if (hipSetDevice(r->ctx->device) != hipSuccess) {
     fail(ABEA_ERR_HIP, "postalign: hipSetDevice failed");
     return -1;
@@ -1433,8 +1526,9 @@ bool abea_recalibrate_model(abea_model_t *pore_model, abea_event_table et,
     fail(ABEA_ERR_INVALID, "recalibrate_model: null argument");
     return false;
   }
-  HostRun *r = nullptr;
-  if (host_run_get(&r)) return false;
+  HostLease lease;
+  if (lease.acquire()) return false;
+  HostRun *r = lease.r;
   hipStream_t s = r->ctx->stream;
   auto go = [&]() -> abea_status_t {
     HIP_TRY(hipSetDevice(r->ctx->device));
@@ -1490,9 +1584,10 @@ abea_status_t abea_postalign_db(int32_t n_bam_rec, char **read, const int32_t *r
   for (int32_t i = 0; i < n_bam_rec; ++i)
     if (n_event_align_pairs[i] > 0 && !event_align_pairs[i])
       return fail(ABEA_ERR_INVALID, "postalign_db: read %d has pairs but a null array", i);
-  HostRun *r = nullptr;
-  abea_status_t st = host_run_get(&r);
+  HostLease lease;
+  abea_status_t st = lease.acquire();
   if (st) return st;
+  HostRun *r = lease.r;
   HIP_TRY(hipSetDevice(r->ctx->device));
   HostPost h;
   host_post_layout(h, n_bam_rec, read, read_len, et, n_event_align_pairs);
@@ -1527,8 +1622,20 @@ abea_status_t abea_postalign_db(int32_t n_bam_rec, char **read, const int32_t *r
                                                           (size_t)n_event_align_pairs[i]);
     base_to_event_map[i] =
         (abea_index_pair_t *)malloc(sizeof(abea_index_pair_t) * (size_t)std::max(m.n_k, 1));
-    if (!event_alignment[i] || !base_to_event_map[i])
+    if (!event_alignment[i] || !base_to_event_map[i]) {
+      // the caller owns nothing from a failed call
+      for (int32_t j = 0; j <= i; ++j) {
+        free(event_alignment[j]);
+        free(base_to_event_map[j]);
+      }
+      for (int32_t j = 0; j < n_bam_rec; ++j) {
+        event_alignment[j] = nullptr;
+        base_to_event_map[j] = nullptr;
+        n_event_alignment[j] = 0;
+        events_per_base[j] = 0.0;
+      }
       return fail(ABEA_ERR_NOMEM, "postalign_db: malloc failed");
+    }
     std::memcpy(event_alignment[i], rec_h.data() + m.pair_off,
                 sizeof(abea_event_alignment_t) * (size_t)n_rec[i]);
     std::memcpy(base_to_event_map[i], bmap_h.data() + m.bmap_off,

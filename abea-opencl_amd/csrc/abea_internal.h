@@ -13,7 +13,7 @@ constexpr int kTileBands = 8;        // bands per trace tile
 constexpr int kTileWords = 64;       // one 32-bit word per lane per tile: 8 bands x 2 slots x 2 bits
 constexpr int kPad = 256;            // elements of slack after staged arrays (prefetch reads past a read)
 
-// Per-read metadata, one 80-byte record per read (read-order index).
+// Per-read metadata, one 96-byte record per read (read-order index).
 struct alignas(16) ReadMeta {
   int64_t trace_off;  // word offset of the read's 2-bit trace tiles (ceil(n_bands/8) * 64 words)
   int64_t km_off;     // offset (in k-mer parameter records) of the read's k-mers
@@ -204,9 +204,11 @@ hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s);
 hipError_t launch_core(const KernelArgs &a, int q_begin, int q_count, bool persistent,
                        int waves_per_block, int grid_blocks, int fuse, int overlap, hipStream_t s);
 hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overlap, hipStream_t s);
+// grad_model: [4096][3] doubles (level_mean, level_stdv, level_log_stdv);
+// acc (4096 * 6 limbs) and special (4096 * 3) are the fixed-point bins.
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
-                          double *grad_model, hipStream_t s);
-hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t s);
+                          double *grad_model, unsigned long long *acc, double *special,
+                          hipStream_t s);
 hipError_t launch_refine(const KernelArgs &a, abea_scalings_t *scal_out, float damping,
                          hipStream_t s);
 int core_max_waves_per_block();

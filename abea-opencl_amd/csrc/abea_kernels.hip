@@ -1311,12 +1311,48 @@ __global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin
 }
 
 // ---------------------------------------------------------------------------
-// adjoint: d(sum_r w_r * score_r)/d(shift, scale, var, level_mean, level_stdv).
-// Only D and U cells of the path carry an emission; per-cell terms are
-// recomputed in double from the float inputs and reduced in double.
+// adjoint: d(sum_r w_r * score_r)/d(shift, scale, var) per read and
+// d/d(level_mean, level_stdv, level_log_stdv) per k-mer.  Only D and U cells
+// of the path carry an emission; per-cell terms are recomputed in double from
+// the float inputs.  With a = (x - (scale*mu + shift)) / sigma and
+// em = c - lambda - a^2/2 (lambda = the cached level_log_stdv the forward
+// reads, GPU/align_cuda.cl:517-518): d em/d mu = a*scale/sigma,
+// d em/d sigma = a^2/sigma (lambda held), d em/d lambda = -1.
+//
+// Per-read sums: each lane adds the cells of its own contiguous chunks in
+// path order, then a fixed butterfly, so the bits do not depend on timing.
+// Per-k-mer sums: a lane adds a run of cells on one k-mer position in double,
+// then adds the run's weighted term to the k-mer's bins as a fixed-point
+// number (two 64-bit integer limbs, hi * 2^-12 + lo * 2^-44) with integer
+// atomics.  Integer addition is associative, so the total does not depend on
+// the order the atomics land in: two runs on the same inputs give the same
+// bits.  Each term is quantised once, with an error below 2^-45; the low
+// limb takes up to 2^31 terms per k-mer without overflow.  Terms that
+// are not finite, or of magnitude 2^40 or more (never for pA-scale inputs),
+// go to a double accumulator instead; a sum of non-finite values does not
+// depend on its order either.
+constexpr int kAdjChunk = 8;               // path cells per lane and step
+constexpr double kFixHi = 4096.0;          // 2^12
+constexpr double kFixLo = 17592186044416.0; // 2^44: rest = lo * 2^-44, lo in [0, 2^32]
+
+__device__ __forceinline__ void fix_add(unsigned long long *acc, double *special, double t) {
+  if (t == 0.0) return;
+  if (!(fabs(t) < 1099511627776.0)) {  // non-finite or |t| >= 2^40
+    atomicAdd(special, t);
+    return;
+  }
+  const double f = floor(t * kFixHi);               // exact scaling, exact floor
+  const double rest = t - f * (1.0 / kFixHi);       // in [0, 2^-12)
+  const long long hi = (long long)f;
+  const long long lo = (long long)rint(rest * kFixLo);  // in [0, 2^32]
+  atomicAdd(&acc[0], (unsigned long long)hi);
+  atomicAdd(&acc[1], (unsigned long long)lo);
+}
+
 __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const float *grad_score,
-                                                          float *grad_scal, double *grad_model,
-                                                          int n_reads) {
+                                                          float *grad_scal,
+                                                          unsigned long long *acc,
+                                                          double *special, int n_reads) {
   const int lane = threadIdx.x & 63;
   const int wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int n_waves = (gridDim.x * blockDim.x) >> 6;
@@ -1331,20 +1367,44 @@ __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const f
     const float *evp = a.ev + m.sev_off;
     const uint16_t *kr = a.km_rank + m.km_off;
     double gs = 0.0, gc = 0.0;
-    for (int i = lane; i < P; i += 64) {
-      if (from[i] == FROM_L) continue;
-      const abea_aligned_pair_t pr = pairs[i];
-      const int kk = min(max(pr.ref_pos, 0), m.n_k - 1);
-      const int ee = min(max(pr.read_pos, 0), m.n_e - 1);
-      const int rank = kr[kk] & (ABEA_NUM_KMER - 1);
-      const abea_model_t md = a.model[rank];
-      const double mu = (double)md.level_mean, sg = (double)md.level_stdv;
-      const double inv = 1.0 / sg;
-      const double z = ((double)evp[ee] - (scale * mu + shift)) * inv;
-      gs += z * inv;
-      gc += z * mu * inv;
-      atomicAdd(&grad_model[2 * rank], w * (z * scale * inv));
-      atomicAdd(&grad_model[2 * rank + 1], w * ((z * z - 1.0) * inv));
+    for (int base = 0; base < P; base += 64 * kAdjChunk) {
+      int cur_k = -1, cur_rank = 0;
+      double s_mu = 0.0, s_sg = 0.0, s_n = 0.0;
+      for (int j = 0; j < kAdjChunk; ++j) {
+        const int i = base + lane * kAdjChunk + j;
+        if (i >= P || from[i] == FROM_L) continue;
+        const abea_aligned_pair_t pr = pairs[i];
+        const int kk = min(max(pr.ref_pos, 0), m.n_k - 1);
+        const int ee = min(max(pr.read_pos, 0), m.n_e - 1);
+        if (kk != cur_k) {
+          if (cur_k >= 0) {
+            unsigned long long *b = acc + 6 * cur_rank;
+            double *sp = special + 3 * cur_rank;
+            fix_add(b + 0, sp + 0, w * s_mu);
+            fix_add(b + 2, sp + 1, w * s_sg);
+            fix_add(b + 4, sp + 2, -w * s_n);
+          }
+          cur_k = kk;
+          cur_rank = kr[kk] & (ABEA_NUM_KMER - 1);
+          s_mu = s_sg = s_n = 0.0;
+        }
+        const abea_model_t md = a.model[cur_rank];
+        const double mu = (double)md.level_mean, sg = (double)md.level_stdv;
+        const double inv = 1.0 / sg;
+        const double z = ((double)evp[ee] - (scale * mu + shift)) * inv;
+        gs += z * inv;
+        gc += z * mu * inv;
+        s_mu += z * scale * inv;
+        s_sg += z * z * inv;
+        s_n += 1.0;
+      }
+      if (cur_k >= 0) {
+        unsigned long long *b = acc + 6 * cur_rank;
+        double *sp = special + 3 * cur_rank;
+        fix_add(b + 0, sp + 0, w * s_mu);
+        fix_add(b + 2, sp + 1, w * s_sg);
+        fix_add(b + 4, sp + 2, -w * s_n);
+      }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -1356,6 +1416,29 @@ __global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const f
       grad_scal[3 * r + 1] = (float)(w * gc);
       grad_scal[3 * r + 2] = 0.0f;
     }
+  }
+}
+
+// Clears the fixed-point bins before an adjoint (a fill kernel rather than a
+// memset, so the sequence can be captured in a graph).
+__global__ void abea_adjoint_clear_kernel(unsigned long long *acc, double *special) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * ABEA_NUM_KMER;
+       i += gridDim.x * blockDim.x) {
+    acc[2 * i] = 0ull;
+    acc[2 * i + 1] = 0ull;
+    special[i] = 0.0;
+  }
+}
+
+// grad_model[3 * rank + column] = hi * 2^-12 + lo * 2^-44 (+ the non-finite
+// or huge terms, which then decide the value).
+__global__ void abea_adjoint_finish_kernel(const unsigned long long *acc, const double *special,
+                                           double *grad_model) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * ABEA_NUM_KMER;
+       i += gridDim.x * blockDim.x) {
+    const double hi = (double)(long long)acc[2 * i];
+    const double lo = (double)(long long)acc[2 * i + 1];
+    grad_model[i] = (hi * (1.0 / kFixHi) + lo * (1.0 / kFixLo)) + special[i];
   }
 }
 
@@ -1418,12 +1501,6 @@ __global__ __launch_bounds__(256) void abea_refine_kernel(KernelArgs a, abea_sca
   }
 }
 
-__global__ void abea_zero_f64_kernel(double *p, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0.0;
-}
-
 // ---------------------------------------------------------------------------
 hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(abea_queue_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
@@ -1468,12 +1545,17 @@ hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overla
 }
 
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
-                          double *grad_model, hipStream_t s) {
+                          double *grad_model, unsigned long long *acc, double *special,
+                          hipStream_t s) {
   const int n = a.n_queue;
-  if (n <= 0) return hipSuccess;
-  const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
-  hipLaunchKernelGGL(abea_adjoint_kernel, dim3(grid), dim3(256), 0, s, a, grad_score,
-                     grad_scalings, grad_model, n);
+  hipLaunchKernelGGL(abea_adjoint_clear_kernel, dim3(48), dim3(256), 0, s, acc, special);
+  if (n > 0) {
+    const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
+    hipLaunchKernelGGL(abea_adjoint_kernel, dim3(grid), dim3(256), 0, s, a, grad_score,
+                       grad_scalings, acc, special, n);
+  }
+  hipLaunchKernelGGL(abea_adjoint_finish_kernel, dim3(48), dim3(256), 0, s, acc, special,
+                     grad_model);
   return hipGetLastError();
 }
 
@@ -1483,14 +1565,6 @@ hipError_t launch_refine(const KernelArgs &a, abea_scalings_t *scal_out, float d
   if (n <= 0) return hipSuccess;
   const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
   hipLaunchKernelGGL(abea_refine_kernel, dim3(grid), dim3(256), 0, s, a, scal_out, damping, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int64_t blocks = (n + 255) / 256;
-  const int grid = (int)(blocks < 1024 ? blocks : 1024);
-  hipLaunchKernelGGL(abea_zero_f64_kernel, dim3(grid), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ namespace {
 
 constexpr int kMinMRecords = 200;     // minNumEventsToRescale of recalibrate_model
 constexpr float kMinCalibrationVar = ABEA_MIN_CALIBRATION_VAR;
+constexpr double kMaxEventsPerBase = ABEA_MAX_EVENTS_PER_BASE;
 constexpr int kWavesPerBlock = 4;
 
 __device__ __forceinline__ float event_mean(const abea_event_t *ev, const float *evm, int64_t i) {
@@ -518,9 +519,13 @@ __global__ __launch_bounds__(256) void abea_postalign_finish_kernel(PostArgs a) 
       flag = ABEA_FAILED_CALIBRATION;
     }
   }
+  const double epb = (double)(eN - e0) / nk;
+  // f5c's stay filter after a passed calibration: too many events per base
+  // fails the quality check (FAILED_QUALITY_CHK, GPU/f5c.h:52)
+  if (a.recalibrate && flag == 0 && epb > kMaxEventsPerBase) flag = ABEA_FAILED_QUALITY_CHK;
   if (lane == 0) {
     a.n_rec[r] = eN - e0 + 1;
-    if (a.epb) a.epb[r] = (double)(eN - e0) / nk;
+    if (a.epb) a.epb[r] = epb;
     a.scal_out[r] = out;
     a.flag[r] = flag;
   }

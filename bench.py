@@ -2,26 +2,37 @@
 
 Headline: batched persistent ABEA (length-sorted queue, persistent core that
 stages, walks and traces back each read, replayed as a hipGraph, autotuned
-launch shapes) over the
-16,384-read long-tailed workload (log-normal median 8 kb, clipped 500 b-200 kb,
-~150 Mbases, ~40 G nominal cells), inputs resident in HBM as f5c event_t
-records.  A "step" is one full forward over that batch.
+launch shapes) over the 16,384-read long-tailed workload (log-normal median
+8 kb, clipped 500 b-200 kb, ~150 Mbases, ~40 G nominal cells), inputs
+resident in HBM as f5c event_t records.  A "step" is one full forward over
+that batch; the core kernel's time comes from hipEvents recorded inside the
+same graph steps.
 
-Also reported (same JSON line): the plain per-read-launch form and the batched
-form on a sample of the 4,096-read log-normal (median 5 kb) workload; the
-adjoint/forward time ratio on 1,024 x 10 kb reads; the scaling-refinement loop
-on 8,192 x 20 kb reads (five graph-replayed forward + Gauss-Newton steps from
-perturbed scalings, with the recovered-scaling error); the CPU baseline (the
-oracle, OpenMP over reads on this host) on a bounded sample.
+Also reported (same JSON line):
+- the plain per-read-launch form against the batched form on all 4,096 reads
+  of the log-normal (median 5 kb) workload, and on the 64 x 2 kb reads;
+- the host-buffer drop-ins: align_cuda()'s batch entry on the 4,096 reads
+  with its phases (flatten, H2D, kernels, D2H, unflatten) and align() per
+  read, sequential and from 8 threads;
+- the adjoint/forward time ratio on 1,024 x 10 kb reads;
+- postalign/recalibrate/MOM and getevents on their workloads;
+- the scaling-refinement loop on 8,192 x 20 kb reads;
+- the CPU leg: the oracle (f5c semantics, OpenMP over reads) timed on a
+  bounded sample of the headline workload with all host threads and with
+  one, and on the 64 x 2 kb reads; the same sample (plus the 64 longest
+  reads) compared read by read with the GPU's headline results; the FP64
+  error of the float scores; fills; finite differences of the adjoint.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -29,15 +40,15 @@ sys.path.insert(0, os.path.join(ROOT, "abea-opencl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 W = 100
-# HBM bytes per launch of the core kernel on the headline workload, from
-# rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 1 GiB copy's readings
-# (profiles/abea_kernels_batched16384_ln8kb.md gives the runs), for the
-# fused-traceback launch shape the autotuner picks here.
-TRAFFIC_BYTES = {"abea_core_kernel<1, true>": 49442908774}
-# VALU wave-instructions per launch of the same kernel on the same workload
-# (rocprofv3 SQ_INSTS_VALU, same file).  The core is bound by VALU issue: a
-# SIMD issues one wave64 VALU instruction per 4 cycles.
-VALU_INSTS = {"abea_core_kernel<1, true>": 2.086e10}
+# Per launch of the core kernel on the headline workload, from rocprofv3
+# runs at the listed launch shape (profiles/abea_kernels_batched16384_ln8kb.md
+# gives the commands): HBM bytes (FETCH_SIZE + WRITE_SIZE passes, scaled by a
+# 1 GiB copy's readings) and VALU wave-instructions (SQ_INSTS_VALU).  Keyed by
+# (kernel, waves per CU, traceback overlap).
+PROFILED = {
+    ("abea_core_kernel<1, true>", 24, 128): {"traffic": 49442908774, "valu": 2.086e10},
+    ("abea_core_kernel<1, true>", 32, 128): {"traffic": 49442908774, "valu": 2.086e10},
+}
 N_SIMD = 1024          # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # peak engine clock
 
@@ -77,6 +88,303 @@ def timed_steps(plan, steps, torch):
     return (time.perf_counter() - t0) / steps
 
 
+def best_ms(fn, torch, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return 1e3 * min(ts)
+
+
+# --------------------------------------------------------------------- legs
+def headline_leg(args, ctx):
+    import numpy as np
+    import torch
+
+    from abea_amd import engine, synth
+    from abea_amd._lib import PLAN_BATCHED
+
+    t = time.perf_counter()
+    wl = "batched16384_ln8kb"
+    b = synth.workload(wl, seed=args.seed, with_aos=True)
+    cells = b.nominal_cells()
+    log(f"[bench] generated {wl}: {b.n_reads} reads, {int(b.read_len.sum()) / 1e6:.1f} Mb, "
+        f"{int(b.n_events.sum()) / 1e6:.1f} M events, {cells / 1e9:.2f} G cells "
+        f"in {time.perf_counter() - t:.1f}s")
+    db = engine.DeviceBatch.upload(b, 0, aos_events=True)
+    plan = engine.Plan(ctx, db, PLAN_BATCHED)
+    log(f"[bench] plan workspace {plan.workspace_bytes() / 1e9:.1f} GB")
+    plan.forward()
+    torch.cuda.synchronize()
+    if args.profile:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE
+        # readings scale the forward's counters to bytes
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
+    if args.tuning:
+        wpb, wpc, fuse, ovl = (int(x) for x in args.tuning.split(","))
+        plan.set_tuning(core_waves_per_block=wpb, core_waves_per_cu=wpc, fuse_traceback=fuse,
+                        traceback_overlap=ovl)
+    elif not args.profile:
+        t = time.perf_counter()
+        plan.autotune(reps=1)
+        log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s")
+    tune = plan.tuning()
+    log(f"[bench] tuning {tune}")
+
+    # timed steps: graph replays; each step's kernel times come from the
+    # hipEvents recorded inside that step's graph
+    plan.set_tuning(use_graph=0 if args.eager else 1)
+    for _ in range(args.warmup):
+        plan.forward()
+    torch.cuda.synchronize()
+    kt = np.zeros(3)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.forward()
+        kt += np.array(plan.times()[:3])
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    kt /= args.steps
+    gcups = cells / sec / 1e9
+    log(f"[bench] batched {'eager' if args.eager else 'graph'}: {sec * 1e3:.2f} ms/step, "
+        f"{gcups:.1f} GCUPS; kernel ms queue_reset/core/post {kt.round(3).tolist()}")
+    res = plan.results()
+    moves = plan.moves()
+    qc_pass = int((res["n_pairs"] > 0).sum())
+    walk_b, tb_b = alg_bytes(b, res["path_len"])
+    # the dominant kernel is the core; fused, it also does the traceback
+    core_bytes = walk_b + (tb_b if tune["fuse_traceback"] else 0)
+    achieved = core_bytes / (kt[1] * 1e-3) / 1e9
+    kname = kernel_name(tune)
+    prof = PROFILED.get((kname, tune["core_waves_per_cu"], tune["traceback_overlap"]))
+    out = {
+        "metric": "ABEA GCUPS (batched persistent form, 1 GPU)",
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 cells, f64 transition sums",
+        "data": "synthetic (seeded; f5c event_t records in HBM)",
+        "config": {"workload": "batched16384_ln8kb", "reads": b.n_reads,
+                   "bases": int(b.read_len.sum()), "events": int(b.n_events.sum()),
+                   "nominal_cells": cells, "bandwidth": W, "tuning": tune},
+        "reads_per_s": round(b.n_reads / sec, 1),
+        "kernel_ms": {"queue_reset": round(kt[0], 3), "core": round(kt[1], 3),
+                      "post": round(kt[2], 3), "source": "hipEvents inside the timed steps"},
+        "qc_pass_reads": qc_pass,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": prof["traffic"] if prof else None,
+                     "alg_bytes_per_launch": core_bytes},
+    }
+    if prof:
+        # VALU issue rate against the ceiling of one wave instruction per SIMD
+        # per 4 cycles at the peak clock
+        rate = prof["valu"] / (kt[1] * 1e-3)
+        peak = N_SIMD * CLOCK_HZ / 4
+        out["issue"] = {"bound": "valu", "valu_insts_per_launch": prof["valu"],
+                        "valu_insts_per_band": round(prof["valu"] / int(b.n_bands.sum()), 2),
+                        "achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
+                        "unit": "G wave-instructions/s", "frac": round(rate / peak, 4)}
+    else:
+        out["roofline"]["traffic_note"] = "no pmc profile at this launch shape"
+    keep = {"pairs": res["pairs"], "path_from": res["path_from"], "path_len": res["path_len"],
+            "n_pairs": res["n_pairs"], "score": res["score"], "avg": res["avg"],
+            "pair_off": res["pair_off"], "moves": moves}
+    del plan, db, res
+    torch.cuda.empty_cache()
+    return out, b, keep, tune
+
+
+def plain_leg(args, ctx, tune):
+    """The reference's shape (one launch set per read) against the batched
+    form on the same reads: all 4,096 reads of the 5 kb log-normal workload,
+    and the 64 x 2 kb reads."""
+    import numpy as np  # noqa: F401
+    import torch
+
+    from abea_amd import engine, synth
+    from abea_amd._lib import PLAN_BATCHED, PLAN_PLAIN
+
+    res = {}
+    for wl, b1 in (("plain4096_ln5kb", synth.workload("plain4096_ln5kb", seed=args.seed + 1)),
+                   ("cpu64x2kb", synth.workload("cpu64x2kb", seed=args.seed + 2))):
+        d1 = engine.DeviceBatch.upload(b1, 0)
+        p_plain = engine.Plan(ctx, d1, PLAN_PLAIN)
+        p_plain.forward()
+        s_plain = timed_steps(p_plain, 1, torch)
+        p_b = engine.Plan(ctx, d1, PLAN_BATCHED)
+        p_b.set_tuning(**{k: v for k, v in tune.items()})
+        p_b.forward()
+        s_b = timed_steps(p_b, 5, torch)
+        cells = b1.nominal_cells()
+        res[wl] = {"reads": b1.n_reads, "nominal_cells": cells,
+                   "plain_gcups": round(cells / s_plain / 1e9, 3),
+                   "plain_reads_per_s": round(b1.n_reads / s_plain, 1),
+                   "batched_gcups": round(cells / s_b / 1e9, 3),
+                   "batched_reads_per_s": round(b1.n_reads / s_b, 1),
+                   "batched_over_plain": round(s_plain / s_b, 1)}
+        log(f"[bench] plain/batched {wl}: {res[wl]}")
+        del p_plain, p_b, d1
+    return res
+
+
+def dropin_leg(args):
+    """Host buffers in and out, PCIe included: align_cuda()'s batch entry
+    (abea_align_db) on the 4,096-read workload, with the phases the
+    reference's host times, and align() (abea_align) per read."""
+    import numpy as np
+
+    from abea_amd import _lib, synth
+
+    L = _lib.lib()
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free.restype = None
+    b = synth.workload("plain4096_ln5kb", seed=args.seed + 1, with_aos=True)
+    n = b.n_reads
+    aos = b.events_aos()
+    model = np.ascontiguousarray(b.model, np.float32)
+    reads, ets = (C.c_char_p * n)(), (_lib.EventTable * n)()
+    seqs = [b.read_seq(i) for i in range(n)]
+    base = aos.ctypes.data
+    for i in range(n):
+        reads[i] = seqs[i]
+        ets[i] = _lib.EventTable(int(b.n_events[i]), 0, int(b.n_events[i]),
+                                 C.cast(base + 24 * int(b.ev_ptr[i]), C.POINTER(_lib.EventT)))
+    rl = np.ascontiguousarray(b.read_len, np.int32)
+    scal = np.ascontiguousarray(b.scalings(), np.float32)
+    outp = (C.POINTER(_lib.AlignedPair) * n)()
+    nout = np.zeros(n, np.int32)
+    phases = []
+    walls = []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        st = L.abea_align_db(n, C.cast(reads, C.c_void_p), rl.ctypes.data_as(C.c_void_p),
+                             C.cast(ets, C.c_void_p), model.ctypes.data_as(C.c_void_p),
+                             scal.ctypes.data_as(C.c_void_p), C.cast(outp, C.c_void_p),
+                             nout.ctypes.data_as(C.c_void_p))
+        walls.append(time.perf_counter() - t0)
+        _lib.check(st, "abea_align_db")
+        t = (C.c_double * 7)()
+        L.abea_host_last_times(t)
+        phases.append(list(t))
+        for i in range(n):
+            libc.free(C.cast(outp[i], C.c_void_p))
+    k = int(np.argmin(walls[1:])) + 1  # the first call also sizes the pooled buffers
+    ph = phases[k]
+    # bytes of the input copies (bases, event_t records, scalings, model,
+    # CSR offsets and counts) and of the output copies (every pair slot and
+    # the counts)
+    h2d_bytes = (int(b.read_len.sum()) + 24 * int(b.n_events.sum()) + 16 * n + 12 * 4096
+                 + 24 * n)
+    cap = int((b.n_events.astype(np.int64) + b.n_kmers).sum())
+    d2h_bytes = 8 * cap + 4 * n
+    cells = b.nominal_cells()
+    res = {"align_db": {"workload": "plain4096_ln5kb", "reads": n,
+                        "gcups": round(cells / walls[k] / 1e9, 3),
+                        "reads_per_s": round(n / walls[k], 1),
+                        "ms": {"flatten": round(ph[0], 3), "h2d": round(ph[1], 3),
+                               "plan": round(ph[2], 3), "kernels": round(ph[3], 3),
+                               "d2h": round(ph[4], 3), "unflatten": round(ph[5], 3),
+                               "total": round(ph[6], 3)},
+                        "h2d_gbs": round(h2d_bytes / (ph[1] * 1e-3) / 1e9, 2),
+                        "d2h_gbs": round(d2h_bytes / (ph[4] * 1e-3) / 1e9, 2)}}
+    # align() per read: the first 64 reads, one thread, then 8 threads at once
+    m = 64
+    outs = [np.zeros((2 * int(b.n_events[i]) + 1, 2), np.int32) for i in range(m)]
+
+    def one(i):
+        ev = ets[i]
+        sc = _lib.ScalingsT(float(b.scale[i]), float(b.shift[i]), 0.0, 0.0)
+        return L.abea_align(outs[i].ctypes.data_as(C.c_void_p), seqs[i], len(seqs[i]), ev,
+                            model.ctypes.data_as(C.c_void_p), sc, 4000.0)
+
+    for i in range(8):
+        one(i)
+    t0 = time.perf_counter()
+    for i in range(m):
+        one(i)
+    seq_s = time.perf_counter() - t0
+
+    def worker(t):
+        for i in range(t, m, 8):
+            one(i)
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(8)]
+    t0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    thr_s = time.perf_counter() - t0
+    cells_m = int(W * b.n_bands[:m].sum())
+    res["align_per_read"] = {"reads": m, "one_thread_reads_per_s": round(m / seq_s, 1),
+                             "one_thread_gcups": round(cells_m / seq_s / 1e9, 3),
+                             "eight_threads_reads_per_s": round(m / thr_s, 1),
+                             "ms_per_read": round(1e3 * seq_s / m, 3),
+                             "host_contexts": int(L.abea_host_contexts())}
+    log(f"[bench] drop-ins: {res}")
+    return res
+
+
+def adjoint_leg(args, ctx, tune):
+    import numpy as np
+    import torch
+
+    from abea_amd import engine, synth
+    from abea_amd._lib import PLAN_BATCHED
+
+    b3 = synth.workload("adjoint1024x10kb", seed=args.seed + 3)
+    d3 = engine.DeviceBatch.upload(b3, 0)
+    p3 = engine.Plan(ctx, d3, PLAN_BATCHED)
+    p3.set_tuning(**{k: v for k, v in tune.items()})
+    p3.forward()
+    s_fwd = timed_steps(p3, 3, torch)
+    gsc = torch.ones(b3.n_reads, device="cuda")
+    p3.adjoint(gsc)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        p3.adjoint(gsc)
+    torch.cuda.synchronize()
+    s_adj = (time.perf_counter() - t) / 3
+    # two adjoints of the same forward give the same bits (fixed-point sums)
+    g1 = p3.adjoint(gsc)[1].cpu().numpy()
+    g2 = p3.adjoint(gsc)[1].cpu().numpy()
+    res = {"workload": "adjoint1024x10kb", "forward_ms": round(s_fwd * 1e3, 3),
+           "adjoint_ms": round(s_adj * 1e3, 3), "adjoint_over_forward": round(s_adj / s_fwd, 4),
+           "deterministic": bool(np.array_equal(g1, g2))}
+    log(f"[bench] adjoint: {res}")
+    del p3, d3
+    # eight of the reads for the finite-difference check of the CPU leg
+    sub = b3.subset(np.arange(8))
+    ps = engine.Plan(ctx, engine.DeviceBatch.upload(sub, 0), PLAN_BATCHED)
+    ps.forward()
+    w = torch.ones(sub.n_reads, device="cuda")
+    g_scal, g_model = ps.adjoint(w)
+    fd_in = {"batch": sub, "res": ps.results(), "moves": ps.moves(),
+             "g_scal": g_scal.cpu().numpy().astype(np.float64),
+             "g_model": g_model.cpu().numpy()}
+    ps.close()
+    return res, fd_in
+
+
 def postalign_leg(args, ctx, tune) -> dict:
     """f5c's steps around align() on 16,384 reads that pass align()'s QC (log-
     normal median 8 kb, the real-data k-mer skip rate): MOM scalings, forward,
@@ -95,22 +403,10 @@ def postalign_leg(args, ctx, tune) -> dict:
         p5.set_tuning(**{k: v for k, v in tune.items()})
     p5.forward()
     torch.cuda.synchronize()
-
-    def best_ms(fn, reps=3):
-        fn()
-        torch.cuda.synchronize()
-        ts = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            fn()
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t0)
-        return 1e3 * min(ts)
-
-    fwd_ms = best_ms(p5.forward)
-    mom_ms = best_ms(lambda: p5.estimate_scalings())
-    full_ms = best_ms(lambda: p5.postalign())
-    nrec_ms = best_ms(lambda: p5.postalign(records=False))
+    fwd_ms = best_ms(p5.forward, torch)
+    mom_ms = best_ms(lambda: p5.estimate_scalings(), torch)
+    full_ms = best_ms(lambda: p5.postalign(), torch)
+    nrec_ms = best_ms(lambda: p5.postalign(records=False), torch)
     pr = engine.postalign_results(p5, p5.postalign())
     n_pairs = int(p5.n_pairs[:b5.n_reads].sum().item())
     n_rec = int(pr["n_alignment"].sum())
@@ -120,7 +416,7 @@ def postalign_leg(args, ctx, tune) -> dict:
     # record (an upper bound: only 'M' records need one); model rows are
     # cache-resident
     post_bytes = 8 * n_pairs + 32 * n_rec + 8 * n_km + 4 * n_rec
-    flags = np.bincount(pr["read_stat_flag"], minlength=3)
+    flags = np.bincount(pr["read_stat_flag"], minlength=5)
     res = {
         "workload": "postalign16384_ln8kb", "reads": b5.n_reads, "pairs": n_pairs,
         "records": n_rec, "forward_ms": round(fwd_ms, 3),
@@ -131,24 +427,19 @@ def postalign_leg(args, ctx, tune) -> dict:
         "achieved_gbs": round(post_bytes / (full_ms * 1e-3) / 1e9, 1),
         "pairs_per_s": round(n_pairs / (full_ms * 1e-3), 1),
         "status": {"ok": int(flags[0]), "failed_calibration": int(flags[1]),
-                   "failed_alignment": int(flags[2])}}
+                   "failed_alignment": int(flags[2]), "failed_quality_chk": int(flags[4])}}
     log(f"[bench] postalign: {res}")
     del p5, d5, b5, pr
     return res
 
 
-def getevents_leg(args, ctx) -> dict:
+def getevents_leg(args, ctx):
     """f5c's step before MOM and align(): event detection on the raw signals
     (pA) of 1,024 reads of the 5 kb log-normal workload, resident in HBM.
-    The oracle's serial time on a bounded sample of the same reads is the CPU
-    baseline."""
-    import numpy as np
+    Returns the leg and the host signal for the CPU leg's timing."""
     import torch
 
     from abea_amd import _lib, engine, synth
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # CPU baseline only
 
     b = synth.workload("plain4096_ln5kb", seed=args.seed + 7, n=1024)
     raw_h, raw_off = synth.raw_signal(b, seed=args.seed + 7)
@@ -175,26 +466,262 @@ def getevents_leg(args, ctx) -> dict:
     # algorithmic bytes: the signal read once (4 B per sample), the event
     # records written (24 B each)
     alg = 4 * n_samples + 24 * n_ev
-    # CPU: the oracle, one thread, on the first reads up to ~2 s of work
+    res = {"workload": "getevents1024_ln5kb", "reads": b.n_reads, "samples": n_samples,
+           "events": n_ev, "ms": round(ms, 3),
+           "gsamples_per_s": round(n_samples / (ms * 1e-3) / 1e9, 3),
+           "alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "chunks_rerun": reruns}
+    log(f"[bench] getevents: {res}")
+    del raw, ev, cnt
+    return res, (raw_h, raw_off)
+
+
+def refine_leg(args, ctx) -> dict:
+    """Scaling refinement: 8,192 x 20 kb reads, scalings perturbed (shift
+    +-5, scale +-10%), five graph-replayed forward + Gauss-Newton steps."""
+    import numpy as np
+    import torch
+
+    from abea_amd import engine, synth
+    from abea_amd._lib import PLAN_BATCHED
+
+    b4 = synth.workload("refine8192x20kb", seed=args.seed + 4)
+    true_scale, true_shift = synth.perturb_scalings(b4, seed=args.seed + 4)
+    d4 = engine.DeviceBatch.upload(b4, 0)
+    p4 = engine.Plan(ctx, d4, PLAN_BATCHED)
+    p4.forward()
+    p4.autotune(reps=1)
+    p4.set_tuning(use_graph=1)
+    torch.cuda.synchronize()
+    s0 = d4.scalings.cpu().numpy()
+    steps = 5
+    t = time.perf_counter()
+    ms = p4.refine(steps, 1.0)
+    wall = (time.perf_counter() - t) / steps
+    s1 = d4.scalings.cpu().numpy()
+
+    def err(s, col, true):
+        d = np.abs(s[:, col] - true)
+        return {"mean": round(float(d.mean()), 5), "median": round(float(np.median(d)), 5),
+                "p90": round(float(np.quantile(d, 0.9)), 5)}
+
+    res = {"workload": "refine8192x20kb", "steps": steps, "tuning": p4.tuning(),
+           "ms_per_step": round(ms, 3), "wall_ms_per_step": round(wall * 1e3, 3),
+           "gcups_per_step": round(b4.nominal_cells() / (ms * 1e-3) / 1e9, 2),
+           "shift_abs_err_before": err(s0, 1, true_shift),
+           "shift_abs_err_after": err(s1, 1, true_shift),
+           "scale_abs_err_before": err(s0, 0, true_scale),
+           "scale_abs_err_after": err(s1, 0, true_scale)}
+    log(f"[bench] refine: {res}")
+    del p4, d4, b4
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(args, b, gpu, fd_in, ge_in) -> dict:
+    """The oracle (f5c semantics, test infrastructure) as the CPU baseline and
+    as the checker of the GPU's headline results."""
+    import numpy as np
+
+    from abea_amd import engine, synth
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    out = {}
+    rng = np.random.default_rng(args.seed)
+    order = rng.permutation(b.n_reads)
+    probe = b.subset(order[:32])
+    t = time.perf_counter()
+    oracle.align_batch(probe)
+    rate = probe.nominal_cells() / (time.perf_counter() - t)
+    cum = np.cumsum(b.n_bands[order] * W)
+    n_s = int(min(b.n_reads, max(32, np.searchsorted(cum, rate * args.cpu_seconds))))
+    longest = np.argsort(b.n_bands, kind="stable")[-64:]
+    idx = np.unique(np.concatenate([order[:n_s], longest]))
+    samp = b.subset(idx)
+    t = time.perf_counter()
+    ob = oracle.align_batch(samp, want_path=True)
+    dt = time.perf_counter() - t
+    cpu_gcups = samp.nominal_cells() / dt / 1e9
+    threads = ob["threads"]
+
+    # parity: every sampled read against the GPU's headline outputs
+    identical = 0
+    first_bad = None
+    fields_bad = {}
+    for j, i in enumerate(idx):
+        n = int(ob["path_len"][j])
+        o = int(ob["pair_ptr"][j])
+        g = int(gpu["pair_off"][i])
+        checks = {
+            "path_len": int(gpu["path_len"][i]) == n,
+            "n_pairs": int(gpu["n_pairs"][i]) == int(ob["n_pairs"][j]),
+            "pairs": np.array_equal(gpu["pairs"][g:g + n], ob["path"][o:o + n]),
+            "from": np.array_equal(gpu["path_from"][g:g + n], ob["from"][o:o + n]),
+            "score": gpu["score"][i] == np.float32(ob["score"][j]),
+            "avg": n == 0 or gpu["avg"][i] == ob["avg"][j]}
+        if all(checks.values()):
+            identical += 1
+        else:
+            for k, v in checks.items():
+                if not v:
+                    fields_bad[k] = fields_bad.get(k, 0) + 1
+            if first_bad is None:
+                first_bad = {"read": int(i), "fields": [k for k, v in checks.items() if not v]}
+    out["parity"] = {"workload": "batched16384_ln8kb", "compared": int(idx.shape[0]),
+                     "identical": identical, "first_mismatch": first_bad,
+                     "mismatched_fields": fields_bad,
+                     "longest_compared_bases": int(b.read_len[longest].max()),
+                     "sample": f"{n_s} random reads + the 64 longest",
+                     "fields": ["path_len", "n_pairs", "pairs", "from", "score", "avg"]}
+    log(f"[bench] parity: {out['parity']}")
+
+    # one thread, on a smaller random sample
+    n1 = int(min(b.n_reads, max(8, np.searchsorted(cum, rate / threads * args.cpu1_seconds))))
+    s1 = b.subset(order[:n1])
+    t = time.perf_counter()
+    oracle.align_batch(s1, n_threads=1)
+    dt1 = time.perf_counter() - t
+
+    # the 64 x 2 kb reads (the reference's CPU-runnable case)
+    c0 = synth.workload("cpu64x2kb", seed=args.seed + 2)
+    t0s = []
+    for _ in range(5):
+        t = time.perf_counter()
+        oracle.align_batch(c0)
+        t0s.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    oracle.align_batch(c0, n_threads=1)
+    t01 = time.perf_counter() - t
+    out["cpu_baseline"] = {
+        "value": round(cpu_gcups, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+        "sample": f"{idx.shape[0]} reads of batched16384_ln8kb ({n_s} random + the 64 longest; "
+                  f"{samp.nominal_cells() / 1e9:.2f} G cells, {dt:.1f} s)",
+        "reads_per_s": round(idx.shape[0] / dt, 2),
+        "cpu_model": cpu_model(), "host_threads": len(os.sched_getaffinity(0)),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+        "one_thread": {"gcups": round(s1.nominal_cells() / dt1 / 1e9, 4),
+                       "sample": f"{n1} random reads ({s1.nominal_cells() / 1e9:.2f} G cells, "
+                                 f"{dt1:.1f} s)"},
+        "cpu64x2kb": {"gcups": round(c0.nominal_cells() / min(t0s) / 1e9, 4),
+                      "one_thread_gcups": round(c0.nominal_cells() / t01 / 1e9, 4),
+                      "reads_per_s": round(c0.n_reads / min(t0s), 1)}}
+    log(f"[bench] cpu: {out['cpu_baseline']}")
+
+    # fills: cells inside the DP matrix, from the GPU's band moves
+    mv_off = engine.move_offsets(b.read_len, b.n_events)
+    fills = oracle.fills_from_moves(b.n_events, b.n_kmers, gpu["moves"], mv_off)
+    assert np.array_equal(fills[idx], ob["fills"]), "fills from the GPU's moves != the oracle's"
+    out["fills"] = {"total": int(fills.sum()), "nominal_cells": b.nominal_cells(),
+                    "fraction_of_nominal": round(float(fills.sum()) / b.nominal_cells(), 4)}
+
+    # FP64 error of the float scores, and how often the FP64 DP's path differs
+    fsub = np.unique(np.concatenate([order[:256], longest[-8:]]))
+    bs = b.subset(fsub)
+    o64 = oracle.align_batch(bs, mode=oracle.MODE_FP64, want_path=True)
+    rel = []
+    diff = 0
+    for j, i in enumerate(fsub):
+        s64 = o64["score"][j]
+        if np.isfinite(s64):
+            rel.append(abs(float(gpu["score"][i]) - s64) / abs(s64))
+        n = int(o64["path_len"][j])
+        g = int(gpu["pair_off"][i])
+        o = int(o64["pair_ptr"][j])
+        if int(gpu["path_len"][i]) != n or not np.array_equal(gpu["pairs"][g:g + n],
+                                                              o64["path"][o:o + n]):
+            diff += 1
+    rel = np.array(rel)
+    out["fp64_error"] = {"reads": int(fsub.shape[0]),
+                         "score_rel_err_max": float(rel.max()),
+                         "score_rel_err_median": float(np.median(rel)),
+                         "path_differs_fraction": round(diff / fsub.shape[0], 4),
+                         "note": "float32 GPU score vs the FP64 oracle's DP score; the FP64 DP "
+                                 "places its own band"}
+    log(f"[bench] fp64: {out['fp64_error']}")
+
+    # adjoint: central differences (h = 1e-3) of the FP64 score of each read's
+    # traced path, held fixed, against the GPU adjoint; and how far the FP64
+    # DP's path moves when only the band is held
+    sub, res, g_scal, g_model = fd_in["batch"], fd_in["res"], fd_in["g_scal"], fd_in["g_model"]
+    paths = []
+    for i in range(sub.n_reads):
+        o, n = int(res["pair_off"][i]), int(res["path_len"][i])
+        paths.append((res["pairs"][o:o + n], res["path_from"][o:o + n]))
+    cnt = np.zeros(4096, np.int64)
+    for i in range(sub.n_reads):
+        seq = sub.read_seq(i)
+        for (k, _), f in zip(*paths[i]):
+            if f != 2:
+                cnt[oracle.kmer_rank(seq[k:k + 6])] += 1
+    top = np.argsort(cnt)[-3:]
+    params = [("shift", i) for i in range(sub.n_reads)] + [("scale", i) for i in range(sub.n_reads)]
+    params += [("model", int(r), c) for r in top for c in range(3)]
+    h = 1e-3
+    worst = 0.0
+    for what in params:
+        reads = [what[1]] if what[0] != "model" else range(sub.n_reads)
+        f, steps = [], []
+        for sign in (1, -1):
+            model, scale, shift = sub.model.copy(), sub.scale.copy(), sub.shift.copy()
+            arr, pos = {"shift": (shift, what[1]), "scale": (scale, what[1])}.get(
+                what[0], (None, None))
+            if arr is not None:
+                v = np.float32(arr[pos] + sign * h)
+                steps.append(float(v) - float(arr[pos]))
+                arr[pos] = v
+            else:
+                v = np.float32(model[what[1], what[2]] + sign * h)
+                steps.append(float(v) - float(model[what[1], what[2]]))
+                model[what[1], what[2]] = v
+            f.append(sum(oracle.path_score(paths[i][0], paths[i][1], sub.read_seq(i),
+                                           sub.read_events(i), model, scale[i], shift[i])
+                         for i in reads))
+        fd = (f[0] - f[1]) / (steps[0] - steps[1])
+        an = {"shift": lambda: g_scal[what[1], 0], "scale": lambda: g_scal[what[1], 1]}.get(
+            what[0], lambda: g_model[what[1], what[2]])()
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-12))
+    cells = []
+    for i in range(sub.n_reads):
+        rm = engine.unpack_moves(fd_in["moves"],
+                                 int(engine.move_offsets(sub.read_len, sub.n_events)[i]),
+                                 int(sub.n_bands[i]))
+        r64 = oracle.align(sub.read_seq(i), sub.read_events(i), sub.model, float(sub.scale[i]),
+                           float(sub.shift[i]), mode=oracle.MODE_FP64, replay_moves=rm)
+        cells.append(len(set(map(tuple, r64["path"])) ^ set(map(tuple, paths[i][0]))))
+    out["adjoint_fd"] = {"workload": "adjoint1024x10kb (8 reads)", "h": h,
+                         "params": len(params), "max_rel_err": float(worst),
+                         "target": 1e-3,
+                         "held": "each read's traced path (FP64 path score)",
+                         "fp64_dp_path_cells_differing_band_held": cells}
+    log(f"[bench] adjoint fd: {out['adjoint_fd']}")
+
+    # getevents: the oracle, one thread, on the first reads up to ~2 s of work
+    raw_h, raw_off = ge_in
     t0 = time.perf_counter()
     done = 0
-    for i in range(b.n_reads):
+    for i in range(raw_off.shape[0] - 1):
         oracle.getevents(raw_h[raw_off[i]:raw_off[i + 1]])
         done = i + 1
         if time.perf_counter() - t0 > 2.0:
             break
     cpu_s = time.perf_counter() - t0
-    cpu_samples = int(raw_off[done])
-    res = {"workload": "getevents1024_ln5kb", "reads": b.n_reads, "samples": n_samples,
-           "events": n_ev, "ms": round(ms, 3),
-           "gsamples_per_s": round(n_samples / (ms * 1e-3) / 1e9, 3),
-           "alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
-           "chunks_rerun": reruns,
-           "cpu": {"gsamples_per_s": round(cpu_samples / cpu_s / 1e9, 4), "cores": 1,
-                   "kind": "port", "sample": f"first {done} reads ({cpu_samples} samples)"}}
-    log(f"[bench] getevents: {res}")
-    del raw, ev, cnt
-    return res
+    out["getevents_cpu"] = {"gsamples_per_s": round(int(raw_off[done]) / cpu_s / 1e9, 4),
+                            "cores": 1, "kind": "port",
+                            "sample": f"first {done} reads ({int(raw_off[done])} samples)"}
+    return out
 
 
 def main():
@@ -205,13 +732,13 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--plain-reads", type=int, default=256)
+    ap.add_argument("--cpu1-seconds", type=float, default=4.0)
     ap.add_argument("--profile", action="store_true",
                     help="headline workload only, default tuning, no autotune; also one "
                          "1 GiB device copy to calibrate HBM byte counters")
     ap.add_argument("--eager", action="store_true",
                     help="time eager launches instead of the captured graph")
-    ap.add_argument("--leg", choices=["postalign", "getevents"], default=None,
+    ap.add_argument("--leg", choices=["postalign", "getevents", "dropin", "plain"], default=None,
                     help="run only this extra measurement (its JSON on stdout)")
     ap.add_argument("--tuning", default=None,
                     help="WPB,WPC,FUSE,OVERLAP: use this core launch shape instead of "
@@ -224,229 +751,37 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
+    from abea_amd import engine
 
-    from abea_amd import engine, synth
-    from abea_amd._lib import PLAN_BATCHED, PLAN_PLAIN
-
+    default_tune = {"core_waves_per_block": 1, "core_waves_per_cu": 24, "traceback_overlap": 128,
+                    "use_graph": 1, "fuse_traceback": 1}
     if args.leg == "postalign":
         print(json.dumps(postalign_leg(args, engine.Context(0), None)), flush=True)
         return
     if args.leg == "getevents":
-        print(json.dumps(getevents_leg(args, engine.Context(0))), flush=True)
+        print(json.dumps(getevents_leg(args, engine.Context(0))[0]), flush=True)
+        return
+    if args.leg == "dropin":
+        print(json.dumps(dropin_leg(args)), flush=True)
+        return
+    if args.leg == "plain":
+        print(json.dumps(plain_leg(args, engine.Context(0), default_tune)), flush=True)
         return
 
-    t = time.perf_counter()
-    wl = "batched16384_ln8kb"
-    b = synth.workload(wl, seed=args.seed, with_aos=True)
-    cells = b.nominal_cells()
-    log(f"[bench] generated {wl}: {b.n_reads} reads, {int(b.read_len.sum()) / 1e6:.1f} Mb, "
-        f"{int(b.n_events.sum()) / 1e6:.1f} M events, {cells / 1e9:.2f} G cells "
-        f"in {time.perf_counter() - t:.1f}s")
     ctx = engine.Context(0)
-    db = engine.DeviceBatch.upload(b, 0, aos_events=True)
-    plan = engine.Plan(ctx, db, PLAN_BATCHED)
-    log(f"[bench] plan workspace {plan.workspace_bytes() / 1e9:.1f} GB")
-    plan.forward()
-    torch.cuda.synchronize()
-    if args.profile:
-        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE
-        # readings scale the forward's counters to bytes
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("[bench] calibration copy: 1 GiB read + 1 GiB written")
-        del src, dst
-    if args.tuning:
-        wpb, wpc, fuse, ovl = (int(x) for x in args.tuning.split(","))
-        plan.set_tuning(core_waves_per_block=wpb, core_waves_per_cu=wpc, fuse_traceback=fuse,
-                        traceback_overlap=ovl)
-    elif not args.profile:
-        t = time.perf_counter()
-        plan.autotune(reps=1)
-        log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s")
-    tune = plan.tuning()
-    log(f"[bench] tuning {tune}")
-
-    # headline: graph replay, whole steps
-    plan.set_tuning(use_graph=0 if args.eager else 1)
-    for _ in range(args.warmup):
-        plan.forward()
-    sec = timed_steps(plan, args.steps, torch)
-    gcups = cells / sec / 1e9
-    log(f"[bench] batched {'eager' if args.eager else 'graph'}: {sec * 1e3:.2f} ms/step, "
-        f"{gcups:.1f} GCUPS")
-
-    # per-kernel device times over the same number of eager steps
-    plan.set_tuning(use_graph=0)
-    plan.forward()
-    kt = np.zeros(3)
-    torch.cuda.synchronize()
-    for _ in range(args.steps):
-        plan.forward()
-        kt += np.array(plan.times()[:3])
-    kt /= args.steps
-    log(f"[bench] kernel ms queue_reset/core/post: {kt.round(3).tolist()}")
-    res = plan.results()
-    qc_pass = int((res["n_pairs"] > 0).sum())
-    walk_b, tb_b = alg_bytes(b, res["path_len"])
-    # the dominant kernel is the core; fused, it also does the traceback
-    core_bytes = walk_b + (tb_b if tune["fuse_traceback"] else 0)
-    achieved = core_bytes / (kt[1] * 1e-3) / 1e9
-    out = {
-        "metric": "ABEA GCUPS (batched persistent form, 1 GPU)",
-        "value": round(gcups, 3),
-        "unit": "GCUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(sec * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 cells, f64 transition sums",
-        "data": "synthetic (seeded; f5c event_t records in HBM)",
-        "config": {"workload": "batched16384_ln8kb", "reads": b.n_reads,
-                   "bases": int(b.read_len.sum()), "events": int(b.n_events.sum()),
-                   "nominal_cells": cells, "bandwidth": W, "tuning": tune},
-        "reads_per_s": round(b.n_reads / sec, 1),
-        "kernel_ms": {"queue_reset": round(kt[0], 3), "core": round(kt[1], 3),
-                      "post": round(kt[2], 3)},
-        "qc_pass_reads": qc_pass,
-        "roofline": {"bound": "hbm", "kernel": kernel_name(tune), "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRAFFIC_BYTES.get(kernel_name(tune)),
-                     "alg_bytes_per_launch": core_bytes},
-    }
-    vi = VALU_INSTS.get(kernel_name(tune))
-    if vi is not None:
-        # VALU issue rate against the ceiling of one wave instruction per SIMD
-        # per 4 cycles at the peak clock
-        rate = vi / (kt[1] * 1e-3)
-        peak = N_SIMD * CLOCK_HZ / 4
-        out["issue"] = {"bound": "valu", "valu_insts_per_launch": vi,
-                        "valu_insts_per_band": round(vi / int(b.n_bands.sum()), 2),
-                        "achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
-                        "unit": "G wave-instructions/s", "frac": round(rate / peak, 4)}
-    del plan, db
-    torch.cuda.empty_cache()
-
+    out, b, gpu, tune = headline_leg(args, ctx)
     if not args.no_extras and not args.profile:
-        # plain per-read launches vs batched on the same sample of the
-        # 4,096-read log-normal (median 5 kb) workload
-        L1 = synth.workload_lengths("plain4096_ln5kb", seed=args.seed)
-        b1 = synth.generate(L1, seed=args.seed + 1)
-        sub = b1.subset(np.arange(min(args.plain_reads, b1.n_reads)))
-        dsub = engine.DeviceBatch.upload(sub, 0)
-        p_plain = engine.Plan(ctx, dsub, PLAN_PLAIN)
-        p_plain.forward()
-        s_plain = timed_steps(p_plain, 1, torch)
-        p_b = engine.Plan(ctx, dsub, PLAN_BATCHED)
-        p_b.forward()
-        s_b = timed_steps(p_b, 3, torch)
-        d1 = engine.DeviceBatch.upload(b1, 0)
-        p_full = engine.Plan(ctx, d1, PLAN_BATCHED)
-        p_full.set_tuning(**{k: v for k, v in tune.items()})
-        p_full.forward()
-        s_full = timed_steps(p_full, 3, torch)
-        out["plain_vs_batched"] = {
-            "workload": "plain4096_ln5kb", "sample_reads": sub.n_reads,
-            "plain_gcups": round(sub.nominal_cells() / s_plain / 1e9, 3),
-            "batched_gcups_same_sample": round(sub.nominal_cells() / s_b / 1e9, 3),
-            "batched_gcups_all_4096": round(b1.nominal_cells() / s_full / 1e9, 3),
-            "plain_reads_per_s": round(sub.n_reads / s_plain, 2),
-            "batched_reads_per_s_all_4096": round(b1.n_reads / s_full, 1),
-        }
-        log(f"[bench] plain/batched: {out['plain_vs_batched']}")
-        del p_plain, p_b, p_full, d1, dsub
-
-        # adjoint / forward on 1,024 x 10 kb
-        b3 = synth.workload("adjoint1024x10kb", seed=args.seed + 3)
-        d3 = engine.DeviceBatch.upload(b3, 0)
-        p3 = engine.Plan(ctx, d3, PLAN_BATCHED)
-        p3.set_tuning(**{k: v for k, v in tune.items()})
-        p3.forward()
-        s_fwd = timed_steps(p3, 3, torch)
-        gsc = torch.ones(b3.n_reads, device="cuda")
-        p3.adjoint(gsc)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(3):
-            p3.adjoint(gsc)
-        torch.cuda.synchronize()
-        s_adj = (time.perf_counter() - t) / 3
-        out["adjoint"] = {"workload": "adjoint1024x10kb", "forward_ms": round(s_fwd * 1e3, 3),
-                          "adjoint_ms": round(s_adj * 1e3, 3),
-                          "adjoint_over_forward": round(s_adj / s_fwd, 4)}
-        log(f"[bench] adjoint: {out['adjoint']}")
-        del p3, d3
-
+        out["plain_vs_batched"] = plain_leg(args, ctx, tune)
+        out["dropins"] = dropin_leg(args)
+        out["adjoint"], fd_in = adjoint_leg(args, ctx, tune)
         out["postalign"] = postalign_leg(args, ctx, tune)
-        out["getevents"] = getevents_leg(args, ctx)
-        torch.cuda.empty_cache()
-
-        # scaling refinement: 8,192 x 20 kb reads, scalings perturbed (shift
-        # +-5, scale +-10%), five graph-replayed forward + Gauss-Newton steps
-        b4 = synth.workload("refine8192x20kb", seed=args.seed + 4)
-        true_scale, true_shift = synth.perturb_scalings(b4, seed=args.seed + 4)
-        d4 = engine.DeviceBatch.upload(b4, 0)
-        p4 = engine.Plan(ctx, d4, PLAN_BATCHED)
-        p4.forward()
-        p4.autotune(reps=1)
-        p4.set_tuning(use_graph=1)
-        torch.cuda.synchronize()
-        s0 = d4.scalings.cpu().numpy()
-        steps = 5
-        t = time.perf_counter()
-        ms = p4.refine(steps, 1.0)
-        wall = (time.perf_counter() - t) / steps
-        s1 = d4.scalings.cpu().numpy()
-        out["refine"] = {
-            "workload": "refine8192x20kb", "steps": steps, "tuning": p4.tuning(),
-            "ms_per_step": round(ms, 3),
-            "wall_ms_per_step": round(wall * 1e3, 3),
-            "gcups_per_step": round(b4.nominal_cells() / (ms * 1e-3) / 1e9, 2),
-            "shift_abs_err_before": {"mean": round(float(np.abs(s0[:, 1] - true_shift).mean()), 4),
-                                     "median": round(float(np.median(np.abs(s0[:, 1] - true_shift))), 4)},
-            "shift_abs_err_after": {"mean": round(float(np.abs(s1[:, 1] - true_shift).mean()), 4),
-                                    "median": round(float(np.median(np.abs(s1[:, 1] - true_shift))), 4)},
-            "scale_abs_err_before": {"mean": round(float(np.abs(s0[:, 0] - true_scale).mean()), 5),
-                                     "median": round(float(np.median(np.abs(s0[:, 0] - true_scale))), 5)},
-            "scale_abs_err_after": {"mean": round(float(np.abs(s1[:, 0] - true_scale).mean()), 5),
-                                    "median": round(float(np.median(np.abs(s1[:, 0] - true_scale))), 5)}}
-        log(f"[bench] refine: {out['refine']}")
-        del p4, d4, b4
-        torch.cuda.empty_cache()
-
-        # CPU baseline: the oracle (f5c semantics), OpenMP over reads, on a
-        # bounded random sample of the headline workload
+        out["getevents"], ge_in = getevents_leg(args, ctx)
+        out["refine"] = refine_leg(args, ctx)
         if args.cpu_seconds > 0:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import oracle
-            rng = np.random.default_rng(args.seed)
-            order = rng.permutation(b.n_reads)
-            probe = b.subset(order[:32])
-            t = time.perf_counter()
-            _, _, used, _ = oracle.align_batch(probe)
-            rate = probe.nominal_cells() / (time.perf_counter() - t)
-            want = rate * args.cpu_seconds
-            cum = np.cumsum(b.n_bands[order] * W)
-            n_s = int(min(b.n_reads, max(32, np.searchsorted(cum, want))))
-            samp = b.subset(order[:n_s])
-            t = time.perf_counter()
-            _, _, used, _ = oracle.align_batch(samp)
-            dt = time.perf_counter() - t
-            cpu_gcups = samp.nominal_cells() / dt / 1e9
-            out["cpu_baseline"] = {
-                "value": round(cpu_gcups, 4), "unit": "GCUPS", "cores": used, "kind": "port",
-                "sample": f"{n_s} random reads of batched16384_ln8kb "
-                          f"({samp.nominal_cells() / 1e9:.2f} G cells, {dt:.1f} s)",
-                "reads_per_s": round(n_s / dt, 2)}
-            out["gpu_over_cpu"] = round(gcups / cpu_gcups, 1)
-            log(f"[bench] cpu: {out['cpu_baseline']}")
+            extra = cpu_leg(args, b, gpu, fd_in, ge_in)
+            out.update(extra)
+            out["getevents"]["cpu"] = out.pop("getevents_cpu")
+            out["gpu_over_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
 

@@ -76,11 +76,15 @@ typedef struct {
   char hmm_state; /* 'M' or 'E' */
 } abea_event_alignment_t; /* event_alignment_t, GPU/f5c.h:191-207 (32 bytes) */
 
-/* read status bits, db_t.read_stat_flag (GPU/f5c.h:49-51) */
+/* read status bits, db_t.read_stat_flag (GPU/f5c.h:49-52) */
 #define ABEA_FAILED_CALIBRATION 0x001
 #define ABEA_FAILED_ALIGNMENT 0x002
+#define ABEA_FAILED_QUALITY_CHK 0x004
 /* f5c's bound on the recalibrated var (read fails calibration above it) */
 #define ABEA_MIN_CALIBRATION_VAR 2.5f
+/* f5c's stay filter: a calibrated read with more events per base than this
+ * fails the quality check */
+#define ABEA_MAX_EVENTS_PER_BASE 5.0
 
 typedef enum {
   ABEA_OK = 0,
@@ -100,9 +104,10 @@ const char *abea_last_error(void);
  * out_2 must hold 2 * events.n pairs (f5c's allocation,
  * GPU/align_cuda.c:496); a path that would not fit is reported as a failed
  * read (0) instead of overrunning.  sample_rate is unused, as in the
- * reference (GPU/align_cuda.cl:84-88).  Runs the plain per-read form on the
- * calling thread's default context; returns -1 on a device error
- * (abea_last_error() tells which). */
+ * reference (GPU/align_cuda.cl:84-88).  Runs the plain per-read form on a
+ * pooled context of the current device (abea_host_contexts()), so threads
+ * may call it concurrently; returns -1 on a device error (abea_last_error()
+ * tells which). */
 int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_len,
                    abea_event_table events, abea_model_t *models, abea_scalings_t scaling,
                    float sample_rate);
@@ -119,9 +124,24 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
                             abea_aligned_pair_t **event_align_pairs,
                             int32_t *n_event_align_pairs);
 
+/* Phases of the calling thread's last abea_align() or abea_align_db() call,
+ * in ms: [0] flatten (host), [1] host-to-device copies of the inputs, [2]
+ * planning (per-read metadata, sort, its upload), [3] kernels, [4]
+ * device-to-host copies ([1]-[4] hipEvent-timed on the call's stream), [5]
+ * unflatten (host), [6] the whole call.  The reference's host times the
+ * same phases (GPU/align_cuda.c:59-74, 378-385). */
+void abea_host_last_times(double times_ms[7]);
+
+/* The host-buffer entry points draw device contexts (a stream and device
+ * buffers) from a process-wide pool: a call takes one for the current device
+ * and returns it when it ends, so calls from any number of threads (f5c's
+ * align() workers, GPU/f5c.h:57-59) hold at most as many contexts as ran at
+ * once.  Returns how many contexts the pool has created. */
+int32_t abea_host_contexts(void);
+
 /* ---------------------------------------------------------------------- */
 /* (1b) The f5c steps either side of align(), per read, host buffers.  Each
- * runs on the calling thread's default context (one small launch); use the
+ * runs on a pooled context of the current device (one small launch); use the
  * batched or plan forms below for many reads.  On a device error they set
  * abea_last_error() and return: NaN scalings (MOM), -1 (postalign), false
  * (recalibrate_model). */
@@ -159,7 +179,9 @@ bool abea_recalibrate_model(abea_model_t *pore_model, abea_event_table et,
  * (n_event_align_pairs[i] records) and base_to_event_map[i] (read_len[i] - 5
  * entries), runs postalign and recalibrate_model(scale_var = true) and sets
  * read_stat_flag[i] to ABEA_FAILED_CALIBRATION when recalibration did not
- * run or var > ABEA_MIN_CALIBRATION_VAR; other reads get
+ * run or var > ABEA_MIN_CALIBRATION_VAR, else to ABEA_FAILED_QUALITY_CHK
+ * when events_per_base[i] > ABEA_MAX_EVENTS_PER_BASE (f5c's filter of reads
+ * with too many stays), else 0; reads align() failed get
  * ABEA_FAILED_ALIGNMENT, NULL arrays, n_event_alignment[i] = 0 and
  * events_per_base[i] = 0.  scalings[] is updated in place, as f5c does.
  * The caller frees the malloc()ed arrays. */
@@ -283,13 +305,19 @@ abea_status_t abea_plan_set_replay(abea_plan_t *plan, const uint32_t *moves_host
 
 /* Adjoint of the summed weighted best-path score.  Needs the pairs and
  * path_from of a forward on the same plan.  grad_score[n_reads] (device,
- * f32) weights each read.  Outputs (device, caller-zeroed is not needed;
- * they are overwritten):
+ * f32) weights each read.  Outputs (device; overwritten, no zeroing needed):
  *   grad_scalings[n_reads*3]  d/d(shift, scale, var), f32 (var column is 0:
  *                             the reference emission ignores var,
  *                             GPU/align_cuda.cl:95)
- *   grad_model[4096*2]        d/d(level_mean, level_stdv), f64 accumulated,
- *                             with level_log_stdv = log(level_stdv). */
+ *   grad_model[4096*3]        per k-mer rank, the partial derivatives in
+ *                             (level_mean, level_stdv, level_log_stdv), f64:
+ *                             the forward reads the cached level_log_stdv as
+ *                             an input of its own (GPU/align_cuda.cl:517-518),
+ *                             so d/d level_stdv holds it fixed; with
+ *                             level_log_stdv = log(level_stdv) the total
+ *                             derivative is column 1 + column 2 / level_stdv.
+ * Deterministic: the k-mer sums are exact fixed-point sums of the per-run
+ * terms (integer atomics), so repeated calls give identical bits. */
 abea_status_t abea_plan_adjoint(abea_plan_t *plan, const abea_output_t *fwd,
                                 const float *grad_score, float *grad_scalings,
                                 double *grad_model);

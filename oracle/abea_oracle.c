@@ -126,12 +126,14 @@ int32_t orc_align(orc_pair_t *out, int32_t out_cap, const char *seq, int32_t seq
 
 void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n, const char *seq,
                        const float *ev_mean, const orc_model_t *models, float scale, float shift,
-                       double weight, double *g_read, double *g_mu, double *g_sigma) {
+                       double weight, double *g_read, double *g_mu, double *g_sigma,
+                       double *g_lambda) {
   /* score = sum over D/U path cells of em(e,k) + terms that do not depend on
    * the scalings or the model.  With a = (x - (scale*mu + shift)) / sigma and
-   * em = c - log(sigma) - a^2/2:
+   * em = c - lambda - a^2/2 (lambda = the cached level_log_stdv):
    *   d em/d shift = a/sigma, d em/d scale = a*mu/sigma,
-   *   d em/d mu = a*scale/sigma, d em/d sigma = (a^2 - 1)/sigma,
+   *   d em/d mu = a*scale/sigma, d em/d sigma = a^2/sigma (lambda held),
+   *   d em/d lambda = -1,
    *   d em/d var = 0 (GPU/align_cuda.cl:95: gp_stdv = level_stdv * 1). */
   double gs = 0, gc = 0;
   for (int32_t i = 0; i < n; ++i) {
@@ -144,7 +146,8 @@ void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n, c
     gs += a / sg;
     gc += a * mu / sg;
     if (g_mu) g_mu[r] += weight * (a * (double)scale / sg);
-    if (g_sigma) g_sigma[r] += weight * ((a * a - 1.0) / sg);
+    if (g_sigma) g_sigma[r] += weight * (a * a / sg);
+    if (g_lambda) g_lambda[r] -= weight;
   }
   if (g_read) {
     g_read[0] += weight * gs;
@@ -195,8 +198,7 @@ static int cmp_desc(const void *a, const void *b) {
 int orc_align_batch(int32_t n_reads, const char *seqs, const int64_t *seq_ptr,
                     const int32_t *seq_len, const float *ev_means, const int64_t *ev_ptr,
                     const int32_t *n_events, const orc_model_t *models, const float *scale,
-                    const float *shift, orc_pair_t *out, const int64_t *pair_ptr,
-                    int32_t *n_pairs, float *scores, int n_threads) {
+                    const float *shift, int mode, const orc_batch_out_t *o, int n_threads) {
   /* The reference's CPU path is OpenMP/pthreads over reads
    * (GPU/f5c.h:57-59); longest reads first keeps the threads balanced. */
   sort_item *order = (sort_item *)malloc(sizeof(sort_item) * (size_t)(n_reads > 0 ? n_reads : 1));
@@ -214,15 +216,52 @@ int orc_align_batch(int32_t n_reads, const char *seqs, const int64_t *seq_ptr,
     int32_t i = order[j].idx;
     orc_info_t info;
     int32_t cap = n_events[i] + (seq_len[i] > 5 ? seq_len[i] - 5 : 0);
-    orc_pair_t *o = out ? out + pair_ptr[i] : NULL;
-    n_pairs[i] = orc_fill_f32(o, o ? cap : 0, seqs + seq_ptr[i], seq_len[i], ev_means + ev_ptr[i],
-                              n_events[i], models, scale[i], shift[i], NULL, NULL, NULL, NULL,
-                              NULL, NULL, &info);
-    if (scores) scores[i] = (float)info.score;
+    int64_t off = o->pair_ptr ? o->pair_ptr[i] : 0;
+    orc_pair_t *pr = o->pairs ? o->pairs + off : NULL;
+    orc_pair_t *path = o->path ? o->path + off : NULL;
+    uint8_t *from = o->from ? o->from + off : NULL;
+    int32_t np = orc_align(pr, pr ? cap : 0, seqs + seq_ptr[i], seq_len[i], ev_means + ev_ptr[i],
+                           n_events[i], models, scale[i], shift[i], mode, NULL, NULL, path, from,
+                           NULL, NULL, &info);
+    if (o->n_pairs) o->n_pairs[i] = np;
+    if (o->path_len) o->path_len[i] = info.path_len;
+    if (o->score) o->score[i] = info.score;
+    if (o->avg) o->avg[i] = info.avg_log_emission;
+    if (o->fills) o->fills[i] = info.fills;
   }
 #ifdef _OPENMP
   used = n_threads;
 #endif
   free(order);
   return used;
+}
+
+void orc_fills_from_moves(int32_t n_reads, const int32_t *n_events, const int32_t *n_kmers,
+                          const uint32_t *moves, const int64_t *mv_off, int64_t *fills) {
+  /* The fill range of each band, align_swi.cl:420-432, replayed from the
+   * moves: band b >= 2 moves right (K + 1) or down (E + 1) from band b - 1;
+   * band 1's lower-left cell is (E, K) = (50, -51), align_swi.cl:306-322. */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int32_t i = 0; i < n_reads; ++i) {
+    const int32_t ne = n_events[i], nk = n_kmers[i];
+    int64_t f = 0;
+    if (ne > 0 && nk > 0) {
+      const int64_t nb = (int64_t)ne + nk + 2;
+      const uint32_t *mv = moves + mv_off[i];
+      int64_t E = HALF_W, K = -1 - HALF_W;
+      for (int64_t b = 2; b < nb; ++b) {
+        if ((mv[b >> 5] >> (b & 31)) & 1u) ++K; else ++E;
+        int64_t lo = -K, emin = E - (ne - 1);
+        if (emin > lo) lo = emin;
+        if (lo < 0) lo = 0;
+        int64_t hi = nk - K;
+        if (E + 1 < hi) hi = E + 1;
+        if (hi > W) hi = W;
+        if (hi > lo) f += hi - lo;
+      }
+    }
+    fills[i] = f;
+  }
 }

@@ -75,33 +75,56 @@ int32_t orc_align(orc_pair_t *out, int32_t out_cap,
 
 /*
  * FP64 gradient of the best-path score along a given path (from codes as
- * returned by orc_align): only D and U cells carry an emission.
+ * returned by orc_align): only D and U cells carry an emission.  The partial
+ * derivatives of the function the DP computes, which reads level_log_stdv as
+ * its own input (CACHED_LOG, GPU/align_cuda.cl:517-518):
  * g_read[0..2] += d/d(shift, scale, var)   (var term is identically zero)
- * g_mu[rank], g_sigma[rank] += d/d level_mean, d/d level_stdv (with
- * level_log_stdv = log(level_stdv) differentiated as well).
+ * g_mu[rank]     += d/d level_mean
+ * g_sigma[rank]  += d/d level_stdv with level_log_stdv held
+ * g_lambda[rank] += d/d level_log_stdv
+ * With level_log_stdv = log(level_stdv) the total derivative in level_stdv
+ * is g_sigma + g_lambda / level_stdv.  Any output may be NULL.
  */
 void orc_path_gradient(const orc_pair_t *path, const uint8_t *from, int32_t n,
                        const char *seq, const float *ev_mean,
                        const orc_model_t *models, float scale, float shift,
                        double weight, double *g_read, double *g_mu,
-                       double *g_sigma);
+                       double *g_sigma, double *g_lambda);
 
 /* FP64 score of a fixed path (from codes as returned by orc_align). */
 double orc_path_score(const orc_pair_t *path, const uint8_t *from, int32_t n, const char *seq,
                       const float *ev_mean, int32_t n_events, const orc_model_t *models,
                       float scale, float shift);
 
-/* Batched CPU baseline: OpenMP over reads, longest first.
- * seq_ptr / ev_ptr are CSR offsets (int64) into seqs / ev_means.
- * Writes n_pairs[i], scores[i]; pairs written at out + pair_ptr[i] when out
- * is non-NULL.  Returns the number of threads used. */
+/* Batched CPU baseline and parity reference: OpenMP over reads, longest
+ * first.  seq_ptr / ev_ptr are CSR offsets (int64) into seqs / ev_means.
+ * Read i's slots start at out->pair_ptr[i] (capacity n_events + n_kmers);
+ * every output array may be NULL.  mode is ORC_MODE_F5C or ORC_MODE_FP64.
+ * Returns the number of threads used. */
+typedef struct {
+  orc_pair_t *pairs;        /* ascending pairs after QC, at pair_ptr[i] */
+  orc_pair_t *path;         /* ascending traced path before QC, at pair_ptr[i] */
+  uint8_t *from;            /* its from codes (0 D, 1 U, 2 L), at pair_ptr[i] */
+  const int64_t *pair_ptr;  /* needed when pairs, path or from is given */
+  int32_t *n_pairs;         /* pairs after QC (0 = failed) */
+  int32_t *path_len;        /* path length before QC */
+  double *score;            /* best end score (float-valued in ORC_MODE_F5C) */
+  double *avg;              /* QC statistic */
+  int64_t *fills;           /* cells computed (the reference's fills counter) */
+} orc_batch_out_t;
+
 int orc_align_batch(int32_t n_reads, const char *seqs, const int64_t *seq_ptr,
                     const int32_t *seq_len, const float *ev_means,
                     const int64_t *ev_ptr, const int32_t *n_events,
                     const orc_model_t *models, const float *scale,
-                    const float *shift, orc_pair_t *out,
-                    const int64_t *pair_ptr, int32_t *n_pairs, float *scores,
+                    const float *shift, int mode, const orc_batch_out_t *out,
                     int n_threads);
+
+/* Cells computed (fills) of each read from its band moves, packed as the
+ * library's move words: read i's band b is bit b % 32 of word
+ * mv_off[i] + b / 32 (1 = right).  Reads with no k-mers or events get 0. */
+void orc_fills_from_moves(int32_t n_reads, const int32_t *n_events, const int32_t *n_kmers,
+                          const uint32_t *moves, const int64_t *mv_off, int64_t *fills);
 
 /* ---- the steps either side of align(): postalign.c ---------------------- */
 typedef struct { int32_t start, stop; } orc_index_pair_t; /* index_pair_t, GPU/f5c.h:184-188 */

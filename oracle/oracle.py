@@ -42,6 +42,12 @@ class Scalings(C.Structure):
                 ("log_var", C.c_float)]
 
 
+class BatchOut(C.Structure):
+    _fields_ = [("pairs", C.c_void_p), ("path", C.c_void_p), ("from_", C.c_void_p),
+                ("pair_ptr", C.c_void_p), ("n_pairs", C.c_void_p), ("path_len", C.c_void_p),
+                ("score", C.c_void_p), ("avg", C.c_void_p), ("fills", C.c_void_p)]
+
+
 # event_alignment_t (GPU/f5c.h:191-207), x86-64 layout, 32 bytes
 EVENT_ALIGNMENT_DTYPE = np.dtype({
     "names": ["ref_kmer", "ref_position", "read_idx", "event_idx", "rc", "model_kmer",
@@ -76,7 +82,7 @@ def lib():
         L.orc_emission_f32.argtypes = [C.c_float, P, C.c_float, C.c_float]
         L.orc_path_gradient.restype = None
         L.orc_path_gradient.argtypes = [P, P, C.c_int32, P, P, P, C.c_float, C.c_float,
-                                        C.c_double, P, P, P]
+                                        C.c_double, P, P, P, P]
         L.orc_path_score.restype = C.c_double
         L.orc_path_score.argtypes = [P, P, C.c_int32, C.c_char_p, P, C.c_int32, P, C.c_float,
                                      C.c_float]
@@ -95,7 +101,10 @@ def lib():
         L.orc_getevents.restype = C.c_int64
         L.orc_getevents.argtypes = [P, C.c_int64, P, C.c_int64]
         L.orc_align_batch.restype = C.c_int
-        L.orc_align_batch.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int]
+        L.orc_align_batch.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P, C.c_int,
+                                      C.POINTER(BatchOut), C.c_int]
+        L.orc_fills_from_moves.restype = None
+        L.orc_fills_from_moves.argtypes = [C.c_int32, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -171,9 +180,15 @@ def align(seq: bytes, ev_mean: np.ndarray, model: np.ndarray, scale: float, shif
     return res
 
 
-def path_gradient(path, frm, seq: bytes, ev_mean, model, scale, shift, weight=1.0):
+def path_gradient(path, frm, seq: bytes, ev_mean, model, scale, shift, weight=1.0,
+                  partials: bool = False):
     """FP64 gradient of one read's best-path score along a fixed path.
-    Returns (g_read[3], g_mu[4096], g_sigma[4096])."""
+
+    partials=False: (g_read[3], g_mu[4096], g_sigma[4096]) with g_sigma the
+    total derivative in level_stdv when level_log_stdv = log(level_stdv).
+    partials=True: (g_read, g_mu, g_sigma, g_lambda), the partial
+    derivatives of the DP's function of (level_mean, level_stdv,
+    level_log_stdv), the log column read as its own input."""
     path = np.ascontiguousarray(path, np.int32)
     frm = np.ascontiguousarray(frm, np.uint8)
     ev = np.ascontiguousarray(ev_mean, np.float32)
@@ -181,11 +196,14 @@ def path_gradient(path, frm, seq: bytes, ev_mean, model, scale, shift, weight=1.
     g_read = np.zeros(3, np.float64)
     g_mu = np.zeros(4096, np.float64)
     g_sg = np.zeros(4096, np.float64)
+    g_lam = np.zeros(4096, np.float64)
     sbuf = C.create_string_buffer(bytes(seq), len(seq) + 1)
     lib().orc_path_gradient(_p(path), _p(frm), int(path.shape[0]), C.cast(sbuf, C.c_void_p),
                             _p(ev), _p(model), float(scale), float(shift), float(weight),
-                            _p(g_read), _p(g_mu), _p(g_sg))
-    return g_read, g_mu, g_sg
+                            _p(g_read), _p(g_mu), _p(g_sg), _p(g_lam))
+    if partials:
+        return g_read, g_mu, g_sg, g_lam
+    return g_read, g_mu, g_sg + g_lam / model[:, 1].astype(np.float64)
 
 
 def path_score(path, frm, seq: bytes, ev_mean, model, scale, shift) -> float:
@@ -198,21 +216,34 @@ def path_score(path, frm, seq: bytes, ev_mean, model, scale, shift) -> float:
                                       int(ev.shape[0]), _p(model), float(scale), float(shift)))
 
 
-def align_batch(batch, n_threads: int = 0, want_pairs: bool = False):
-    """OpenMP CPU baseline over a Batch (abea_amd.synth.Batch layout).
-    Returns (n_pairs, scores, threads_used, pairs_or_None)."""
+def align_batch(batch, n_threads: int = 0, mode: int = MODE_F5C, want_path: bool = False,
+                want_pairs: bool = False):
+    """OpenMP CPU baseline / parity reference over a Batch
+    (abea_amd.synth.Batch layout), longest reads first.
+
+    Returns a dict: n_pairs, path_len, score (float64; float-valued in
+    MODE_F5C), avg, fills, threads; with want_path the pre-QC path and its
+    from codes ("path", "from") and with want_pairs the post-QC pairs, read
+    i's at pair_ptr[i]."""
     n = batch.n_reads
-    n_pairs = np.zeros(n, np.int32)
-    scores = np.zeros(n, np.float32)
-    pairs = None
-    pair_ptr = None
-    if want_pairs:
-        cap = batch.n_events.astype(np.int64) + np.maximum(batch.read_len.astype(np.int64) - 5, 0)
-        pair_ptr = np.zeros(n, np.int64)
+    cap = batch.n_events.astype(np.int64) + np.maximum(batch.read_len.astype(np.int64) - 5, 0)
+    pair_ptr = np.zeros(n, np.int64)
+    if n > 1:
         pair_ptr[1:] = np.cumsum(cap)[:-1]
-        pairs = np.zeros((int(cap.sum()) + 1, 2), np.int32)
+    tot = int(cap.sum()) + 1
+    res = {"n_pairs": np.zeros(n, np.int32), "path_len": np.zeros(n, np.int32),
+           "score": np.zeros(n, np.float64), "avg": np.zeros(n, np.float64),
+           "fills": np.zeros(n, np.int64), "pair_ptr": pair_ptr}
+    if want_path:
+        res["path"] = np.zeros((tot, 2), np.int32)
+        res["from"] = np.zeros(tot, np.uint8)
+    if want_pairs:
+        res["pairs"] = np.zeros((tot, 2), np.int32)
+    o = BatchOut(_p(res.get("pairs")), _p(res.get("path")), _p(res.get("from")), _p(pair_ptr),
+                 _p(res["n_pairs"]), _p(res["path_len"]), _p(res["score"]), _p(res["avg"]),
+                 _p(res["fills"]))
     seqs = np.ascontiguousarray(batch.seq, np.uint8)
-    used = lib().orc_align_batch(
+    res["threads"] = int(lib().orc_align_batch(
         n, _p(seqs), _p(np.ascontiguousarray(batch.seq_ptr, np.int64)),
         _p(np.ascontiguousarray(batch.read_len, np.int32)),
         _p(np.ascontiguousarray(batch.ev_mean, np.float32)),
@@ -220,9 +251,21 @@ def align_batch(batch, n_threads: int = 0, want_pairs: bool = False):
         _p(np.ascontiguousarray(batch.n_events, np.int32)),
         _p(np.ascontiguousarray(batch.model, np.float32)),
         _p(np.ascontiguousarray(batch.scale, np.float32)),
-        _p(np.ascontiguousarray(batch.shift, np.float32)),
-        _p(pairs), _p(pair_ptr), _p(n_pairs), _p(scores), int(n_threads))
-    return n_pairs, scores, int(used), (pairs, pair_ptr)
+        _p(np.ascontiguousarray(batch.shift, np.float32)), int(mode), C.byref(o),
+        int(n_threads)))
+    return res
+
+
+def fills_from_moves(n_events, n_kmers, moves, mv_off) -> np.ndarray:
+    """Cells computed per read (the reference's fills counter,
+    GPU/align_cuda.cl:559) from band moves in the library's packed layout."""
+    ne = np.ascontiguousarray(n_events, np.int32)
+    nk = np.ascontiguousarray(n_kmers, np.int32)
+    mv = np.ascontiguousarray(moves, np.uint32)
+    off = np.ascontiguousarray(mv_off, np.int64)
+    out = np.zeros(ne.shape[0], np.int64)
+    lib().orc_fills_from_moves(int(ne.shape[0]), _p(ne), _p(nk), _p(mv), _p(off), _p(out))
+    return out
 
 
 def postalign(seq: bytes, pairs: np.ndarray):
@@ -269,14 +312,18 @@ def estimate_scalings_mom(seq: bytes, ev_mean, model):
 
 FAILED_CALIBRATION = 0x1  # GPU/f5c.h:50
 FAILED_ALIGNMENT = 0x2    # GPU/f5c.h:51
+FAILED_QUALITY_CHK = 0x4  # GPU/f5c.h:52
 MIN_CALIBRATION_VAR = 2.5
+MAX_EVENTS_PER_BASE = 5.0  # f5c's filter of reads with too many stays
 
 
 def process_read(seq: bytes, ev_mean, model, scalings, scale_var: bool = True):
     """f5c's per-read flow after align(): QC-passed pairs -> postalign ->
     recalibrate_model -> read status flag (FAILED_ALIGNMENT when align's QC
     failed; FAILED_CALIBRATION when recalibration did not run or var >
-    2.5).  Returns a dict."""
+    2.5; then FAILED_QUALITY_CHK when events_per_base > 5, the filter of
+    reads with too many stays that f5c applies after a passed calibration).
+    Returns a dict."""
     sc = np.asarray(scalings, np.float32)
     r = align(seq, ev_mean, model, float(sc[0]), float(sc[1]))
     out = {"align": r, "records": np.zeros(0, EVENT_ALIGNMENT_DTYPE), "map": None,
@@ -289,6 +336,8 @@ def process_read(seq: bytes, ev_mean, model, scalings, scale_var: bool = True):
     out.update(records=rec, map=bmap, events_per_base=epb, scalings=new, calibrated=ok)
     if not ok or new[2] > MIN_CALIBRATION_VAR:
         out["flag"] = FAILED_CALIBRATION
+    elif epb > MAX_EVENTS_PER_BASE:
+        out["flag"] = FAILED_QUALITY_CHK
     return out
 
 

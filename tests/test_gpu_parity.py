@@ -204,9 +204,9 @@ def test_medium_reads_vs_oracle(ctx):
     assert compare_to_oracle(b, res, idx=range(0, 256, 4)) == []
     # every read: path is a monotone lattice walk ending on the last k-mer
     import oracle
-    n_pairs, scores, _, _ = oracle.align_batch(b)
-    assert np.array_equal(n_pairs, res["n_pairs"])
-    assert np.array_equal(scores, res["score"])
+    ob = oracle.align_batch(b)
+    assert np.array_equal(ob["n_pairs"], res["n_pairs"])
+    assert np.array_equal(ob["score"].astype(np.float32), res["score"])
 
 
 def test_drop_in_align_and_align_db(ctx):
@@ -242,6 +242,8 @@ def test_drop_in_align_and_align_db(ctx):
                          nout.ctypes.data_as(C.c_void_p))
     assert st == 0, L.abea_last_error()
     libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free.restype = None
     for i in range(n):
         assert nout[i] == golden[i].shape[0]
         got = np.ctypeslib.as_array(C.cast(outp[i], C.POINTER(C.c_int32)), (int(nout[i]) * 2,))
@@ -250,7 +252,8 @@ def test_drop_in_align_and_align_db(ctx):
 
 
 def test_adjoint_matches_fp64_path_gradient(ctx):
-    """GPU adjoint vs the oracle's FP64 gradient along the same paths."""
+    """GPU adjoint vs the oracle's FP64 partial derivatives along the same
+    paths, and bitwise equal on a second call (fixed-point k-mer sums)."""
     import oracle
     from abea_amd import engine, synth
     b = synth.workload("cpu64x2kb", seed=41, n=12)
@@ -263,20 +266,24 @@ def test_adjoint_matches_fp64_path_gradient(ctx):
     g_scal = g_scal.cpu().numpy()
     g_model = g_model.cpu().numpy()
     ref_read = np.zeros((b.n_reads, 3))
-    ref_mu = np.zeros(4096)
-    ref_sg = np.zeros(4096)
+    ref = np.zeros((4096, 3))
     wn = w.cpu().numpy().astype(np.float64)
     for i in range(b.n_reads):
         pairs, frm = read_path(res, i)
-        gr, gm, gsg = oracle.path_gradient(pairs, frm, b.read_seq(i), b.read_events(i), b.model,
-                                           b.scale[i], b.shift[i], weight=wn[i])
+        gr, gm, gsg, glam = oracle.path_gradient(pairs, frm, b.read_seq(i), b.read_events(i),
+                                                 b.model, b.scale[i], b.shift[i], weight=wn[i],
+                                                 partials=True)
         ref_read[i] = gr
-        ref_mu += gm
-        ref_sg += gsg
+        ref[:, 0] += gm
+        ref[:, 1] += gsg
+        ref[:, 2] += glam
     np.testing.assert_allclose(g_scal[:, :2], ref_read[:, :2], rtol=1e-5, atol=1e-3)
     assert np.all(g_scal[:, 2] == 0)
-    np.testing.assert_allclose(g_model[:, 0], ref_mu, rtol=1e-9, atol=1e-9)
-    np.testing.assert_allclose(g_model[:, 1], ref_sg, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(g_model, ref, rtol=1e-9, atol=1e-9)
+    assert np.any(g_model[:, 2] != 0)
+    g_scal2, g_model2 = plan.adjoint(w)
+    assert np.array_equal(g_model2.cpu().numpy(), g_model)
+    assert np.array_equal(g_scal2.cpu().numpy(), g_scal)
     plan.close()
 
 
@@ -299,23 +306,32 @@ def test_autograd_score_and_gradient(ctx):
     wn = w.cpu().numpy().astype(np.float64)
     gs = scal.grad.cpu().numpy()
     gm = model.grad.cpu().numpy()
-    ref_mu = np.zeros(4096)
-    ref_sg = np.zeros(4096)
+    ref = np.zeros((4096, 3))
     for i in range(b.n_reads):
         r = oracle.align(b.read_seq(i), b.read_events(i), b.model, float(b.scale[i]),
                          float(b.shift[i]))
         assert np.float32(r["score"]) == score[i].item()
         pairs, frm = read_path(res, i)
-        gr, g_mu, g_sg = oracle.path_gradient(pairs, frm, b.read_seq(i), b.read_events(i),
-                                              b.model, b.scale[i], b.shift[i], weight=wn[i])
+        gr, g_mu, g_sg, g_lam = oracle.path_gradient(pairs, frm, b.read_seq(i), b.read_events(i),
+                                                     b.model, b.scale[i], b.shift[i],
+                                                     weight=wn[i], partials=True)
         np.testing.assert_allclose(gs[i, 0], gr[1], rtol=1e-5, atol=1e-3)  # scale
         np.testing.assert_allclose(gs[i, 1], gr[0], rtol=1e-5, atol=1e-3)  # shift
         assert gs[i, 2] == 0 and gs[i, 3] == 0
-        ref_mu += g_mu
-        ref_sg += g_sg
-    np.testing.assert_allclose(gm[:, 0], ref_mu, rtol=1e-6, atol=1e-6)
-    np.testing.assert_allclose(gm[:, 1], ref_sg, rtol=1e-6, atol=1e-6)
-    assert np.all(gm[:, 2] == 0)
+        ref[:, 0] += g_mu
+        ref[:, 1] += g_sg
+        ref[:, 2] += g_lam
+    # model columns: the partial derivatives of the forward's function of
+    # (level_mean, level_stdv, level_log_stdv), f32 as the model tensor is
+    np.testing.assert_allclose(gm, ref, rtol=1e-6, atol=1e-6)
+    # tying level_log_stdv = log(level_stdv) in torch gives the total
+    # derivative (a^2 - 1)/sigma through the chain rule
+    sd = torch.tensor(b.model[:, 1].astype(np.float32), device="cuda", requires_grad=True)
+    tied = torch.stack([model.detach()[:, 0], sd, torch.log(sd)], 1)
+    score_t = abea_score(plan, scal.detach(), tied)
+    (score_t * w).sum().backward()
+    tot = ref[:, 1] + ref[:, 2] / b.model[:, 1].astype(np.float64)
+    np.testing.assert_allclose(sd.grad.cpu().numpy(), tot, rtol=1e-5, atol=1e-5)
     # a second forward on the plan invalidates the first one's backward
     s1 = abea_score(plan, scal, model)
     plan.forward()

@@ -436,3 +436,74 @@ def test_qc_rules_against_path_statistics():
         fired["span"].add(not spanned)
     assert fired["avg"] == {True, False} and fired["gap"] == {True, False}, fired
     assert fired["span"] == {False}
+
+
+# ------------------------------------------------------ batch driver, fills
+def test_batch_driver_equals_per_read():
+    """The OpenMP batch driver (CPU baseline and bulk parity reference) gives
+    each read exactly what the per-read oracle gives, in both modes."""
+    from abea_amd import synth
+    b = synth.workload("cpu64x2kb", seed=17, n=6)
+    for mode in (oracle.MODE_F5C, oracle.MODE_FP64):
+        ob = oracle.align_batch(b, n_threads=3, mode=mode, want_path=True, want_pairs=True)
+        for i in range(b.n_reads):
+            r = run_oracle_on(b, i, mode=mode)
+            off, n = int(ob["pair_ptr"][i]), int(ob["path_len"][i])
+            assert n == r["path_len"] and ob["n_pairs"][i] == r["n_pairs"]
+            assert np.array_equal(ob["path"][off:off + n], r["path"])
+            assert np.array_equal(ob["from"][off:off + n], r["from"])
+            assert np.array_equal(ob["pairs"][off:off + r["n_pairs"]], r["pairs"])
+            assert ob["score"][i] == r["score"] and ob["avg"][i] == r["avg_log_emission"]
+            assert ob["fills"][i] == r["fills"]
+
+
+def test_fills_from_moves():
+    """Cells computed per read, recomputed from packed band moves, equal the
+    fill's own count; when the band covers the whole matrix (n_e + n_k <=
+    49) every matrix cell is filled exactly once: fills = n_e * n_k."""
+    from abea_amd import engine, synth
+    b = synth.workload("cpu64x2kb", seed=19, n=5)
+    rng = np.random.default_rng(19)
+    seqs = [b.read_seq(i) for i in range(b.n_reads)]
+    evs = [b.read_events(i) for i in range(b.n_reads)]
+    for n_k, n_e in [(20, 25), (1, 40), (30, 3), (44, 5)]:
+        seqs.append(rng.choice(np.frombuffer(b"ACGT", np.uint8), n_k + 5).tobytes())
+        evs.append(rng.uniform(60, 130, n_e).astype(np.float32))
+    res = [oracle.align(s, e, b.model, 1.0, 0.0) for s, e in zip(seqs, evs)]
+    rl = np.array([len(s) for s in seqs], np.int32)
+    ne = np.array([len(e) for e in evs], np.int32)
+    words = engine.pack_moves([r["moves"] for r in res], rl, ne)
+    f = oracle.fills_from_moves(ne, rl - 5, words, engine.move_offsets(rl, ne))
+    assert np.array_equal(f, [r["fills"] for r in res])
+    for i in range(b.n_reads, len(seqs)):
+        assert f[i] == int(ne[i]) * int(rl[i] - 5), i
+
+
+def test_gradient_partials_vs_fd():
+    """The partial derivatives in level_stdv (level_log_stdv held) and in
+    level_log_stdv (level_stdv held) against central differences of the FP64
+    score of the held path; their combination is the total derivative when
+    level_log_stdv = log(level_stdv)."""
+    from abea_amd import synth
+    b = synth.workload("cpu64x2kb", seed=9, n=2)
+    seq, ev, sc, sh, base, g, top = _grad_case(b, 0)
+    g_read, g_mu, g_sg, g_lam = oracle.path_gradient(base["path"], base["from"], seq, ev, b.model,
+                                                     sc, sh, partials=True)
+    assert np.array_equal(g_read, g[0]) and np.array_equal(g_mu, g[1])
+    np.testing.assert_allclose(g_sg + g_lam / b.model[:, 1].astype(np.float64), g[2],
+                               rtol=1e-12, atol=1e-12)
+    for r in top:
+        n_cells = sum(1 for (k, _), f in zip(base["path"], base["from"])
+                      if f != 2 and oracle.kmer_rank(seq[k:k + 6]) == r)
+        assert g_lam[r] == -n_cells
+        for col, an in ((1, g_sg[r]), (2, g_lam[r])):
+            fs = []
+            steps = []
+            for sign in (1, -1):
+                m = b.model.copy()
+                v = np.float32(m[r, col] + sign * 1e-3)
+                steps.append(float(v) - float(m[r, col]))
+                m[r, col] = v
+                fs.append(oracle.path_score(base["path"], base["from"], seq, ev, m, sc, sh))
+            fd = (fs[0] - fs[1]) / (steps[0] - steps[1])
+            assert abs(fd - an) <= 1e-6 * abs(an) + 1e-6, (r, col, fd, an)
