@@ -76,6 +76,7 @@ class PostalignOutputT(C.Structure):
 FAILED_CALIBRATION = 0x001
 FAILED_ALIGNMENT = 0x002
 FAILED_QUALITY_CHK = 0x004
+REFINE_MOM_START = 1
 
 
 class TuningT(C.Structure):
@@ -157,7 +158,7 @@ def lib():
     L.abea_plan_refine_step.restype = C.c_int
     L.abea_plan_refine_step.argtypes = [P, C.POINTER(OutputT), P, C.c_float]
     L.abea_plan_refine.restype = C.c_int
-    L.abea_plan_refine.argtypes = [P, C.POINTER(OutputT), C.c_int, C.c_float,
+    L.abea_plan_refine.argtypes = [P, C.POINTER(OutputT), C.c_int, C.c_float, C.c_int,
                                    C.POINTER(C.c_double)]
     L.abea_estimate_scalings_using_mom.restype = ScalingsT
     L.abea_estimate_scalings_using_mom.argtypes = [C.c_char_p, C.c_int32, P, EventTable]

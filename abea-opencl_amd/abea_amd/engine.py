@@ -223,15 +223,19 @@ class Plan:
         self.ctx.leave()
         return dst
 
-    def refine(self, steps: int, damping: float = 1.0) -> float:
+    def refine(self, steps: int, damping: float = 1.0, mom_start: bool = True) -> float:
         """steps x (forward + refine step), graph-replayed, scalings updated
-        in place; returns ms per step."""
+        in place; with mom_start each read first continues from the better
+        scoring of its given and its method-of-moments scalings (one more
+        forward).  Returns the loop's device ms per step."""
         ms = C.c_double()
         self.ctx.enter()
         _lib.check(_lib.lib().abea_plan_refine(self.handle, C.byref(self.out), int(steps),
-                                               float(damping), C.byref(ms)), "abea_plan_refine")
+                                               float(damping),
+                                               _lib.REFINE_MOM_START if mom_start else 0,
+                                               C.byref(ms)), "abea_plan_refine")
         self.ctx.leave()
-        self.generation += steps
+        self.generation += steps + (1 if mom_start else 0)
         return ms.value
 
     def postalign(self, scale_var: bool = True, records: bool = True, base_map: bool = True,

@@ -76,8 +76,10 @@ struct DevBuf {
   }
 };
 
-// Per-read transition constants exactly as the CPU reference computes them
-// (FPGA/code/device/align_swi.cl:233-250): double p_stay, glibc log/exp.
+}  // namespace
+
+namespace abea {
+
 void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_step) {
   double events_per_kmer = (double)n_e / n_k;
   double p_stay = 1 - (1 / (events_per_kmer + 1));
@@ -86,7 +88,11 @@ void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_
   *lp_step = log(1.0 - exp(lp_skip) - exp(*lp_stay));
 }
 
-}  // namespace
+void set_last_error(const char *msg) { g_err = msg; }
+
+}  // namespace abea
+
+using abea::transition_constants;
 
 // Event-detection scratch of a context (grows, never shrinks).
 struct GeScratch {
@@ -121,6 +127,8 @@ struct abea_plan {
   DevBuf meta, queue, counter, km, km_rank, ev, ends, trace, moves, kck, replay, seg;
   // adjoint: fixed-point k-mer gradient bins (allocated by the first adjoint)
   DevBuf gacc, gspecial;
+  // refinement loop: start scalings, per-read start state, step counter
+  DevBuf rstart, rstate, rstep;
   // postalign scratch, allocated by the first postalign call: per-k-mer
   // recalibration terms, per-read walk status (zeroed once, then kept zero by
   // the finish kernel) and the walk's work items
@@ -658,7 +666,8 @@ int64_t abea_plan_workspace_bytes(const abea_plan_t *p) {
   if (!p) return 0;
   return (int64_t)(p->meta.n + p->queue.n + p->counter.n + p->km.n + p->km_rank.n + p->ev.n +
                    p->ends.n + p->trace.n + p->moves.n + p->kck.n + p->replay.n + p->seg.n +
-                   p->terms.n + p->pstat.n + p->items.n + p->gacc.n + p->gspecial.n);
+                   p->terms.n + p->pstat.n + p->items.n + p->gacc.n + p->gspecial.n +
+                   p->rstart.n + p->rstate.n + p->rstep.n);
 }
 
 int64_t abea_plan_move_words(const abea_plan_t *p) { return p ? p->move_words : 0; }
@@ -722,7 +731,8 @@ abea_status_t abea_plan_refine_step(abea_plan_t *p, const abea_output_t *fwd,
   if (p->n == 0) return ABEA_OK;
   HIP_TRY(hipSetDevice(p->ctx->device));
   const KernelArgs a = make_args(p, fwd);
-  HIP_TRY(abea::launch_refine(a, scalings_out, damping, p->ctx->stream));
+  HIP_TRY(abea::launch_refine(a, scalings_out, damping, nullptr, nullptr, nullptr,
+                              p->ctx->stream));
   return ABEA_OK;
 }
 
@@ -813,11 +823,14 @@ abea_status_t abea_plan_estimate_scalings(abea_plan_t *p, abea_scalings_t *out) 
 }
 
 abea_status_t abea_plan_refine(abea_plan_t *p, const abea_output_t *out, int steps, float damping,
-                               double *ms_per_step) {
+                               int flags, double *ms_per_step) {
   if (!p || !out) return fail(ABEA_ERR_INVALID, "refine: null argument");
   if (!out->pairs || !out->path_from || !out->path_len || !out->n_pairs)
     return fail(ABEA_ERR_INVALID, "refine: out must record pairs, path_from, path_len, n_pairs");
   if (steps < 0) return fail(ABEA_ERR_INVALID, "refine: negative step count");
+  if (flags & ~ABEA_REFINE_MOM_START) return fail(ABEA_ERR_INVALID, "refine: unknown flags %d", flags);
+  const bool mom = (flags & ABEA_REFINE_MOM_START) != 0;
+  if (mom && !out->score) return fail(ABEA_ERR_INVALID, "refine: the start selection needs score");
   if (ms_per_step) *ms_per_step = 0.0;
   if (p->n == 0 || steps == 0) return ABEA_OK;
   HIP_TRY(hipSetDevice(p->ctx->device));
@@ -826,12 +839,33 @@ abea_status_t abea_plan_refine(abea_plan_t *p, const abea_output_t *out, int ste
   hipStream_t s = p->ctx->stream;
   // the batch's own scalings are updated in place; the next forward reads them
   abea_scalings_t *scal = const_cast<abea_scalings_t *>(p->batch.scalings);
+  if (mom) {
+    if ((st = p->rstart.ensure(sizeof(abea_scalings_t) * (size_t)p->n)) ||
+        (st = p->rstate.ensure(sizeof(abea::RefineState) * (size_t)p->n)) ||
+        (st = p->rstep.ensure(sizeof(int32_t))))
+      return st;
+  }
   const KernelArgs a = make_args(p, out);
+  hipError_t e = hipEventRecord(p->evt[4], s);
+  if (mom) {
+    // the alternative start, and the step counter at 0 (a kernel, not a
+    // memset, ahead of the captured steps)
+    if (e == hipSuccess && (st = abea_plan_estimate_scalings(p, p->rstart.as<abea_scalings_t>())))
+      return st;
+    if (e == hipSuccess) e = abea::launch_refine_step_counter(p->rstep.as<int32_t>(), 0, s);
+  }
+  if (e != hipSuccess) return fail(ABEA_ERR_HIP, "refine: %s", hipGetErrorString(e));
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   abea_status_t cst = enqueue_forward(p, a, false);
-  hipError_t le = cst ? hipSuccess : abea::launch_refine(a, scal, damping, s);
+  hipError_t le = hipSuccess;
+  if (!cst)
+    le = abea::launch_refine(a, scal, damping, mom ? p->rstep.as<int32_t>() : nullptr,
+                             mom ? p->rstart.as<abea_scalings_t>() : nullptr,
+                             mom ? p->rstate.as<abea::RefineState>() : nullptr, s);
+  if (!cst && le == hipSuccess && mom)
+    le = abea::launch_refine_step_counter(p->rstep.as<int32_t>(), -1, s);
   hipError_t ce = hipStreamEndCapture(s, &g);
   if (cst || le != hipSuccess || ce != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
@@ -843,8 +877,8 @@ abea_status_t abea_plan_refine(abea_plan_t *p, const abea_output_t *out, int ste
   hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ie != hipSuccess) return fail(ABEA_ERR_HIP, "refine: hipGraphInstantiate failed: %s", hipGetErrorString(ie));
-  hipError_t e = hipEventRecord(p->evt[4], s);
-  for (int i = 0; i < steps && e == hipSuccess; ++i) e = hipGraphLaunch(ge, s);
+  const int launches = steps + (mom ? 1 : 0);
+  for (int i = 0; i < launches && e == hipSuccess; ++i) e = hipGraphLaunch(ge, s);
   if (e == hipSuccess) e = hipEventRecord(p->evt[3], s);
   if (e == hipSuccess) e = hipEventSynchronize(p->evt[3]);
   float ms = 0;

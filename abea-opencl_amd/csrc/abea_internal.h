@@ -188,6 +188,12 @@ struct GeArgs {
 };
 hipError_t launch_getevents(const GeArgs &a, hipStream_t s);
 
+// Per-read transition constants exactly as the CPU reference computes them
+// (FPGA/code/device/align_swi.cl:233-250): double p_stay, glibc log/exp (host).
+void transition_constants(int32_t n_e, int32_t n_k, double *lp_stay, double *lp_step);
+// Sets the calling thread's abea_last_error() message.
+void set_last_error(const char *msg);
+
 // flag bit for a read whose pairs are not an alignment path (kernel guard)
 constexpr int32_t kBadPath = 0x100;
 
@@ -209,8 +215,21 @@ hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overla
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
                           double *grad_model, unsigned long long *acc, double *special,
                           hipStream_t s);
+// Per-read state of the refinement loop's start selection.
+struct RefineState {
+  abea_scalings_t given;  // the scalings the loop started from
+  float2 given_step;      // their Newton step's (scale, shift)
+  float given_score;      // their best-path score
+  int32_t pad;
+};
+static_assert(sizeof(RefineState) == 32, "RefineState must stay 32 bytes");
+// step: device counter selecting the start-selection phase (null: a plain
+// Newton step); start: the alternative start (method-of-moments scalings).
 hipError_t launch_refine(const KernelArgs &a, abea_scalings_t *scal_out, float damping,
+                         const int32_t *step, const abea_scalings_t *start, RefineState *state,
                          hipStream_t s);
+// set_to >= 0 stores it, otherwise increments the counter.
+hipError_t launch_refine_step_counter(int32_t *step, int32_t set_to, hipStream_t s);
 int core_max_waves_per_block();
 
 }  // namespace abea

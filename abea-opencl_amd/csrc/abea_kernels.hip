@@ -1448,11 +1448,24 @@ __global__ void abea_adjoint_finish_kernel(const unsigned long long *acc, const 
 // score is a quadratic in (scale, shift) and one Newton step reaches its
 // maximum: solve [Sum mu^2/s^2, Sum mu/s^2; Sum mu/s^2, Sum 1/s^2] d = g with
 // g the path gradient (the adjoint's per-read terms).  Sums in double.
+//
+// With a start state (state non-null) the loop's first two steps also pick
+// each read's start: step 0 follows the forward at the given scalings,
+// keeps them, their score and their Newton step, and proposes the
+// method-of-moments scalings instead; step 1 follows the forward at those
+// and keeps whichever of the two scored higher, proceeding with its Newton
+// step.  A read whose alignment is lost under its given scalings (its path
+// wanders through the trim column and skips) carries no information about
+// them along that path, so Newton steps from there do not recover it.
 __global__ __launch_bounds__(256) void abea_refine_kernel(KernelArgs a, abea_scalings_t *scal_out,
-                                                         float damping, int n_reads) {
+                                                         float damping, int n_reads,
+                                                         const int32_t *step,
+                                                         const abea_scalings_t *start,
+                                                         RefineState *state) {
   const int lane = threadIdx.x & 63;
   const int wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int n_waves = (gridDim.x * blockDim.x) >> 6;
+  const int mode = step ? min(uniform(*(volatile const int32_t *)step), 2) : 2;
   for (int r = wave; r < n_reads; r += n_waves) {
     const ReadMeta m = load_meta(a.meta, r);
     const int P = (m.n_k > 0 && m.n_e > 0) ? a.out.path_len[r] : 0;
@@ -1496,9 +1509,38 @@ __global__ __launch_bounds__(256) void abea_refine_kernel(KernelArgs a, abea_sca
         o.scale = (float)(scale + (double)damping * d_scale);
         o.shift = (float)(shift + (double)damping * d_shift);
       }
+      if (mode == 0) {
+        const float score = P > 0 ? a.out.score[r] : ABEA_NEG_INF;
+        RefineState st;
+        st.given = sc;
+        st.given_step = make_float2(o.scale, o.shift);
+        st.given_score = score;
+        st.pad = 0;
+        state[r] = st;
+        const abea_scalings_t ms = start[r];
+        if (__builtin_isfinite(ms.scale) && __builtin_isfinite(ms.shift)) {
+          o = sc;
+          o.scale = ms.scale;
+          o.shift = ms.shift;
+        }
+      } else if (mode == 1) {
+        const RefineState st = state[r];
+        const float score = P > 0 ? a.out.score[r] : ABEA_NEG_INF;
+        if (!(score > st.given_score)) {
+          o = st.given;
+          o.scale = st.given_step.x;
+          o.shift = st.given_step.y;
+        }
+      }
       scal_out[r] = o;
     }
   }
+}
+
+// Advances the refinement loop's step counter (one thread; inside the
+// captured step, after the refine kernel).
+__global__ void abea_refine_advance_kernel(int32_t *step, int32_t set_to) {
+  if (threadIdx.x == 0) *step = set_to >= 0 ? set_to : *step + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1560,11 +1602,18 @@ hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *g
 }
 
 hipError_t launch_refine(const KernelArgs &a, abea_scalings_t *scal_out, float damping,
+                         const int32_t *step, const abea_scalings_t *start, RefineState *state,
                          hipStream_t s) {
   const int n = a.n_queue;
   if (n <= 0) return hipSuccess;
   const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
-  hipLaunchKernelGGL(abea_refine_kernel, dim3(grid), dim3(256), 0, s, a, scal_out, damping, n);
+  hipLaunchKernelGGL(abea_refine_kernel, dim3(grid), dim3(256), 0, s, a, scal_out, damping, n,
+                     step, start, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine_step_counter(int32_t *step, int32_t set_to, hipStream_t s) {
+  hipLaunchKernelGGL(abea_refine_advance_kernel, dim3(1), dim3(64), 0, s, step, set_to);
   return hipGetLastError();
 }
 

@@ -496,7 +496,7 @@ def refine_leg(args, ctx) -> dict:
     s0 = d4.scalings.cpu().numpy()
     steps = 5
     t = time.perf_counter()
-    ms = p4.refine(steps, 1.0)
+    ms = p4.refine(steps, 1.0, mom_start=True)
     wall = (time.perf_counter() - t) / steps
     s1 = d4.scalings.cpu().numpy()
 
@@ -505,9 +505,12 @@ def refine_leg(args, ctx) -> dict:
         return {"mean": round(float(d.mean()), 5), "median": round(float(np.median(d)), 5),
                 "p90": round(float(np.quantile(d, 0.9)), 5)}
 
-    res = {"workload": "refine8192x20kb", "steps": steps, "tuning": p4.tuning(),
+    res = {"workload": "refine8192x20kb", "steps": steps, "forwards": steps + 1,
+           "start": "per read, the better scoring of the given and the MOM scalings",
+           "tuning": p4.tuning(),
            "ms_per_step": round(ms, 3), "wall_ms_per_step": round(wall * 1e3, 3),
-           "gcups_per_step": round(b4.nominal_cells() / (ms * 1e-3) / 1e9, 2),
+           "gcups_per_forward": round(b4.nominal_cells() * (steps + 1) / steps
+                                      / (ms * 1e-3) / 1e9, 2),
            "shift_abs_err_before": err(s0, 1, true_shift),
            "shift_abs_err_after": err(s1, 1, true_shift),
            "scale_abs_err_before": err(s0, 0, true_scale),
@@ -738,7 +741,8 @@ def main():
                          "1 GiB device copy to calibrate HBM byte counters")
     ap.add_argument("--eager", action="store_true",
                     help="time eager launches instead of the captured graph")
-    ap.add_argument("--leg", choices=["postalign", "getevents", "dropin", "plain"], default=None,
+    ap.add_argument("--leg", choices=["postalign", "getevents", "dropin", "plain", "refine"],
+                    default=None,
                     help="run only this extra measurement (its JSON on stdout)")
     ap.add_argument("--tuning", default=None,
                     help="WPB,WPC,FUSE,OVERLAP: use this core launch shape instead of "
@@ -766,6 +770,9 @@ def main():
         return
     if args.leg == "plain":
         print(json.dumps(plain_leg(args, engine.Context(0), default_tune)), flush=True)
+        return
+    if args.leg == "refine":
+        print(json.dumps(refine_leg(args, engine.Context(0))), flush=True)
         return
 
     ctx = engine.Context(0)

@@ -353,11 +353,19 @@ abea_status_t abea_plan_autotune(abea_plan_t *plan, const abea_output_t *out, in
  * fwd.  Asynchronous on the context stream. */
 abea_status_t abea_plan_refine_step(abea_plan_t *plan, const abea_output_t *fwd,
                                     abea_scalings_t *scalings_out, float damping);
-/* steps x (forward + refine step) with the batch's scalings updated in
- * place, replayed as one captured graph per step; returns when done.
- * ms_per_step (optional) receives the average device time of one step. */
+/* The refinement loop: steps x (forward + refine step) with the batch's
+ * scalings updated in place, each step replayed as one captured graph;
+ * returns when done.  flags = ABEA_REFINE_MOM_START first picks each read's
+ * start: the loop runs one more forward, at the method-of-moments scalings
+ * (abea_plan_estimate_scalings), and each read continues from whichever of
+ * its given and its MOM scalings gave the higher best-path score (needs
+ * score in out).  A read whose alignment is lost under its given scalings
+ * has a path that says nothing about them, and Newton steps along it do not
+ * recover the read.  ms_per_step (optional) receives the loop's device time
+ * divided by steps (the start's forward and the MOM estimate included). */
+#define ABEA_REFINE_MOM_START 1
 abea_status_t abea_plan_refine(abea_plan_t *plan, const abea_output_t *out, int steps,
-                               float damping, double *ms_per_step);
+                               float damping, int flags, double *ms_per_step);
 
 /* Post-align outputs (device pointers).  Records of read i start at its pair
  * offset (abea_pair_offsets()); its base-to-event map entries at

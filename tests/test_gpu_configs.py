@@ -280,7 +280,7 @@ def test_refine_20kb_reads_vs_oracle(ctx):
     below the last event under such scalings)."""
     from abea_amd import engine, synth
     b = synth.workload("refine8192x20kb", seed=4, n=8)
-    synth.perturb_scalings(b, seed=4)
+    true_scale, true_shift = synth.perturb_scalings(b, seed=4)
     plan, res = _forward(ctx, b)
     assert _oracle_batch_mismatches(b, res) == []
     plan.refine(2, 1.0)
@@ -292,4 +292,7 @@ def test_refine_20kb_reads_vs_oracle(ctx):
     b2.shift = sc[:, 1].copy()
     assert not np.array_equal(b2.scale, b.scale)
     assert _oracle_batch_mismatches(b2, res2) == []
+    print(f"[refine] |shift error| {np.abs(b.shift - true_shift).round(2).tolist()} -> "
+          f"{np.abs(b2.shift - true_shift).round(2).tolist()}", flush=True)
+    assert np.all(np.abs(b2.shift - true_shift) < 1.5), (b2.shift, true_shift)
     plan.close()
