@@ -97,7 +97,7 @@ EXPORTS = [
     "abea_kmer_offsets", "abea_plan_postalign", "abea_plan_estimate_scalings",
     "abea_getevents", "abea_getevents_db", "abea_getevents_device",
     "abea_ctx_set_getevents_overlap", "abea_ctx_getevents_reruns", "abea_host_last_times",
-    "abea_host_contexts",
+    "abea_host_contexts", "abea_align_noalloc",
 ]
 
 _lib = None
@@ -119,6 +119,9 @@ def lib():
     L.abea_last_error.argtypes = []
     L.abea_align.restype = C.c_int32
     L.abea_align.argtypes = [P, C.c_char_p, C.c_int32, EventTable, P, ScalingsT, C.c_float]
+    L.abea_align_noalloc.restype = C.c_int32
+    L.abea_align_noalloc.argtypes = [P, C.c_char_p, C.c_int32, EventTable, P, ScalingsT,
+                                     C.c_float]
     L.abea_align_db.restype = C.c_int
     L.abea_align_db.argtypes = [C.c_int32, P, P, P, P, P, P, P]
     L.abea_ctx_create.restype = C.c_int

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libabea.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("abea_kernels.hip", "abea_postalign.hip", "abea_getevents.hip",
+           ("abea_kernels.hip", "abea_postalign.hip", "abea_getevents.hip", "abea_noalloc.hip",
             "abea_api.cpp")]
 HEADERS = [os.path.join(ROOT, "include", "abea.h"), os.path.join(HERE, "csrc", "abea_internal.h"),
            os.path.join(HERE, "csrc", "abea_wave.h")]

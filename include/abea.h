@@ -112,6 +112,21 @@ int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_
                    abea_event_table events, abea_model_t *models, abea_scalings_t scaling,
                    float sample_rate);
 
+/* (1') abea_align() for callers that cannot allocate device memory: a
+ * serving loop or harness that forbids hipMalloc after start-up.  No
+ * counterpart in the reference.  Same arguments, results and return values
+ * as abea_align(), but it allocates nothing: every device buffer the read
+ * needs comes from one static 64 MiB device array that each GPU the library
+ * runs on holds until the process ends, and it runs on the null stream of
+ * the current device.  A read needs about 90 bytes per band (n_events +
+ * n_kmers + 2 bands), so reads up to about 700 k bands (a 250 kb read at
+ * 1.6 events per base) fit; a larger read returns -1 with
+ * abea_last_error() saying so.  Calls on one device share its array and
+ * run one at a time (a mutex). */
+int32_t abea_align_noalloc(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_len,
+                           abea_event_table events, abea_model_t *models,
+                           abea_scalings_t scaling, float sample_rate);
+
 /* (2) f5c align_cuda(core, db): a batch in host buffers.
  * GPU/f5cmisc.h:99-102 / align_ocl GPU/align_cuda.c:391-943.
  * Reads read[i] (read_len[i] bases), et[i], scalings[i] and model[4096];
