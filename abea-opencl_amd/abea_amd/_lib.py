@@ -97,7 +97,8 @@ EXPORTS = [
     "abea_kmer_offsets", "abea_plan_postalign", "abea_plan_estimate_scalings",
     "abea_getevents", "abea_getevents_db", "abea_getevents_device",
     "abea_ctx_set_getevents_overlap", "abea_ctx_getevents_reruns", "abea_host_last_times",
-    "abea_host_contexts", "abea_align_noalloc",
+    "abea_host_contexts", "abea_align_noalloc", "abea_ctx_set_getevents_chunk",
+    "abea_ctx_getevents_chunk",
 ]
 
 _lib = None
@@ -191,6 +192,10 @@ def lib():
     L.abea_ctx_set_getevents_overlap.argtypes = [P, C.c_int32]
     L.abea_ctx_getevents_reruns.restype = C.c_int64
     L.abea_ctx_getevents_reruns.argtypes = [P]
+    L.abea_ctx_set_getevents_chunk.restype = C.c_int
+    L.abea_ctx_set_getevents_chunk.argtypes = [P, C.c_int32]
+    L.abea_ctx_getevents_chunk.restype = C.c_int32
+    L.abea_ctx_getevents_chunk.argtypes = [P]
     L.abea_host_last_times.restype = None
     L.abea_host_last_times.argtypes = [C.POINTER(C.c_double)]
     L.abea_host_contexts.restype = C.c_int32

@@ -110,6 +110,8 @@ struct abea_ctx {
   int n_cu = 256;
   std::string arch;
   int32_t ge_overlap = abea::kGeOverlap;
+  int32_t ge_chunk = 0;  // detector chunk samples; 0: chosen per call
+  int32_t ge_last_chunk = abea::kGeDetChunk;
   GeScratch ge;
 };
 
@@ -222,6 +224,34 @@ void *abea_ctx_stream(abea_ctx_t *c) { return c ? (void *)c->stream : nullptr; }
 
 namespace {
 
+// Detector chunk size for a call.  The detector kernel is bound by VALU issue
+// and runs four 4-wave blocks per CU, one lane per chunk, each lane walking
+// its chunk plus the lead-in; the kernel takes about (rounds of resident
+// blocks) x (chunk + lead-in) sample steps.  The chunk is chosen to fill the
+// last round, among multiples of 32 from kGeDetChunkMin to kGeDetChunkMax
+// (ties to the larger chunk: fewer chunks to check and re-run).
+int32_t pick_det_chunk(int32_t n, const int64_t *raw_off_h, int n_cu, int32_t overlap) {
+  const int64_t slots = (int64_t)std::max(n_cu, 1) * 4;  // resident 256-lane blocks
+  int32_t best = abea::kGeDetChunk;
+  double best_cost = 1e300;
+  for (int32_t D = abea::kGeDetChunkMin; D <= abea::kGeDetChunkMax; D += 32) {
+    int64_t chunks = 0, longest = 0;
+    for (int32_t r = 0; r < n; ++r) {
+      const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
+      chunks += (len + D - 1) / D;
+      longest = std::max(longest, len);
+    }
+    const int64_t blocks = (chunks + 255) / 256;
+    const int64_t rounds = std::max<int64_t>((blocks + slots - 1) / slots, 1);
+    const double cost = (double)rounds * (double)(std::min<int64_t>(D, longest) + overlap);
+    if (cost <= best_cost) {
+      best_cost = cost;
+      best = D;
+    }
+  }
+  return best;
+}
+
 // getevents over n signals in device memory (see abea_getevents_device).
 abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, const float *raw,
                          abea_event_t *events, int64_t *n_events) {
@@ -230,6 +260,15 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
   if (n == 0) return ABEA_OK;
   if (raw_off_h[0] < 0) return fail(ABEA_ERR_INVALID, "getevents: negative raw offset");
   GeScratch &g = ctx->ge;
+  for (int32_t r = 0; r < n; ++r) {
+    const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
+    if (len < 0 || len >= ((int64_t)1 << 31))
+      return fail(ABEA_ERR_INVALID, "getevents: read %d has %lld samples (need 0 .. 2^31-1)", r,
+                  (long long)len);
+  }
+  const int32_t D = ctx->ge_chunk > 0 ? ctx->ge_chunk
+                                      : pick_det_chunk(n, raw_off_h, ctx->n_cu, ctx->ge_overlap);
+  ctx->ge_last_chunk = D;
   g.raw_off_h.assign(raw_off_h, raw_off_h + n + 1);
   g.chunk_off_h.assign((size_t)n + 1, 0);
   g.det_off_h.assign((size_t)n + 1, 0);
@@ -237,11 +276,8 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
   g.det_read_h.clear();
   for (int32_t r = 0; r < n; ++r) {
     const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
-    if (len < 0 || len >= ((int64_t)1 << 31))
-      return fail(ABEA_ERR_INVALID, "getevents: read %d has %lld samples (need 0 .. 2^31-1)", r,
-                  (long long)len);
     const int64_t nch = (len + abea::kGeChunk - 1) / abea::kGeChunk;
-    const int64_t ndc = (len + abea::kGeDetChunk - 1) / abea::kGeDetChunk;
+    const int64_t ndc = (len + D - 1) / D;
     g.chunk_off_h[r + 1] = g.chunk_off_h[r] + nch;
     g.det_off_h[r + 1] = g.det_off_h[r] + ndc;
     g.chunk_read_h.insert(g.chunk_read_h.end(), (size_t)nch, r);
@@ -265,7 +301,7 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
       (st = g.csum.ensure(sizeof(abea::GeChunkSum) * nc1)) ||
       (st = g.carry.ensure(2 * sizeof(double) * nc1)) ||
       (st = g.exact.ensure(sizeof(int32_t) * (size_t)n)) ||
-      (st = g.peaks.ensure(sizeof(int32_t) * nd1 * abea::kGeDetChunk)) ||
+      (st = g.peaks.ensure(sizeof(int32_t) * nd1 * (size_t)D)) ||
       (st = g.n_peaks.ensure(sizeof(int32_t) * nd1)) ||
       (st = g.entry.ensure(sizeof(abea::GeState) * nd1)) ||
       (st = g.exit.ensure(sizeof(abea::GeState) * nd1)) ||
@@ -287,6 +323,7 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
   abea::GeArgs a{};
   a.n_reads = n;
   a.overlap = ctx->ge_overlap;
+  a.det_chunk = D;
   a.n_chunks = nc;
   a.n_det = nd;
   a.raw_off = g.raw_off.as<int64_t>();
@@ -334,6 +371,17 @@ abea_status_t abea_ctx_set_getevents_overlap(abea_ctx_t *ctx, int32_t samples) {
   ctx->ge_overlap = samples;
   return ABEA_OK;
 }
+
+abea_status_t abea_ctx_set_getevents_chunk(abea_ctx_t *ctx, int32_t samples) {
+  if (!ctx) return fail(ABEA_ERR_INVALID, "abea_ctx_set_getevents_chunk: null context");
+  if (samples != 0 && (samples < abea::kGeDetChunkMin || samples > abea::kGeDetChunkMax))
+    return fail(ABEA_ERR_INVALID, "getevents chunk %d: 0 or %d..%d", samples,
+                abea::kGeDetChunkMin, abea::kGeDetChunkMax);
+  ctx->ge_chunk = samples;
+  return ABEA_OK;
+}
+
+int32_t abea_ctx_getevents_chunk(abea_ctx_t *ctx) { return ctx ? ctx->ge_last_chunk : 0; }
 
 int64_t abea_ctx_getevents_reruns(abea_ctx_t *ctx) {
   if (!ctx || ctx->ge.n_reads <= 0) return 0;

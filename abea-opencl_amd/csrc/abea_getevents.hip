@@ -486,21 +486,22 @@ __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_
 }
 
 // One lane per detector chunk: a fresh run from a.overlap samples before the
-// chunk.  At most 2 * (ceil(512 / 3) + 1) = 344 boundaries: a detector that
-// emits is back to no peak and needs 3 more samples to emit again.
+// chunk.  A chunk of D samples holds at most 2 * (ceil(D / 3) + 1) <= D
+// boundaries (D >= 8): a detector that emits is back to no peak and needs 3
+// more samples to emit again.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void abea_ge_detect_kernel(GeArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_det) return;
   const int r = a.det_read[c];
   const int64_t off = a.raw_off[r];
   const int32_t n = (int32_t)(a.raw_off[r + 1] - off);
-  const int32_t c0 = (int32_t)(c - a.det_off[r]) * kGeDetChunk;
-  const int32_t c1 = min(c0 + kGeDetChunk, n);
+  const int32_t c0 = (int32_t)(c - a.det_off[r]) * a.det_chunk;
+  const int32_t c1 = min(c0 + a.det_chunk, n);
   const int32_t from = max(c0 - a.overlap, 0);
   GeState st = fresh_state();
   const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
                   a.exact[r] == 2, a.raw_off[a.n_reads] - off};
-  run_detector(st, rd, from, c1, c0, a.peaks + c * kGeDetChunk, a.n_peaks + c, a.entry + c);
+  run_detector(st, rd, from, c1, c0, a.peaks + c * a.det_chunk, a.n_peaks + c, a.entry + c);
   a.exit[c] = st;
 }
 
@@ -511,7 +512,7 @@ __global__ __launch_bounds__(256) void abea_ge_check_kernel(GeArgs a) {
   const int r = a.det_read[c];
   const int32_t lc = (int32_t)(c - a.det_off[r]);
   if (lc == 0) return;  // the first chunk starts from the true initial state
-  if (!same_state(a.entry[c], a.exit[c - 1], lc * kGeDetChunk)) atomicMin(&a.first_bad[r], lc);
+  if (!same_state(a.entry[c], a.exit[c - 1], lc * a.det_chunk)) atomicMin(&a.first_bad[r], lc);
 }
 
 // One lane per read.  The chunks before the first mismatch ran from the true
@@ -530,12 +531,12 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
     const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
                     a.exact[r] == 2, a.raw_off[a.n_reads] - off};
     for (int64_t c = cb + fb; c < a.det_off[r + 1]; ++c) {
-      const int32_t c0 = (int32_t)(c - cb) * kGeDetChunk;
+      const int32_t c0 = (int32_t)(c - cb) * a.det_chunk;
       if (same_state(a.entry[c], st, c0)) {
         st = a.exit[c];
       } else {
         ++reruns;
-        run_detector(st, rd, c0, min(c0 + kGeDetChunk, n), c0, a.peaks + c * kGeDetChunk,
+        run_detector(st, rd, c0, min(c0 + a.det_chunk, n), c0, a.peaks + c * a.det_chunk,
                      a.n_peaks + c, nullptr);
       }
     }
@@ -544,11 +545,13 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
 }
 
 // One wave per read (4 per block): where each detector chunk's events go.
-// Chunk c's boundaries end events ev_base[c] .. ev_base[c] + n_peaks[c] - 1
-// of the read (an exclusive scan of the counts); the event that ends at its
-// first boundary starts at prev_last[c], the last boundary in the chunks
-// before it (an exclusive maximum scan: boundaries increase along the read;
-// 0 where there is none).  Also the event count.
+// Chunk c's boundaries are the read's boundaries ev_base[c] .. ev_base[c] +
+// n_peaks[c] - 1 (an exclusive scan of the counts), and boundary g ends event
+// g; the event that ends at its first boundary starts at prev_last[c], the
+// last boundary in the chunks before it (an exclusive maximum scan:
+// boundaries increase along the read; 0 where there is none).  Also the
+// event count: m boundaries give max(m, 1) events, scrappie's create_events
+// count (the last boundary ends no event; the last event runs to the end).
 __global__ __launch_bounds__(256) void abea_ge_scan_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -559,7 +562,7 @@ __global__ __launch_bounds__(256) void abea_ge_scan_kernel(GeArgs a) {
   for (int64_t c0 = cb; c0 < ce; c0 += 64) {
     const int64_t c = c0 + lane;
     const int32_t cnt = c < ce ? a.n_peaks[c] : 0;
-    const int32_t lst = cnt > 0 ? a.peaks[c * kGeDetChunk + cnt - 1] : 0;
+    const int32_t lst = cnt > 0 ? a.peaks[(int64_t)c * a.det_chunk + cnt - 1] : 0;
     const int incl = wave_scan_add(cnt);
     int32_t mx = lst;  // inclusive maximum scan
 #pragma unroll
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(256) void abea_ge_scan_kernel(GeArgs a) {
     base += __shfl(incl, 63, 64);
     last = max(last, __shfl(mx, 63, 64));
   }
-  if (lane == 0) a.n_events[r] = n > 0 ? base + 1 : 0;
+  if (lane == 0) a.n_events[r] = n > 0 ? max(base, 1) : 0;
 }
 
 // --------------------------------------------------------------------------
@@ -648,10 +651,12 @@ __device__ __forceinline__ abea_event_t make_event_exact(const GeArgs &a, const 
   return e;
 }
 
-// One wave per detector chunk (4 per block): the events that end at its
-// boundaries, and for a read's last chunk also the event after the read's
-// last boundary (to the end of the signal).  Event j runs from boundary
-// j - 1 (0 for the first) to boundary j (n for the last).
+// One wave per detector chunk (4 per block): the events its boundaries end.
+// Event j runs from boundary j - 1 (0 for the first) to boundary j, except
+// the read's last event, which runs from its second-last boundary to the end
+// of the signal: its last boundary ends no event (scrappie's create_events,
+// whose event count is the boundary count).  A read with no boundary gets
+// one event, written by its last chunk.
 __global__ __launch_bounds__(256) void abea_ge_events_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -660,8 +665,10 @@ __global__ __launch_bounds__(256) void abea_ge_events_kernel(GeArgs a) {
   const int64_t off = a.raw_off[r];
   const int64_t n = a.raw_off[r + 1] - off;
   const int32_t cnt = a.n_peaks[c];
-  const int32_t m = cnt + (c == a.det_off[r + 1] - 1 ? 1 : 0);
-  const int32_t *pk = a.peaks + c * kGeDetChunk;
+  const int64_t lc = a.det_off[r + 1] - 1;  // the read's last chunk
+  const int64_t nb = (int64_t)a.ev_base[lc] + a.n_peaks[lc];  // the read's boundaries
+  const int32_t m = (nb == 0 && c == lc) ? 1 : cnt;
+  const int32_t *pk = a.peaks + c * a.det_chunk;
   const int64_t jb = a.ev_base[c];
   const int32_t pl = a.prev_last[c];
   abea_event_t *ev = a.events + off + r;
@@ -671,7 +678,7 @@ __global__ __launch_bounds__(256) void abea_ge_events_kernel(GeArgs a) {
   const float *x = a.raw + off;
   for (int32_t k = lane; k < m; k += 64) {
     const int64_t start = k == 0 ? pl : pk[k - 1];
-    const int64_t end = k < cnt ? pk[k] : n;
+    const int64_t end = jb + k + 1 < nb ? pk[k] : n;
     ev[jb + k] = exact ? make_event_exact(a, x, cbase, start, end) : make_event(start, end, S, Q);
   }
 }

@@ -137,9 +137,12 @@ struct MomArgs {
 
 // Event detection (getevents), one read per raw signal.  Samples are cut
 // into kGeChunk-sample chunks for the prefix sums and the events, and into
-// kGeDetChunk-sample chunks for the peak detector (with the t-statistics).
+// detector chunks of GeArgs::det_chunk samples for the peak detector (with
+// the t-statistics), chosen per call between kGeDetChunkMin and
+// kGeDetChunkMax (kGeDetChunk when set by hand).
 constexpr int kGeChunk = 2048;
 constexpr int kGeDetChunk = 512;
+constexpr int kGeDetChunkMin = 256, kGeDetChunkMax = 4096;
 constexpr int kGeOverlap = 64;  // samples a detector chunk's speculative run starts early
 struct GeDet {
   int32_t pos;        // -1: no peak yet
@@ -162,6 +165,8 @@ struct GeChunkSum {
 struct GeArgs {
   int32_t n_reads;
   int32_t overlap;
+  int32_t det_chunk;          // samples per detector chunk
+  int32_t pad_;
   int64_t n_chunks;           // prefix chunks
   int64_t n_det;              // detector chunks
   const int64_t *raw_off;     // n_reads + 1: read r is raw[raw_off[r] .. raw_off[r+1])
@@ -175,7 +180,7 @@ struct GeArgs {
                               // every sample is below 2^60 in magnitude (tame)
   const int64_t *det_off;     // n_reads + 1: read r's detector chunks
   const int32_t *det_read;    // the read of each detector chunk
-  int32_t *peaks;             // kGeDetChunk boundaries per detector chunk, in emission order
+  int32_t *peaks;             // det_chunk boundaries per detector chunk, in emission order
   int32_t *n_peaks;           // per detector chunk
   GeState *entry, *exit;      // per detector chunk: state before its first / after its last sample
   int32_t *ev_base;           // per detector chunk: the read's event ending at its first boundary

@@ -216,11 +216,15 @@ abea_status_t abea_postalign_db(int32_t n_bam_rec, char **read, const int32_t *r
  * signal in picoamperes (f5c's rawptr), by scrappie's DNA detector as f5c
  * ports it (the event types are scrappie's, GPU/f5c.h:112-130): two windowed
  * t-statistics (w = 3 and 6) over double prefix sums, the short/long peak
- * detector (thresholds 1.4 and 9.0, peak height 0.2), one event between
- * consecutive boundaries.  Returns the event table with a malloc()ed event
- * array that the caller frees, start = 0 and end = n, as getevents does; a
- * signal with no boundary is one event (nsample = 0: no events).  On error:
- * n = 0, event = NULL, abea_last_error() set.  nsample < 2^31. */
+ * detector (thresholds 1.4 and 9.0, peak height 0.2), and events counted
+ * as scrappie's create_events counts them: m boundaries give m events, from
+ * 0 to the first boundary, between consecutive boundaries, and the last from
+ * the second-last boundary to the end (the last boundary ends no event).
+ * Returns the event table with a malloc()ed event array that the caller
+ * frees, start = 0 and end = n, as getevents does; a signal with fewer than
+ * two boundaries is one event (the C code reads before its boundary array
+ * there; nsample = 0: no events).  On error: n = 0, event = NULL,
+ * abea_last_error() set.  nsample < 2^31. */
 abea_event_table abea_getevents(size_t nsample, float *rawptr);
 
 /* getevents over a batch in host buffers (signal i: rawptr[i], nsample[i]);
@@ -251,12 +255,18 @@ void *abea_ctx_stream(abea_ctx_t *ctx);
 abea_status_t abea_getevents_device(abea_ctx_t *ctx, int32_t n_reads,
                                     const int64_t *raw_off_host, const float *raw,
                                     abea_event_t *events, int64_t *n_events);
-/* Samples each 512-sample detector chunk's speculative run starts before
- * the chunk (default 64, 0..512).  Results do not depend on it, only how many
+/* Samples each detector chunk's speculative run starts before the chunk
+ * (default 64, 0..512).  Results do not depend on it, only how many
  * chunks are re-run from the true state; abea_ctx_getevents_reruns() gives
  * that count for the last abea_getevents_device() call (synchronizes). */
 abea_status_t abea_ctx_set_getevents_overlap(abea_ctx_t *ctx, int32_t samples);
 int64_t abea_ctx_getevents_reruns(abea_ctx_t *ctx);
+/* Samples per detector chunk (one lane walks each chunk): 0 (the default)
+ * picks it per call, 256..4096, so that the detector's lanes fill whole
+ * rounds of the GPU's resident waves; otherwise 256..4096 fixed.  Results do
+ * not depend on it.  abea_ctx_getevents_chunk() gives the last call's. */
+abea_status_t abea_ctx_set_getevents_chunk(abea_ctx_t *ctx, int32_t samples);
+int32_t abea_ctx_getevents_chunk(abea_ctx_t *ctx);
 
 typedef struct {
   int32_t n_reads;

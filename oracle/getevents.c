@@ -31,14 +31,18 @@
  *     the current value.  While the short detector holds a peak above its
  *     threshold, the long detector is masked to w_short samples past that
  *     peak and restarted.
- *  4. Events between consecutive boundaries 0, b_1, ..., b_m, n: start,
- *     length, mean and stdv from the prefix sums, in float.
+ *  4. Events from the boundaries b_0 < ... < b_{m-1} as scrappie's
+ *     create_events counts them: nevent = 1 + #{i >= 1 : 0 < b_i < n}, so m
+ *     boundaries give m events: [0, b_0), [b_0, b_1), ..., [b_{m-3}, b_{m-2})
+ *     and the last one [b_{m-2}, n).  The last boundary ends no event.
+ *     Start, length, mean and stdv from the prefix sums, in float.
  *
  * The mixed float/double arithmetic follows the C source's types, noted at
- * each step.  With no boundary at all, that code reads before its boundary
- * array; here the one event then covers the whole signal.  The reference's
- * fixtures hold no raw signal, so parity against f5c is unpinned;
- * tests/test_getevents_oracle.py pins this restatement by closed forms.
+ * each step.  With fewer than two boundaries that code reads before its
+ * boundary array (b_{-1}); here the one event then covers the whole signal,
+ * as b_{-1} = 0 would give.  The reference's fixtures hold no raw signal, so
+ * parity against f5c is unpinned; tests/test_getevents_oracle.py pins this
+ * restatement by closed forms.
  */
 #include <float.h>
 #include <math.h>
@@ -189,14 +193,18 @@ int64_t orc_getevents(const float *raw, int64_t n, orc_event_t *out, int64_t cap
     orc_tstat(S, Q, n, W_SHORT, t1);
     orc_tstat(S, Q, n, W_LONG, t2);
     const int64_t np = orc_detect_peaks(t1, t2, n, peaks);
+#ifdef ORC_MUTATE_GE_KEEP_LAST_BOUNDARY /* planted mistake: every boundary ends an event */
     m = np + 1;
+#else
+    m = np > 1 ? np : 1;
+#endif
     if (m <= cap) {
       int64_t prev = 0;
-      for (int64_t j = 0; j < np; ++j) {
+      for (int64_t j = 0; j + 1 < m; ++j) {
         out[j] = make_event(prev, peaks[j], S, Q);
         prev = peaks[j];
       }
-      out[np] = make_event(prev, n, S, Q);
+      out[m - 1] = make_event(prev, n, S, Q);
     } else {
       m = -m; /* capacity too small: minus the event count */
     }

@@ -5,8 +5,10 @@ reference declares it and its event types (GPU/f5c.h:112-130) and holds no
 raw signal, so there is nothing of the reference's to compare against
 (parity with f5c unpinned).  These checks pin the restatement instead:
 - closed forms: a piecewise-constant signal is cut exactly at its steps,
-  with the event means its exact sums give; with noise every step is still
-  a cut;
+  with the event means its exact sums give, except that the detector's
+  last boundary ends no event (scrappie's create_events counts one event
+  per boundary, so the last event runs from the second-last boundary to
+  the end); with noise every step is still a boundary;
 - the windowed t-statistic against a direct float64 computation over the
   two windows (no prefix sums), and its zero edges;
 - each event's mean and deviation against its own samples;
@@ -49,22 +51,41 @@ def step_signal(rng, n_seg, lo=7, hi=40, noise=0.0):
     return x, bounds.astype(np.int64), lengths, np.array(levels, np.float32)
 
 
+def expected_events(lengths, levels):
+    """Events of a clean step signal: one per segment, the last two segments
+    merged (the detector's last boundary, the last step, ends no event).
+    Returns (starts, lengths, means): a segment's double sum L*v is exact,
+    and an event mean is its sum rounded to float, then divided by its
+    length in float."""
+    sums = lengths * levels.astype(np.float64)
+    lens = np.concatenate([lengths[:-2], [lengths[-2] + lengths[-1]]]) if len(lengths) > 1 \
+        else lengths.copy()
+    tot = np.concatenate([sums[:-2], [sums[-2] + sums[-1]]]) if len(lengths) > 1 else sums
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    means = (np.float32(tot) / lens.astype(np.float32)).astype(np.float32)
+    return starts, lens, means
+
+
 def test_step_signals_cut_exactly_at_steps():
     rng = np.random.default_rng(11)
     for _ in range(20):
         x, bounds, lengths, levels = step_signal(rng, int(rng.integers(2, 60)))
+        S, Q = oracle.signal_prefix(x)
+        assert oracle.detect_peaks(oracle.tstat(S, Q, 3), oracle.tstat(S, Q, 6)).tolist() == \
+            bounds.tolist()
         ev = oracle.getevents(x)
-        assert ev.shape[0] == lengths.shape[0]
-        assert np.array_equal(ev["start"][1:].astype(np.int64), bounds)
-        assert ev["start"][0] == 0
-        assert np.array_equal(ev["length"], lengths.astype(np.float32))
-        # a segment's double sum L*v is exact; the event mean is that sum
-        # rounded to float, then divided by L in float
-        L = lengths.astype(np.float32)
-        want = (np.float32(lengths * levels.astype(np.float64)) / L).astype(np.float32)
-        assert np.array_equal(ev["mean"], want)
-        # the variance is a difference of rounded terms: at most a few ulps of v^2
-        assert np.all(ev["stdv"] <= np.sqrt(8 * np.spacing(levels * levels)))
+        starts, lens, means = expected_events(lengths, levels)
+        assert ev.shape[0] == lengths.shape[0] - 1
+        assert np.array_equal(ev["start"].astype(np.int64), starts)
+        assert np.array_equal(ev["length"], lens.astype(np.float32))
+        assert np.array_equal(ev["mean"], means)
+        # the variance is a difference of rounded terms: at most a few ulps
+        # of v^2 on one level; the merged last event holds two levels
+        assert np.all(ev["stdv"][:-1] <= np.sqrt(8 * np.spacing(levels[:-2] * levels[:-2])))
+        a, b = levels[-2].astype(np.float64), levels[-1].astype(np.float64)
+        L1, L2 = float(lengths[-2]), float(lengths[-1])
+        sd = abs(a - b) * np.sqrt(L1 * L2) / (L1 + L2)
+        assert abs(float(ev["stdv"][-1]) - sd) <= 1e-3 * sd + 1e-2
 
 
 def test_noisy_steps_are_cut_and_stats_match_samples():
@@ -74,10 +95,13 @@ def test_noisy_steps_are_cut_and_stats_match_samples():
     rng = np.random.default_rng(12)
     for _ in range(10):
         x, bounds, lengths, _ = step_signal(rng, int(rng.integers(5, 80)), noise=0.05)
+        S, Q = oracle.signal_prefix(x)
+        peaks = oracle.detect_peaks(oracle.tstat(S, Q, 3), oracle.tstat(S, Q, 6))
+        assert set(bounds.tolist()) <= set(peaks.tolist())
         ev = oracle.getevents(x)
         starts = ev["start"].astype(np.int64)
+        assert np.array_equal(starts, np.concatenate([[0], peaks[:-1]]))
         assert np.all(np.diff(starts) > 0)
-        assert set(bounds.tolist()) <= set(starts.tolist())
         assert int(ev["start"][-1] + ev["length"][-1]) == x.shape[0]
         for e in ev:
             seg = x[int(e["start"]):int(e["start"]) + int(e["length"])].astype(np.float64)
@@ -202,8 +226,7 @@ def _with_lib(so, fn):
 
 
 def _step_means(lengths, levels):
-    return (np.float32(lengths * levels.astype(np.float64)) / lengths.astype(np.float32)).astype(
-        np.float32)
+    return expected_events(lengths, levels)[2]
 
 
 def _long_step_signal():
@@ -213,7 +236,8 @@ def _long_step_signal():
 
 
 @pytest.mark.parametrize("mutant", ["ORC_MUTATE_GE_FLOAT_PREFIX", "ORC_MUTATE_GE_NO_MASK",
-                                    "ORC_MUTATE_GE_EMIT_EARLY"])
+                                    "ORC_MUTATE_GE_EMIT_EARLY",
+                                    "ORC_MUTATE_GE_KEEP_LAST_BOUNDARY"])
 def test_getevents_checks_catch_planted_mistakes(mutant):
     so = _mutant(mutant)
 
@@ -225,7 +249,7 @@ def test_getevents_checks_catch_planted_mistakes(mutant):
             failures += 1
         x, bounds, lengths, levels = _long_step_signal()
         ev = oracle.getevents(x)
-        if not (ev.shape[0] == lengths.shape[0] and np.array_equal(ev["mean"], _step_means(
+        if not (ev.shape[0] == lengths.shape[0] - 1 and np.array_equal(ev["mean"], _step_means(
                 lengths, levels))):
             failures += 1
         return failures

@@ -106,6 +106,25 @@ def test_whole_reads_and_reruns(ctx):
             assert reruns > 0  # the re-run path was exercised
 
 
+def test_detector_chunk_sizes_agree(ctx):
+    """The detector chunk size (chosen per call unless fixed) changes only
+    which lane walks which samples: every size gives the oracle's events."""
+    b = synth.workload("plain4096_ln5kb", seed=6, n=12)
+    raw, off = synth.raw_signal(b, seed=6)
+    sigs = [raw[off[i]:off[i + 1]] for i in range(b.n_reads)] + _signals([700, 3], seed=6)
+    want = [oracle.getevents(s) for s in sigs]
+    L = _lib.lib()
+    for chunk in (256, 512, 1000, 4096, 0):
+        _lib.check(L.abea_ctx_set_getevents_chunk(ctx.handle, chunk), "chunk")
+        got = _run_device(ctx, sigs)
+        used = L.abea_ctx_getevents_chunk(ctx.handle)
+        print(f"[getevents] chunk {chunk}: used {used}", flush=True)
+        assert used == chunk or chunk == 0
+        for i, (g, w) in enumerate(zip(got, want)):
+            assert _events_equal(g, w), f"chunk {chunk} read {i}: {_first_diff(g, w)}"
+    assert L.abea_ctx_set_getevents_chunk(ctx.handle, 100) != 0
+
+
 def test_serial_prefix_path(ctx):
     # terms as fine as 2^-60 cannot be summed exactly in double over a pA
     # signal: the prefix sums take the serial path
@@ -150,8 +169,32 @@ def test_step_signals_and_offset_reads(ctx):
     got = _run_device(ctx, sigs, lead=5)
     for s, g in zip(sigs, got):
         assert _events_equal(g, oracle.getevents(s))
+    # every step is a boundary; the last boundary ends no event
     for c, g in zip(cuts, [got[0], got[2], got[3]]):
-        assert np.array_equal(g["start"][1:].astype(np.int64), c)
+        assert np.array_equal(g["start"][1:].astype(np.int64), c[:-1])
+        assert g.shape[0] == max(len(c), 1)
+
+
+def test_mixed_kinds_in_one_call(ctx):
+    """Reads of every kind next to each other in one call: a tame read, then
+    one scaled by 2^54 (IEEE divisions in its statistics), then one with
+    1e-18 samples (serial prefix sums), then a tame one.  Near a read's end a
+    detector lane loads the next read's samples, which must enter no
+    statistic of its own read; each read must equal the oracle on its own."""
+    base = _signals([4100, 3000, 2600, 5000], seed=21)
+    sigs = [base[0], (base[1] * np.float32(2.0 ** 54)).astype(np.float32), base[2].copy(),
+            base[3]]
+    sigs[2][::89] = np.float32(1e-18)
+    got = _run_device(ctx, sigs)
+    for i, (s, g) in enumerate(zip(sigs, got)):
+        want = oracle.getevents(s)
+        assert _events_equal(g, want), f"read {i}: {_first_diff(g, want)}"
+    # and in the other order, the fine and the large reads first
+    rev = sigs[::-1]
+    got = _run_device(ctx, rev)
+    for i, (s, g) in enumerate(zip(rev, got)):
+        want = oracle.getevents(s)
+        assert _events_equal(g, want), f"reversed read {i}: {_first_diff(g, want)}"
 
 
 def test_host_dropins(ctx):
