@@ -40,12 +40,16 @@ abea_status_t fail(abea_status_t st, const char *fmt, ...) {
   return st;
 }
 
+// A failed call is reported once: HIP's last-error state is cleared, so a
+// later launch's hipGetLastError() does not report it again.
 #define HIP_TRY(call)                                                                  \
   do {                                                                                 \
     hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
+    if (e_ != hipSuccess) {                                                            \
+      (void)hipGetLastError();                                                         \
       return fail(ABEA_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
                   __FILE__, __LINE__);                                                 \
+    }                                                                                  \
   } while (0)
 
 struct DevBuf {

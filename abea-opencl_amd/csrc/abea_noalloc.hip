@@ -114,8 +114,10 @@ int32_t scratch_base(int dev, unsigned char **base) {
 #define NA_TRY(call)                                                                      \
   do {                                                                                    \
     const hipError_t e_ = (call);                                                         \
-    if (e_ != hipSuccess)                                                                 \
+    if (e_ != hipSuccess) {                                                               \
+      (void)hipGetLastError();                                                            \
       return noalloc_fail("abea_align_noalloc: %s failed: %s", #call, hipGetErrorString(e_)); \
+    }                                                                                     \
   } while (0)
 
 }  // namespace

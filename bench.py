@@ -46,8 +46,8 @@ W = 100
 # 1 GiB copy's readings) and VALU wave-instructions (SQ_INSTS_VALU).  Keyed by
 # (kernel, waves per CU, traceback overlap).
 PROFILED = {
-    ("abea_core_kernel<1, true>", 24, 128): {"traffic": 49442908774, "valu": 2.086e10},
-    ("abea_core_kernel<1, true>", 32, 128): {"traffic": 49442908774, "valu": 2.086e10},
+    ("abea_core_kernel<1, true>", 24, 128): {"traffic": 49144735264, "valu": 2.0855e10},
+    ("abea_core_kernel<1, true>", 32, 128): {"traffic": 49049624960, "valu": 2.0855e10},
 }
 N_SIMD = 1024          # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # peak engine clock
@@ -147,12 +147,26 @@ def headline_leg(args, ctx):
         plan.forward()
     torch.cuda.synchronize()
     kt = np.zeros(3)
+    kt_source = "hipEvents inside the timed steps"
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.forward()
-        kt += np.array(plan.times()[:3])
+        if kt_source is not None:
+            try:
+                kt += np.array(plan.times()[:3])
+            except Exception:  # noqa: BLE001
+                # event records inside a graph are refused under some
+                # profilers: time the kernels on eager steps below
+                kt_source = None
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
+    if kt_source is None:
+        plan.set_tuning(use_graph=0)
+        kt[:] = 0
+        for _ in range(args.steps):
+            plan.forward()
+            kt += np.array(plan.times()[:3])
+        kt_source = "hipEvents of eager steps (graph event timing unavailable)"
     kt /= args.steps
     gcups = cells / sec / 1e9
     log(f"[bench] batched {'eager' if args.eager else 'graph'}: {sec * 1e3:.2f} ms/step, "
@@ -184,7 +198,7 @@ def headline_leg(args, ctx):
                    "nominal_cells": cells, "bandwidth": W, "tuning": tune},
         "reads_per_s": round(b.n_reads / sec, 1),
         "kernel_ms": {"queue_reset": round(kt[0], 3), "core": round(kt[1], 3),
-                      "post": round(kt[2], 3), "source": "hipEvents inside the timed steps"},
+                      "post": round(kt[2], 3), "source": kt_source},
         "qc_pass_reads": qc_pass,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
