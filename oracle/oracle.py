@@ -17,7 +17,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
+# ORC_LIB names another build of the same sources (the sanitizer build that
+# the CPU tests run the oracle's checks under)
+LIB_PATH = os.environ.get("ORC_LIB") or os.path.join(HERE, "_build", "liborc.so")
 
 MODE_F5C = 0
 MODE_FP64 = 1
