@@ -150,7 +150,6 @@ struct abea_plan {
   abea_output_t gout{};
   abea_tuning_t gtune{};
   bool greplay = false;
-  bool graph_timing_ok = true;
   hipEvent_t evt[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double times[4] = {0, 0, 0, 0};
   bool have_times = false;
@@ -653,19 +652,18 @@ abea_status_t abea_plan_forward(abea_plan_t *p, const abea_output_t *out) {
         (void)hipGraphExecDestroy(p->gexec);
         p->gexec = nullptr;
       }
+      // The graph holds the kernels only.  Events recorded inside a graph
+      // are timed unreliably (elapsed times a percent or so above the
+      // stream's own, and refused outright when other events are recorded
+      // on the stream between replays), so a replay is timed as a whole by
+      // events around its launch.
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      abea_status_t cst = enqueue_forward(p, a, p->graph_timing_ok);
+      abea_status_t cst = enqueue_forward(p, a, false);
       hipError_t ce = hipStreamEndCapture(s, &g);
       if (cst || ce != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();
-        if (p->graph_timing_ok) {
-          // event records inside a captured graph were refused: capture
-          // without them and time the graph as a whole
-          p->graph_timing_ok = false;
-          return abea_plan_forward(p, out);
-        }
         return cst ? cst : fail(ABEA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
       }
       hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
@@ -678,9 +676,9 @@ abea_status_t abea_plan_forward(abea_plan_t *p, const abea_output_t *out) {
       p->gtune = p->tune;
       p->greplay = p->replay_on;
     }
-    if (!p->graph_timing_ok) HIP_TRY(hipEventRecord(p->evt[0], s));
+    HIP_TRY(hipEventRecord(p->evt[0], s));
     HIP_TRY(hipGraphLaunch(p->gexec, s));
-    if (!p->graph_timing_ok) HIP_TRY(hipEventRecord(p->evt[3], s));
+    HIP_TRY(hipEventRecord(p->evt[3], s));
     p->have_times = true;
     return ABEA_OK;
   }
@@ -696,7 +694,7 @@ abea_status_t abea_plan_last_times(const abea_plan_t *pc, double times_ms[4]) {
   if (p->have_times && p->evt[0]) {
     HIP_TRY(hipEventSynchronize(p->evt[3]));
     float ms = 0;
-    const bool split = !(p->tune.use_graph && p->form == ABEA_PLAN_BATCHED && !p->graph_timing_ok);
+    const bool split = !(p->tune.use_graph && p->form == ABEA_PLAN_BATCHED);
     if (split && p->form == ABEA_PLAN_BATCHED) {
       HIP_TRY(hipEventElapsedTime(&ms, p->evt[0], p->evt[1]));
       times_ms[0] = ms;
@@ -705,7 +703,8 @@ abea_status_t abea_plan_last_times(const abea_plan_t *pc, double times_ms[4]) {
       HIP_TRY(hipEventElapsedTime(&ms, p->evt[2], p->evt[3]));
       times_ms[2] = ms;
     } else {
-      // whole forward only (plain form interleaves the kernels)
+      // whole forward only: the plain form interleaves the kernels, and a
+      // graph replay is timed as a whole
       HIP_TRY(hipEventElapsedTime(&ms, p->evt[0], p->evt[3]));
       times_ms[1] = ms;
     }

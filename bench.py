@@ -140,34 +140,27 @@ def headline_leg(args, ctx):
     tune = plan.tuning()
     log(f"[bench] tuning {tune}")
 
-    # timed steps: graph replays; each step's kernel times come from the
-    # hipEvents recorded inside that step's graph
+    # timed steps: graph replays, each timed on the plan's stream by hipEvents
+    # around its launch (the graph holds the one-wave queue reset and the
+    # core); eager steps (--eager) time each kernel
     plan.set_tuning(use_graph=0 if args.eager else 1)
     for _ in range(args.warmup):
         plan.forward()
     torch.cuda.synchronize()
     kt = np.zeros(3)
-    kt_source = "hipEvents inside the timed steps"
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.forward()
-        if kt_source is not None:
-            try:
-                kt += np.array(plan.times()[:3])
-            except Exception:  # noqa: BLE001
-                # event records inside a graph are refused under some
-                # profilers: time the kernels on eager steps below
-                kt_source = None
+        kt += np.array(plan.times()[:3])
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
-    if kt_source is None:
-        plan.set_tuning(use_graph=0)
-        kt[:] = 0
-        for _ in range(args.steps):
-            plan.forward()
-            kt += np.array(plan.times()[:3])
-        kt_source = "hipEvents of eager steps (graph event timing unavailable)"
     kt /= args.steps
+    if args.eager:
+        kt_source = "hipEvents between the kernels of each timed eager step"
+    else:
+        kt_source = ("hipEvents around each timed graph replay on the plan's stream: the core "
+                     "plus the one-wave queue reset before it")
+        kt[0] = kt[2] = -1.0
     gcups = cells / sec / 1e9
     log(f"[bench] batched {'eager' if args.eager else 'graph'}: {sec * 1e3:.2f} ms/step, "
         f"{gcups:.1f} GCUPS; kernel ms queue_reset/core/post {kt.round(3).tolist()}")
@@ -197,8 +190,9 @@ def headline_leg(args, ctx):
                    "bases": int(b.read_len.sum()), "events": int(b.n_events.sum()),
                    "nominal_cells": cells, "bandwidth": W, "tuning": tune},
         "reads_per_s": round(b.n_reads / sec, 1),
-        "kernel_ms": {"queue_reset": round(kt[0], 3), "core": round(kt[1], 3),
-                      "post": round(kt[2], 3), "source": kt_source},
+        "kernel_ms": {"queue_reset": round(kt[0], 3) if kt[0] >= 0 else None,
+                      "core": round(kt[1], 3),
+                      "post": round(kt[2], 3) if kt[2] >= 0 else None, "source": kt_source},
         "qc_pass_reads": qc_pass,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -477,6 +471,7 @@ def getevents_leg(args, ctx):
     ms = min(ts)
     n_ev = int(cnt.sum().item())
     reruns = int(_lib.lib().abea_ctx_getevents_reruns(ctx.handle))
+    det_chunk = int(_lib.lib().abea_ctx_getevents_chunk(ctx.handle))
     # algorithmic bytes: the signal read once (4 B per sample), the event
     # records written (24 B each)
     alg = 4 * n_samples + 24 * n_ev
@@ -484,7 +479,7 @@ def getevents_leg(args, ctx):
            "events": n_ev, "ms": round(ms, 3),
            "gsamples_per_s": round(n_samples / (ms * 1e-3) / 1e9, 3),
            "alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
-           "chunks_rerun": reruns}
+           "chunks_rerun": reruns, "detector_chunk": det_chunk}
     log(f"[bench] getevents: {res}")
     del raw, ev, cnt
     return res, (raw_h, raw_off)

@@ -422,7 +422,11 @@ abea_status_t abea_plan_estimate_scalings(abea_plan_t *plan, abea_scalings_t *ou
 
 /* Per-kernel device time (ms) of the last forward/adjoint, hipEvent-timed
  * on the plan's stream: [0]=queue reset (batched form; the core stages each
- * read itself) [1]=core [2]=post [3]=adjoint. */
+ * read itself) [1]=core [2]=post [3]=adjoint.  Where the kernels cannot be
+ * told apart, [1] holds the whole forward and [0], [2] are -1: the plain
+ * form (one launch pair per read) and graph replays of the batched form
+ * (timed by events around the launch; the queue reset is a one-wave kernel
+ * of a few microseconds). */
 abea_status_t abea_plan_last_times(const abea_plan_t *plan, double times_ms[4]);
 /* Bytes of device workspace held by the plan. */
 int64_t abea_plan_workspace_bytes(const abea_plan_t *plan);
