@@ -759,12 +759,14 @@ abea_status_t abea_plan_adjoint(abea_plan_t *p, const abea_output_t *fwd, const 
   if (st) return st;
   hipStream_t s = p->ctx->stream;
   const KernelArgs a = make_args(p, fwd);
-  if ((st = p->gacc.ensure(sizeof(unsigned long long) * 6 * ABEA_NUM_KMER)) ||
+  const int nb = abea::adjoint_blocks(p->n, p->ctx->n_cu);
+  if ((st = p->gacc.ensure(sizeof(unsigned long long) * 6 * ABEA_NUM_KMER * (size_t)nb)) ||
       (st = p->gspecial.ensure(sizeof(double) * 3 * ABEA_NUM_KMER)))
     return st;
   HIP_TRY(hipEventRecord(p->evt[4], s));
   HIP_TRY(abea::launch_adjoint(a, grad_score, grad_scalings, grad_model,
-                               p->gacc.as<unsigned long long>(), p->gspecial.as<double>(), s));
+                               p->gacc.as<unsigned long long>(), nb, p->gspecial.as<double>(),
+                               s));
   HIP_TRY(hipEventRecord(p->evt[3], s));
   HIP_TRY(hipEventSynchronize(p->evt[3]));
   float ms = 0;

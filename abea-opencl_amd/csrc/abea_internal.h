@@ -215,11 +215,14 @@ hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s);
 hipError_t launch_core(const KernelArgs &a, int q_begin, int q_count, bool persistent,
                        int waves_per_block, int grid_blocks, int fuse, int overlap, hipStream_t s);
 hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overlap, hipStream_t s);
-// grad_model: [4096][3] doubles (level_mean, level_stdv, level_log_stdv);
-// acc (4096 * 6 limbs) and special (4096 * 3) are the fixed-point bins.
+// grad_model: [4096][3] doubles (level_mean, level_stdv, level_log_stdv).
+// partial holds n_blocks slices of 2 * 4096 * 3 fixed-point limbs (one per
+// workgroup, n_blocks = adjoint_blocks()); special (4096 * 3 doubles) takes
+// the non-finite or huge terms.
+int adjoint_blocks(int n_reads, int n_cu);
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
-                          double *grad_model, unsigned long long *acc, double *special,
-                          hipStream_t s);
+                          double *grad_model, unsigned long long *partial, int n_blocks,
+                          double *special, hipStream_t s);
 // Per-read state of the refinement loop's start selection.
 struct RefineState {
   abea_scalings_t given;  // the scalings the loop started from

@@ -1321,123 +1321,174 @@ __global__ __launch_bounds__(64) void abea_post_kernel(KernelArgs a, int q_begin
 //
 // Per-read sums: each lane adds the cells of its own contiguous chunks in
 // path order, then a fixed butterfly, so the bits do not depend on timing.
+//
 // Per-k-mer sums: a lane adds a run of cells on one k-mer position in double,
 // then adds the run's weighted term to the k-mer's bins as a fixed-point
-// number (two 64-bit integer limbs, hi * 2^-12 + lo * 2^-44) with integer
-// atomics.  Integer addition is associative, so the total does not depend on
-// the order the atomics land in: two runs on the same inputs give the same
-// bits.  Each term is quantised once, with an error below 2^-45; the low
-// limb takes up to 2^31 terms per k-mer without overflow.  Terms that
-// are not finite, or of magnitude 2^40 or more (never for pA-scale inputs),
-// go to a double accumulator instead; a sum of non-finite values does not
+// number (two 64-bit integer limbs, hi * 2^-12 + lo * 2^-44).  Integer
+// addition is associative, so the total does not depend on the order the
+// adds land in: two runs on the same inputs give the same bits.  Each term is
+// quantised once, with an error below 2^-45; the low limb takes up to 2^31
+// terms per k-mer without overflow.  Terms that are not finite, or of
+// magnitude 2^40 or more (never for pA-scale inputs), go to a double
+// accumulator in global memory instead; a sum of non-finite values does not
 // depend on its order either.
-constexpr int kAdjChunk = 8;               // path cells per lane and step
-constexpr double kFixHi = 4096.0;          // 2^12
+//
+// The bins are privatised: each workgroup (4 to 16 waves, persistent over
+// reads; one per CU, since the bins fill most of a CU's LDS) keeps the bins of half the k-mer ranks in LDS (2048 ranks x 3 columns x
+// 2 limbs x 8 B = 96 KB) and walks its reads' paths once per half, adding
+// only the runs whose rank falls in that half with LDS atomics.  At the end
+// of each half it stores its bins to its own slice of a partial-sum buffer;
+// the finish kernel adds the workgroups' slices in block order.  The only
+// global atomics left are those of the rare double terms.  The paths are
+// short next to the work per cell, so walking them twice costs little.
+constexpr int kAdjChunk = 8;                // path cells per lane and step
+constexpr int kAdjMaxWaves = 16;            // waves per workgroup, at most
+constexpr int kAdjPassRanks = ABEA_NUM_KMER / 2;
+constexpr int kAdjPassBins = 3 * kAdjPassRanks;  // (rank, column) bins per pass
+constexpr int kAdjBins = 3 * ABEA_NUM_KMER;
+constexpr double kFixHi = 4096.0;           // 2^12
 constexpr double kFixLo = 17592186044416.0; // 2^44: rest = lo * 2^-44, lo in [0, 2^32]
 
-__device__ __forceinline__ void fix_add(unsigned long long *acc, double *special, double t) {
+// Adds t to bin i (hi limb at bins[i], lo limb at bins[kAdjPassBins + i]).
+__device__ __forceinline__ void fix_add_lds(unsigned long long *bins, double *special, int i,
+                                            int gi, double t) {
   if (t == 0.0) return;
   if (!(fabs(t) < 1099511627776.0)) {  // non-finite or |t| >= 2^40
-    atomicAdd(special, t);
+    atomicAdd(&special[gi], t);
     return;
   }
   const double f = floor(t * kFixHi);               // exact scaling, exact floor
   const double rest = t - f * (1.0 / kFixHi);       // in [0, 2^-12)
   const long long hi = (long long)f;
   const long long lo = (long long)rint(rest * kFixLo);  // in [0, 2^32]
-  atomicAdd(&acc[0], (unsigned long long)hi);
-  atomicAdd(&acc[1], (unsigned long long)lo);
+  atomicAdd(&bins[i], (unsigned long long)hi);
+  atomicAdd(&bins[kAdjPassBins + i], (unsigned long long)lo);
 }
 
-__global__ __launch_bounds__(256) void abea_adjoint_kernel(KernelArgs a, const float *grad_score,
-                                                          float *grad_scal,
-                                                          unsigned long long *acc,
-                                                          double *special, int n_reads) {
+__global__ __launch_bounds__(64 * kAdjMaxWaves) void abea_adjoint_kernel(
+    KernelArgs a, const float *grad_score, float *grad_scal, unsigned long long *partial,
+    double *special, int n_reads) {
+  __shared__ unsigned long long bins[2 * kAdjPassBins];
   const int lane = threadIdx.x & 63;
-  const int wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (int r = wave; r < n_reads; r += n_waves) {
-    const ReadMeta m = load_meta(a.meta, r);
-    const int P = (m.n_k > 0 && m.n_e > 0) ? a.out.path_len[r] : 0;
-    const double w = (double)grad_score[r];
-    const abea_scalings_t sc = a.scalings[r];
-    const double scale = (double)sc.scale, shift = (double)sc.shift;
-    const abea_aligned_pair_t *pairs = a.out.pairs + m.pair_off;
-    const uint8_t *from = a.out.path_from + m.pair_off;
-    const float *evp = a.ev + m.sev_off;
-    const uint16_t *kr = a.km_rank + m.km_off;
-    double gs = 0.0, gc = 0.0;
-    for (int base = 0; base < P; base += 64 * kAdjChunk) {
-      int cur_k = -1, cur_rank = 0;
-      double s_mu = 0.0, s_sg = 0.0, s_n = 0.0;
-      for (int j = 0; j < kAdjChunk; ++j) {
-        const int i = base + lane * kAdjChunk + j;
-        if (i >= P || from[i] == FROM_L) continue;
-        const abea_aligned_pair_t pr = pairs[i];
-        const int kk = min(max(pr.ref_pos, 0), m.n_k - 1);
-        const int ee = min(max(pr.read_pos, 0), m.n_e - 1);
-        if (kk != cur_k) {
-          if (cur_k >= 0) {
-            unsigned long long *b = acc + 6 * cur_rank;
-            double *sp = special + 3 * cur_rank;
-            fix_add(b + 0, sp + 0, w * s_mu);
-            fix_add(b + 2, sp + 1, w * s_sg);
-            fix_add(b + 4, sp + 2, -w * s_n);
+  const int wave = uniform(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int rank_lo = pass * kAdjPassRanks;
+    for (int i = threadIdx.x; i < 2 * kAdjPassBins; i += blockDim.x) bins[i] = 0ull;
+    __syncthreads();
+    for (int r = blockIdx.x * wpb + wave; r < n_reads; r += gridDim.x * wpb) {
+      const ReadMeta m = load_meta(a.meta, r);
+      const int P = (m.n_k > 0 && m.n_e > 0) ? a.out.path_len[r] : 0;
+      const double w = (double)grad_score[r];
+      const abea_scalings_t sc = a.scalings[r];
+      const double scale = (double)sc.scale, shift = (double)sc.shift;
+      const abea_aligned_pair_t *pairs = a.out.pairs + m.pair_off;
+      const uint8_t *from = a.out.path_from + m.pair_off;
+      const float *evp = a.ev + m.sev_off;
+      const uint16_t *kr = a.km_rank + m.km_off;
+      double gs = 0.0, gc = 0.0;
+      for (int base = 0; base < P; base += 64 * kAdjChunk) {
+        int cur_k = -1, cur_bin = -1;  // cur_bin < 0: the run's rank is outside this pass
+        double s_mu = 0.0, s_sg = 0.0, s_n = 0.0;
+        auto flush = [&]() {
+          if (cur_bin >= 0) {
+            const int gi = 3 * rank_lo + cur_bin;
+            fix_add_lds(bins, special, cur_bin + 0, gi + 0, w * s_mu);
+            fix_add_lds(bins, special, cur_bin + 1, gi + 1, w * s_sg);
+            fix_add_lds(bins, special, cur_bin + 2, gi + 2, -w * s_n);
           }
-          cur_k = kk;
-          cur_rank = kr[kk] & (ABEA_NUM_KMER - 1);
-          s_mu = s_sg = s_n = 0.0;
-        }
-        const abea_model_t md = a.model[cur_rank];
-        const double mu = (double)md.level_mean, sg = (double)md.level_stdv;
-        const double inv = 1.0 / sg;
-        const double z = ((double)evp[ee] - (scale * mu + shift)) * inv;
-        gs += z * inv;
-        gc += z * mu * inv;
-        s_mu += z * scale * inv;
-        s_sg += z * z * inv;
-        s_n += 1.0;
-      }
-      if (cur_k >= 0) {
-        unsigned long long *b = acc + 6 * cur_rank;
-        double *sp = special + 3 * cur_rank;
-        fix_add(b + 0, sp + 0, w * s_mu);
-        fix_add(b + 2, sp + 1, w * s_sg);
-        fix_add(b + 4, sp + 2, -w * s_n);
-      }
-    }
+        };
+        // the lane's cells: all loads first (pairs and codes, then ranks, then
+        // events and model rows), so that each stage's loads are in flight
+        // together; indices are clamped into the read's arrays
+        const int i0 = base + lane * kAdjChunk;
+        int kk[kAdjChunk], ee[kAdjChunk], rk[kAdjChunk];
+        bool use[kAdjChunk];
+        float xv[kAdjChunk];
+        abea_model_t md[kAdjChunk];
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      gs += __shfl_xor(gs, off);
-      gc += __shfl_xor(gc, off);
+        for (int j = 0; j < kAdjChunk; ++j) {
+          const int i = min(i0 + j, P - 1);
+          const abea_aligned_pair_t pr = pairs[i];
+          use[j] = i0 + j < P && from[i] != FROM_L;
+          kk[j] = min(max(pr.ref_pos, 0), m.n_k - 1);
+          ee[j] = min(max(pr.read_pos, 0), m.n_e - 1);
+        }
+#pragma unroll
+        for (int j = 0; j < kAdjChunk; ++j) rk[j] = kr[kk[j]] & (ABEA_NUM_KMER - 1);
+#pragma unroll
+        for (int j = 0; j < kAdjChunk; ++j) {
+          xv[j] = evp[ee[j]];
+          md[j] = a.model[rk[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < kAdjChunk; ++j) {
+          if (!use[j]) continue;
+          if (kk[j] != cur_k) {
+            flush();
+            cur_k = kk[j];
+            cur_bin = (unsigned)(rk[j] - rank_lo) < (unsigned)kAdjPassRanks ? 3 * (rk[j] - rank_lo)
+                                                                             : -1;
+            s_mu = s_sg = s_n = 0.0;
+          }
+          // the per-read terms need every cell, in the first pass only
+          if (cur_bin < 0 && pass != 0) continue;
+          const double mu = (double)md[j].level_mean, sg = (double)md[j].level_stdv;
+          const double inv = 1.0 / sg;
+          const double z = ((double)xv[j] - (scale * mu + shift)) * inv;
+          gs += z * inv;
+          gc += z * mu * inv;
+          s_mu += z * scale * inv;
+          s_sg += z * z * inv;
+          s_n += 1.0;
+        }
+        flush();
+      }
+      if (pass == 0) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          gs += __shfl_xor(gs, off);
+          gc += __shfl_xor(gc, off);
+        }
+        if (lane == 0) {
+          grad_scal[3 * r + 0] = (float)(w * gs);
+          grad_scal[3 * r + 1] = (float)(w * gc);
+          grad_scal[3 * r + 2] = 0.0f;
+        }
+      }
     }
-    if (lane == 0) {
-      grad_scal[3 * r + 0] = (float)(w * gs);
-      grad_scal[3 * r + 1] = (float)(w * gc);
-      grad_scal[3 * r + 2] = 0.0f;
+    __syncthreads();
+    // this workgroup's slice: hi limbs of all bins, then lo limbs
+    unsigned long long *dst = partial + (size_t)blockIdx.x * 2 * kAdjBins + 3 * rank_lo;
+    for (int i = threadIdx.x; i < kAdjPassBins; i += blockDim.x) {
+      dst[i] = bins[i];
+      dst[kAdjBins + i] = bins[kAdjPassBins + i];
     }
+    __syncthreads();  // before the next pass clears the bins
   }
 }
 
-// Clears the fixed-point bins before an adjoint (a fill kernel rather than a
-// memset, so the sequence can be captured in a graph).
-__global__ void abea_adjoint_clear_kernel(unsigned long long *acc, double *special) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * ABEA_NUM_KMER;
-       i += gridDim.x * blockDim.x) {
-    acc[2 * i] = 0ull;
-    acc[2 * i + 1] = 0ull;
+// Clears the double accumulator before an adjoint (a fill kernel rather than
+// a memset, so the sequence can be captured in a graph).  The partial-sum
+// slices need no clearing: every workgroup stores all of its bins.
+__global__ void abea_adjoint_clear_kernel(double *special) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kAdjBins; i += gridDim.x * blockDim.x)
     special[i] = 0.0;
-  }
 }
 
 // grad_model[3 * rank + column] = hi * 2^-12 + lo * 2^-44 (+ the non-finite
-// or huge terms, which then decide the value).
-__global__ void abea_adjoint_finish_kernel(const unsigned long long *acc, const double *special,
-                                           double *grad_model) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * ABEA_NUM_KMER;
-       i += gridDim.x * blockDim.x) {
-    const double hi = (double)(long long)acc[2 * i];
-    const double lo = (double)(long long)acc[2 * i + 1];
+// or huge terms, which then decide the value), the limbs summed over the
+// workgroups' slices in block order.
+__global__ void abea_adjoint_finish_kernel(const unsigned long long *partial, int n_blocks,
+                                           const double *special, double *grad_model) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kAdjBins; i += gridDim.x * blockDim.x) {
+    unsigned long long h = 0ull, l = 0ull;
+    for (int b = 0; b < n_blocks; ++b) {
+      h += partial[(size_t)b * 2 * kAdjBins + i];
+      l += partial[(size_t)b * 2 * kAdjBins + kAdjBins + i];
+    }
+    const double hi = (double)(long long)h;
+    const double lo = (double)(long long)l;
     grad_model[i] = (hi * (1.0 / kFixHi) + lo * (1.0 / kFixLo)) + special[i];
   }
 }
@@ -1586,18 +1637,23 @@ hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overla
   return hipGetLastError();
 }
 
+// One workgroup per CU when there are reads for every CU; its waves: as many
+// as the reads give each CU, between 4 and 16.
+int adjoint_blocks(int n_reads, int n_cu) {
+  const int want = (n_reads + 3) / 4;
+  return std::max(1, std::min(want, n_cu));
+}
+
 hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *grad_scalings,
-                          double *grad_model, unsigned long long *acc, double *special,
-                          hipStream_t s) {
+                          double *grad_model, unsigned long long *partial, int n_blocks,
+                          double *special, hipStream_t s) {
   const int n = a.n_queue;
-  hipLaunchKernelGGL(abea_adjoint_clear_kernel, dim3(48), dim3(256), 0, s, acc, special);
-  if (n > 0) {
-    const int grid = (n + 3) / 4 < 16384 ? (n + 3) / 4 : 16384;
-    hipLaunchKernelGGL(abea_adjoint_kernel, dim3(grid), dim3(256), 0, s, a, grad_score,
-                       grad_scalings, acc, special, n);
-  }
-  hipLaunchKernelGGL(abea_adjoint_finish_kernel, dim3(48), dim3(256), 0, s, acc, special,
-                     grad_model);
+  const int wpb = std::max(4, std::min(kAdjMaxWaves, (n + n_blocks - 1) / std::max(n_blocks, 1)));
+  hipLaunchKernelGGL(abea_adjoint_clear_kernel, dim3(48), dim3(256), 0, s, special);
+  hipLaunchKernelGGL(abea_adjoint_kernel, dim3(n_blocks), dim3(64 * wpb), 0, s, a, grad_score,
+                     grad_scalings, partial, special, n);
+  hipLaunchKernelGGL(abea_adjoint_finish_kernel, dim3(48), dim3(256), 0, s, partial, n_blocks,
+                     special, grad_model);
   return hipGetLastError();
 }
 
