@@ -18,6 +18,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "abea_internal.h"
@@ -51,6 +52,40 @@ abea_status_t fail(abea_status_t st, const char *fmt, ...) {
                   __FILE__, __LINE__);                                                 \
     }                                                                                  \
   } while (0)
+
+// Pinned host staging (hipHostMalloc), grown as needed and kept: the
+// host-buffer drop-ins flatten their inputs into it and copy results back
+// into it, so that both copies run at the link's rate.
+struct PinBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  ~PinBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  abea_status_t ensure(size_t bytes) {
+    if (bytes <= n && p) return ABEA_OK;
+    if (p) {
+      (void)hipHostFree(p);
+      p = nullptr;
+      n = 0;
+    }
+    // a quarter more than asked, so that slowly growing batches reallocate rarely
+    size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(ABEA_ERR_NOMEM, "hipHostMalloc(%zu bytes) failed: %s", want,
+                  hipGetErrorString(e));
+    }
+    n = want;
+    return ABEA_OK;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
 
 struct DevBuf {
   void *p = nullptr;
@@ -1048,6 +1083,10 @@ struct HostRun {
   abea_plan *plan = nullptr;
   DevBuf seq, seq_ptr, read_len, events, ev_ptr, n_events, scal, model, pairs, from, n_pairs,
       path_len, score;
+  // align()/align_cuda() drop-ins: pinned inputs (bases, event means) and
+  // outputs (pair counts, compacted pairs), and the compaction's offsets
+  PinBuf h_seq, h_mean, h_np, h_pairs;
+  DevBuf mean, coff, cpairs;
   // post-align and MOM entries
   DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms, pstat, items;
   // getevents entries
@@ -1130,62 +1169,118 @@ abea_status_t upload(DevBuf &d, const T *h, size_t count, hipStream_t s) {
   return ABEA_OK;
 }
 
-// Run a host-resident batch; pairs_h gets all pair slots (abea_pair_offsets layout).
-abea_status_t run_host_batch(int form, int32_t n, const std::vector<char> &seq,
-                             const std::vector<int64_t> &seq_ptr, const std::vector<int32_t> &read_len,
-                             const std::vector<abea_event_t> &events,
-                             const std::vector<int64_t> &ev_ptr, const std::vector<int32_t> &n_events,
+// Runs fn(b, e) over [0, n) in up to 16 threads, split into contiguous
+// ranges of about equal weight (wpre[i]: the weight of items before i;
+// wpre[n] the total).  Small batches, and any failure to start a thread,
+// run on the calling thread.
+template <class F>
+void parallel_ranges(int32_t n, const std::vector<int64_t> &wpre, F fn) {
+  const int64_t total = wpre[(size_t)n];
+  int T = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+  if (n < 2 || T < 2 || total < ((int64_t)1 << 22)) {
+    fn(0, n);
+    return;
+  }
+  T = std::min<int>(T, n);
+  std::vector<std::thread> th;
+  int32_t b = 0;
+  try {
+    for (int t = 0; t < T && b < n; ++t) {
+      int32_t e = n;
+      if (t + 1 < T) {
+        const int64_t target = total * (t + 1) / T;
+        e = (int32_t)(std::lower_bound(wpre.begin(), wpre.begin() + n, target) - wpre.begin());
+        e = std::max(e, b + 1);
+      }
+      th.emplace_back(fn, b, e);
+      b = e;
+    }
+  } catch (...) {
+  }
+  if (b < n) fn(b, n);  // what no thread took
+  for (auto &x : th) x.join();
+}
+
+// A host-resident batch, flattened into the context's pinned staging: bases
+// (h_seq, CSR by seq_ptr) and event means (h_mean, CSR by ev_ptr).
+struct HostBatch {
+  int32_t n = 0;
+  std::vector<int64_t> seq_ptr, ev_ptr;
+  std::vector<int32_t> read_len, n_events;
+  int64_t n_bases = 0, n_ev = 0;
+};
+
+// Runs a flattened batch.  On return the reads' pairs sit back to back in the
+// pinned r->h_pairs, read i's at coff[i], with np[i] of them.
+abea_status_t run_host_batch(HostRun *r, int form, const HostBatch &hb,
                              const abea_scalings_t *scal, const abea_model_t *model,
-                             std::vector<abea_aligned_pair_t> &pairs_h,
-                             std::vector<int32_t> &n_pairs_h, std::vector<int64_t> &pair_off) {
-  HostLease lease;
-  abea_status_t st = lease.acquire();
-  if (st) return st;
-  HostRun *r = lease.r;
+                             std::vector<int32_t> &np, std::vector<int64_t> &coff) {
+  const int32_t n = hb.n;
   HIP_TRY(hipSetDevice(r->ctx->device));
   hipStream_t s = r->ctx->stream;
+  abea_status_t st;
   HIP_TRY(hipEventRecord(r->ev[0], s));
-  if ((st = upload(r->seq, seq.data(), seq.size(), s))) return st;
-  if ((st = upload(r->seq_ptr, seq_ptr.data(), seq_ptr.size(), s))) return st;
-  if ((st = upload(r->read_len, read_len.data(), read_len.size(), s))) return st;
-  if ((st = upload(r->events, events.data(), events.size(), s))) return st;
-  if ((st = upload(r->ev_ptr, ev_ptr.data(), ev_ptr.size(), s))) return st;
-  if ((st = upload(r->n_events, n_events.data(), n_events.size(), s))) return st;
-  if ((st = upload(r->scal, scal, (size_t)n, s))) return st;
-  if ((st = upload(r->model, model, (size_t)ABEA_NUM_KMER, s))) return st;
+  if ((st = upload(r->seq, r->h_seq.as<char>(), (size_t)hb.n_bases, s)) ||
+      (st = upload(r->seq_ptr, hb.seq_ptr.data(), hb.seq_ptr.size(), s)) ||
+      (st = upload(r->read_len, hb.read_len.data(), hb.read_len.size(), s)) ||
+      (st = upload(r->mean, r->h_mean.as<float>(), (size_t)hb.n_ev, s)) ||
+      (st = upload(r->ev_ptr, hb.ev_ptr.data(), hb.ev_ptr.size(), s)) ||
+      (st = upload(r->n_events, hb.n_events.data(), hb.n_events.size(), s)) ||
+      (st = upload(r->scal, scal, (size_t)n, s)) ||
+      (st = upload(r->model, model, (size_t)ABEA_NUM_KMER, s)))
+    return st;
   HIP_TRY(hipEventRecord(r->ev[1], s));
   abea_batch_t b{};
   b.n_reads = n;
   b.seq = r->seq.as<char>();
   b.seq_ptr = r->seq_ptr.as<int64_t>();
   b.read_len = r->read_len.as<int32_t>();
-  b.events = r->events.as<abea_event_t>();
+  b.events = nullptr;
+  b.ev_mean = r->mean.as<float>();
   b.ev_ptr = r->ev_ptr.as<int64_t>();
   b.n_events = r->n_events.as<int32_t>();
   b.scalings = r->scal.as<abea_scalings_t>();
   b.model = r->model.as<abea_model_t>();
-  b.read_len_host = read_len.data();
-  b.n_events_host = n_events.data();
-  b.seq_ptr_host = seq_ptr.data();
-  b.ev_ptr_host = ev_ptr.data();
+  b.read_len_host = hb.read_len.data();
+  b.n_events_host = hb.n_events.data();
+  b.seq_ptr_host = hb.seq_ptr.data();
+  b.ev_ptr_host = hb.ev_ptr.data();
   if ((st = plan_bind(r->plan, &b, form))) return st;
   const int64_t cap = r->plan->total_pairs;
   if ((st = r->pairs.ensure(sizeof(abea_aligned_pair_t) * (size_t)std::max<int64_t>(cap, 1)))) return st;
   if ((st = r->n_pairs.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
+  if ((st = r->h_np.ensure(sizeof(int32_t) * (size_t)std::max(n, 1)))) return st;
   abea_output_t out{};
   out.pairs = r->pairs.as<abea_aligned_pair_t>();
   out.n_pairs = r->n_pairs.as<int32_t>();
   HIP_TRY(hipEventRecord(r->ev[2], s));
   if ((st = abea_plan_forward(r->plan, &out))) return st;
   HIP_TRY(hipEventRecord(r->ev[3], s));
-  pairs_h.resize((size_t)cap);
-  n_pairs_h.resize(n);
-  pair_off.resize(n);
-  abea_pair_offsets(n, read_len.data(), n_events.data(), pair_off.data());
-  if (cap)
-    HIP_TRY(hipMemcpyAsync(pairs_h.data(), out.pairs, sizeof(abea_aligned_pair_t) * cap,
+  // the counts, then only the pairs (compacted on the device)
+  HIP_TRY(hipMemcpyAsync(r->h_np.p, out.n_pairs, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  np.assign(r->h_np.as<int32_t>(), r->h_np.as<int32_t>() + n);
+  coff.resize((size_t)n + 1);
+  int64_t tot = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const int64_t slots = (int64_t)hb.n_events[i] + std::max<int64_t>((int64_t)hb.read_len[i] - 5, 0);
+    if (np[i] < 0 || np[i] > slots)
+      return fail(ABEA_ERR_HIP, "align: read %d returned %d pairs for %lld slots", i, np[i],
+                  (long long)slots);
+    coff[i] = tot;
+    tot += np[i];
+  }
+  coff[n] = tot;
+  if (tot) {
+    if ((st = upload(r->coff, coff.data(), (size_t)n, s))) return st;
+    if ((st = r->cpairs.ensure(sizeof(abea_aligned_pair_t) * (size_t)tot))) return st;
+    if ((st = r->h_pairs.ensure(sizeof(abea_aligned_pair_t) * (size_t)tot))) return st;
+    HIP_TRY(abea::launch_compact_pairs(r->plan->meta.as<ReadMeta>(), out.pairs, out.n_pairs,
+                                       r->coff.as<int64_t>(), r->cpairs.as<abea_aligned_pair_t>(),
+                                       n, s));
+    HIP_TRY(hipMemcpyAsync(r->h_pairs.p, r->cpairs.p, sizeof(abea_aligned_pair_t) * (size_t)tot,
                            hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(n_pairs_h.data(), out.n_pairs, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  }
   HIP_TRY(hipEventRecord(r->ev[4], s));
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
@@ -1208,26 +1303,40 @@ int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_
     fail(ABEA_ERR_INVALID, "abea_align: null argument");
     return -1;
   }
-  const auto t0 = std::chrono::steady_clock::now();
-  const int32_t n_e = (int32_t)events.n;
-  std::vector<char> seq(sequence, sequence + std::max(sequence_len, 0));
-  std::vector<int64_t> seq_ptr{0}, ev_ptr{0};
-  std::vector<int32_t> read_len{sequence_len}, n_events{n_e};
-  std::vector<abea_event_t> ev(events.event, events.event + events.n);
-  std::vector<abea_aligned_pair_t> pairs;
-  std::vector<int32_t> n_pairs;
-  std::vector<int64_t> off;
-  t_host_ms[0] = ms_since(t0);
-  if (run_host_batch(ABEA_PLAN_PLAIN, 1, seq, seq_ptr, read_len, ev, ev_ptr, n_events, &scaling,
-                     models, pairs, n_pairs, off))
+  if (events.n > (size_t)0x7fffffff) {
+    fail(ABEA_ERR_INVALID, "abea_align: %zu events (need < 2^31)", events.n);
     return -1;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  HostLease lease;
+  if (lease.acquire()) return -1;
+  HostRun *r = lease.r;
+  const int32_t n_e = (int32_t)events.n;
+  HostBatch hb;
+  hb.n = 1;
+  hb.seq_ptr = {0};
+  hb.ev_ptr = {0};
+  hb.read_len = {sequence_len};
+  hb.n_events = {n_e};
+  hb.n_bases = std::max(sequence_len, 0);
+  hb.n_ev = n_e;
+  if (r->h_seq.ensure((size_t)std::max<int64_t>(hb.n_bases, 1)) ||
+      r->h_mean.ensure(sizeof(float) * (size_t)std::max<int64_t>(hb.n_ev, 1)))
+    return -1;
+  if (hb.n_bases) std::memcpy(r->h_seq.p, sequence, (size_t)hb.n_bases);
+  float *mean = r->h_mean.as<float>();
+  for (int32_t j = 0; j < n_e; ++j) mean[j] = events.event[j].mean;
+  t_host_ms[0] = ms_since(t0);
+  std::vector<int32_t> np;
+  std::vector<int64_t> coff;
+  if (run_host_batch(r, ABEA_PLAN_PLAIN, hb, &scaling, models, np, coff)) return -1;
   const auto t4 = std::chrono::steady_clock::now();
-  int32_t np = n_pairs[0];
-  if ((int64_t)np > 2 * (int64_t)n_e) np = 0;  // would overrun f5c's 2*n_events buffer
-  std::memcpy(out_2, pairs.data(), sizeof(abea_aligned_pair_t) * np);
+  int32_t k = np[0];
+  if ((int64_t)k > 2 * (int64_t)n_e) k = 0;  // would overrun f5c's 2*n_events buffer
+  if (k) std::memcpy(out_2, r->h_pairs.p, sizeof(abea_aligned_pair_t) * (size_t)k);
   t_host_ms[5] = ms_since(t4);
   t_host_ms[6] = ms_since(t0);
-  return np;
+  return k;
 }
 
 abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
@@ -1239,48 +1348,74 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
                                           !event_align_pairs || !n_event_align_pairs)))
     return fail(ABEA_ERR_INVALID, "abea_align_db: null argument");
   const auto t0 = std::chrono::steady_clock::now();
-  // flatten, as GPU/align_cuda.c:432-492 does (no NUL terminators needed)
-  std::vector<int64_t> seq_ptr(n_bam_rec), ev_ptr(n_bam_rec);
-  std::vector<int32_t> rl(read_len, read_len + n_bam_rec), ne(n_bam_rec);
-  int64_t sl = 0, se = 0;
-  for (int32_t i = 0; i < n_bam_rec; ++i) {
-    seq_ptr[i] = sl;
-    sl += std::max(read_len[i], 0);
-    ev_ptr[i] = se;
-    ne[i] = (int32_t)et[i].n;
-    se += (int64_t)et[i].n;
+  const int32_t n = n_bam_rec;
+  HostBatch hb;
+  hb.n = n;
+  hb.seq_ptr.resize(n);
+  hb.ev_ptr.resize(n);
+  hb.read_len.assign(read_len, read_len + n);
+  hb.n_events.resize(n);
+  std::vector<int64_t> wpre((size_t)n + 1);
+  for (int32_t i = 0; i < n; ++i) {
+    if ((read_len[i] > 0 && !read[i]) || (et[i].n && !et[i].event))
+      return fail(ABEA_ERR_INVALID, "abea_align_db: read %d: null bases or events", i);
+    if (et[i].n > (size_t)0x7fffffff)
+      return fail(ABEA_ERR_INVALID, "abea_align_db: read %d: %zu events (need < 2^31)", i,
+                  et[i].n);
+    hb.seq_ptr[i] = hb.n_bases;
+    hb.ev_ptr[i] = hb.n_ev;
+    hb.n_events[i] = (int32_t)et[i].n;
+    wpre[i] = 24 * hb.n_ev + hb.n_bases;  // bytes the flatten reads before read i
+    hb.n_bases += std::max(read_len[i], 0);
+    hb.n_ev += (int64_t)et[i].n;
   }
-  std::vector<char> seq((size_t)sl);
-  std::vector<abea_event_t> ev((size_t)se);
-  for (int32_t i = 0; i < n_bam_rec; ++i) {
-    if (read_len[i] > 0) std::memcpy(seq.data() + seq_ptr[i], read[i], (size_t)read_len[i]);
-    if (et[i].n) std::memcpy(ev.data() + ev_ptr[i], et[i].event, sizeof(abea_event_t) * et[i].n);
-  }
-  std::vector<abea_aligned_pair_t> pairs;
-  std::vector<int32_t> n_pairs;
-  std::vector<int64_t> off;
-  t_host_ms[0] = ms_since(t0);
-  abea_status_t st = run_host_batch(ABEA_PLAN_BATCHED, n_bam_rec, seq, seq_ptr, rl, ev, ev_ptr, ne,
-                                    scalings, model, pairs, n_pairs, off);
+  wpre[n] = 24 * hb.n_ev + hb.n_bases;
+  HostLease lease;
+  abea_status_t st = lease.acquire();
   if (st) return st;
+  HostRun *r = lease.r;
+  if ((st = r->h_seq.ensure((size_t)std::max<int64_t>(hb.n_bases, 1))) ||
+      (st = r->h_mean.ensure(sizeof(float) * (size_t)std::max<int64_t>(hb.n_ev, 1))))
+    return st;
+  // flatten, as GPU/align_cuda.c:432-492 does, into pinned staging: the bases
+  // (no NUL terminators needed) and the event means, the one field of the
+  // event records the alignment reads
+  char *hs = r->h_seq.as<char>();
+  float *hm = r->h_mean.as<float>();
+  parallel_ranges(n, wpre, [&](int32_t b, int32_t e) {
+    for (int32_t i = b; i < e; ++i) {
+      if (read_len[i] > 0) std::memcpy(hs + hb.seq_ptr[i], read[i], (size_t)read_len[i]);
+      const abea_event_t *src = et[i].event;
+      float *dst = hm + hb.ev_ptr[i];
+      for (size_t j = 0; j < et[i].n; ++j) dst[j] = src[j].mean;
+    }
+  });
+  t_host_ms[0] = ms_since(t0);
+  std::vector<int32_t> np;
+  std::vector<int64_t> coff;
+  if ((st = run_host_batch(r, ABEA_PLAN_BATCHED, hb, scalings, model, np, coff))) return st;
   const auto t4 = std::chrono::steady_clock::now();
   // unflatten into callee-malloc'd per-read arrays, GPU/align_cuda.c:926-931
-  for (int32_t i = 0; i < n_bam_rec; ++i) {
-    const int32_t np = n_pairs[i];
+  for (int32_t i = 0; i < n; ++i) {
     event_align_pairs[i] =
-        (abea_aligned_pair_t *)malloc(sizeof(abea_aligned_pair_t) * (size_t)std::max(np, 1));
+        (abea_aligned_pair_t *)malloc(sizeof(abea_aligned_pair_t) * (size_t)std::max(np[i], 1));
     if (!event_align_pairs[i]) {
       // the caller owns nothing from a failed call
       for (int32_t j = 0; j < i; ++j) free(event_align_pairs[j]);
-      for (int32_t j = 0; j < n_bam_rec; ++j) {
+      for (int32_t j = 0; j < n; ++j) {
         event_align_pairs[j] = nullptr;
         n_event_align_pairs[j] = 0;
       }
       return fail(ABEA_ERR_NOMEM, "abea_align_db: malloc failed");
     }
-    n_event_align_pairs[i] = np;
-    if (np) std::memcpy(event_align_pairs[i], pairs.data() + off[i], sizeof(abea_aligned_pair_t) * np);
+    n_event_align_pairs[i] = np[i];
   }
+  const abea_aligned_pair_t *hp = r->h_pairs.as<abea_aligned_pair_t>();
+  parallel_ranges(n, coff, [&](int32_t b, int32_t e) {
+    for (int32_t i = b; i < e; ++i)
+      if (np[i])
+        std::memcpy(event_align_pairs[i], hp + coff[i], sizeof(abea_aligned_pair_t) * (size_t)np[i]);
+  });
   t_host_ms[5] = ms_since(t4);
   t_host_ms[6] = ms_since(t0);
   return ABEA_OK;

@@ -215,6 +215,11 @@ hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s);
 hipError_t launch_core(const KernelArgs &a, int q_begin, int q_count, bool persistent,
                        int waves_per_block, int grid_blocks, int fuse, int overlap, hipStream_t s);
 hipError_t launch_post(const KernelArgs &a, int q_begin, int q_count, int overlap, hipStream_t s);
+// Copies read r's n_pairs[r] pairs from its slots to dst + coff[r] (the
+// host-buffer drop-ins' device-to-host compaction).
+hipError_t launch_compact_pairs(const ReadMeta *meta, const abea_aligned_pair_t *pairs,
+                                const int32_t *n_pairs, const int64_t *coff,
+                                abea_aligned_pair_t *dst, int n_reads, hipStream_t s);
 // grad_model: [4096][3] doubles (level_mean, level_stdv, level_log_stdv).
 // partial holds n_blocks slices of 2 * 4096 * 3 fixed-point limbs (one per
 // workgroup, n_blocks = adjoint_blocks()); special (4096 * 3 doubles) takes

@@ -1594,9 +1594,39 @@ __global__ void abea_refine_advance_kernel(int32_t *step, int32_t set_to) {
   if (threadIdx.x == 0) *step = set_to >= 0 ? set_to : *step + 1;
 }
 
+// Host-buffer drop-ins: each read's pairs, from its slots (ReadMeta
+// pair_off) to dst + coff[r], one wave per read, so that the device-to-host
+// copy moves only the pairs.
+__global__ __launch_bounds__(256) void abea_compact_pairs_kernel(const ReadMeta *meta,
+                                                                const abea_aligned_pair_t *pairs,
+                                                                const int32_t *n_pairs,
+                                                                const int64_t *coff,
+                                                                abea_aligned_pair_t *dst,
+                                                                int n_reads) {
+  const int lane = threadIdx.x & 63;
+  const int wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int n_waves = (gridDim.x * blockDim.x) >> 6;
+  for (int r = wave; r < n_reads; r += n_waves) {
+    const int np = n_pairs[r];
+    const abea_aligned_pair_t *src = pairs + meta[r].pair_off;
+    abea_aligned_pair_t *d = dst + coff[r];
+    for (int i = lane; i < np; i += 64) d[i] = src[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 hipError_t launch_queue_reset(const KernelArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(abea_queue_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact_pairs(const ReadMeta *meta, const abea_aligned_pair_t *pairs,
+                                const int32_t *n_pairs, const int64_t *coff,
+                                abea_aligned_pair_t *dst, int n_reads, hipStream_t s) {
+  if (n_reads <= 0) return hipSuccess;
+  const int grid = std::min((n_reads + 3) / 4, 4096);
+  hipLaunchKernelGGL(abea_compact_pairs_kernel, dim3(grid), dim3(256), 0, s, meta, pairs, n_pairs,
+                     coff, dst, n_reads);
   return hipGetLastError();
 }
 

@@ -264,55 +264,74 @@ def dropin_leg(args):
     libc = C.CDLL(None)
     libc.free.argtypes = [C.c_void_p]
     libc.free.restype = None
-    b = synth.workload("plain4096_ln5kb", seed=args.seed + 1, with_aos=True)
-    n = b.n_reads
-    aos = b.events_aos()
-    model = np.ascontiguousarray(b.model, np.float32)
-    reads, ets = (C.c_char_p * n)(), (_lib.EventTable * n)()
-    seqs = [b.read_seq(i) for i in range(n)]
-    base = aos.ctypes.data
-    for i in range(n):
-        reads[i] = seqs[i]
-        ets[i] = _lib.EventTable(int(b.n_events[i]), 0, int(b.n_events[i]),
-                                 C.cast(base + 24 * int(b.ev_ptr[i]), C.POINTER(_lib.EventT)))
-    rl = np.ascontiguousarray(b.read_len, np.int32)
-    scal = np.ascontiguousarray(b.scalings(), np.float32)
-    outp = (C.POINTER(_lib.AlignedPair) * n)()
-    nout = np.zeros(n, np.int32)
-    phases = []
-    walls = []
-    for rep in range(3):
-        t0 = time.perf_counter()
-        st = L.abea_align_db(n, C.cast(reads, C.c_void_p), rl.ctypes.data_as(C.c_void_p),
-                             C.cast(ets, C.c_void_p), model.ctypes.data_as(C.c_void_p),
-                             scal.ctypes.data_as(C.c_void_p), C.cast(outp, C.c_void_p),
-                             nout.ctypes.data_as(C.c_void_p))
-        walls.append(time.perf_counter() - t0)
-        _lib.check(st, "abea_align_db")
-        t = (C.c_double * 7)()
-        L.abea_host_last_times(t)
-        phases.append(list(t))
+    def host_batch(b):
+        n = b.n_reads
+        aos = b.events_aos()
+        reads, ets = (C.c_char_p * n)(), (_lib.EventTable * n)()
+        seqs = [b.read_seq(i) for i in range(n)]
+        base = aos.ctypes.data
         for i in range(n):
-            libc.free(C.cast(outp[i], C.c_void_p))
-    k = int(np.argmin(walls[1:])) + 1  # the first call also sizes the pooled buffers
-    ph = phases[k]
-    # bytes of the input copies (bases, event_t records, scalings, model,
-    # CSR offsets and counts) and of the output copies (every pair slot and
-    # the counts)
-    h2d_bytes = (int(b.read_len.sum()) + 24 * int(b.n_events.sum()) + 16 * n + 12 * 4096
-                 + 24 * n)
-    cap = int((b.n_events.astype(np.int64) + b.n_kmers).sum())
-    d2h_bytes = 8 * cap + 4 * n
-    cells = b.nominal_cells()
-    res = {"align_db": {"workload": "plain4096_ln5kb", "reads": n,
-                        "gcups": round(cells / walls[k] / 1e9, 3),
-                        "reads_per_s": round(n / walls[k], 1),
-                        "ms": {"flatten": round(ph[0], 3), "h2d": round(ph[1], 3),
-                               "plan": round(ph[2], 3), "kernels": round(ph[3], 3),
-                               "d2h": round(ph[4], 3), "unflatten": round(ph[5], 3),
-                               "total": round(ph[6], 3)},
-                        "h2d_gbs": round(h2d_bytes / (ph[1] * 1e-3) / 1e9, 2),
-                        "d2h_gbs": round(d2h_bytes / (ph[4] * 1e-3) / 1e9, 2)}}
+            reads[i] = seqs[i]
+            ets[i] = _lib.EventTable(int(b.n_events[i]), 0, int(b.n_events[i]),
+                                     C.cast(base + 24 * int(b.ev_ptr[i]), C.POINTER(_lib.EventT)))
+        return aos, seqs, reads, ets
+
+    def align_db(wl, b):
+        n = b.n_reads
+        model = np.ascontiguousarray(b.model, np.float32)
+        aos, seqs, reads, ets = host_batch(b)
+        rl = np.ascontiguousarray(b.read_len, np.int32)
+        scal = np.ascontiguousarray(b.scalings(), np.float32)
+        outp = (C.POINTER(_lib.AlignedPair) * n)()
+        nout = np.zeros(n, np.int32)
+        phases = []
+        walls = []
+        for rep in range(3):
+            t0 = time.perf_counter()
+            st = L.abea_align_db(n, C.cast(reads, C.c_void_p), rl.ctypes.data_as(C.c_void_p),
+                                 C.cast(ets, C.c_void_p), model.ctypes.data_as(C.c_void_p),
+                                 scal.ctypes.data_as(C.c_void_p), C.cast(outp, C.c_void_p),
+                                 nout.ctypes.data_as(C.c_void_p))
+            walls.append(time.perf_counter() - t0)
+            _lib.check(st, "abea_align_db")
+            t = (C.c_double * 7)()
+            L.abea_host_last_times(t)
+            phases.append(list(t))
+            for i in range(n):
+                libc.free(C.cast(outp[i], C.c_void_p))
+        k = int(np.argmin(walls[1:])) + 1  # the first call also sizes the pooled buffers
+        ph = phases[k]
+        # bytes of the input copies (bases, event means, scalings, model, CSR
+        # offsets and counts) and of the output copies (the counts, then the
+        # pairs, compacted on the device)
+        h2d_bytes = (int(b.read_len.sum()) + 4 * int(b.n_events.sum()) + 16 * n + 12 * 4096
+                     + 24 * n)
+        n_pairs = int(nout.astype(np.int64).sum())
+        d2h_bytes = 4 * n + 8 * n_pairs
+        cells = b.nominal_cells()
+        return {"workload": wl, "reads": n, "qc_pass_reads": int((nout > 0).sum()),
+                "gcups": round(cells / walls[k] / 1e9, 3),
+                "reads_per_s": round(n / walls[k], 1),
+                "ms": {"flatten": round(ph[0], 3), "h2d": round(ph[1], 3),
+                       "plan": round(ph[2], 3), "kernels": round(ph[3], 3),
+                       "d2h": round(ph[4], 3), "unflatten": round(ph[5], 3),
+                       "total": round(ph[6], 3)},
+                "h2d_bytes": h2d_bytes, "d2h_bytes": d2h_bytes,
+                "h2d_gbs": round(h2d_bytes / (ph[1] * 1e-3) / 1e9, 2),
+                "d2h_gbs": round(d2h_bytes / (ph[4] * 1e-3) / 1e9, 2),
+                "pairs_returned": n_pairs}
+
+    b = synth.workload("plain4096_ln5kb", seed=args.seed + 1, with_aos=True)
+    res = {"align_db": align_db("plain4096_ln5kb", b)}
+    # the same lengths at the real reads' k-mer skip rate: every read passes
+    # QC and returns its pairs
+    bq = synth.generate(b.read_len, seed=args.seed + 1, with_aos=True, skip=0.015)
+    res["align_db_qc_pass"] = align_db("plain4096_ln5kb, skip rate 1.5%", bq)
+    del bq
+    log(f"[bench] align_db: {res}")
+    n = b.n_reads
+    model = np.ascontiguousarray(b.model, np.float32)
+    aos, seqs, reads, ets = host_batch(b)
     # align() per read: the first 64 reads, one thread, then 8 threads at once
     m = 64
     outs = [np.zeros((2 * int(b.n_events[i]) + 1, 2), np.int32) for i in range(m)]
@@ -326,26 +345,38 @@ def dropin_leg(args):
     for i in range(8):
         one(i)
     t0 = time.perf_counter()
+    ph1 = np.zeros(7)
     for i in range(m):
         one(i)
+        t = (C.c_double * 7)()
+        L.abea_host_last_times(t)
+        ph1 += np.array(list(t))
     seq_s = time.perf_counter() - t0
+    ph1 /= m
 
     def worker(t):
         for i in range(t, m, 8):
             one(i)
 
-    ts = [threading.Thread(target=worker, args=(t,)) for t in range(8)]
-    t0 = time.perf_counter()
-    for t in ts:
-        t.start()
-    for t in ts:
-        t.join()
-    thr_s = time.perf_counter() - t0
+    def threaded():
+        ts = [threading.Thread(target=worker, args=(t,)) for t in range(8)]
+        t0 = time.perf_counter()
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return time.perf_counter() - t0
+
+    threaded()  # creates and sizes the pooled contexts the threads use
+    thr_s = threaded()
     cells_m = int(W * b.n_bands[:m].sum())
     res["align_per_read"] = {"reads": m, "one_thread_reads_per_s": round(m / seq_s, 1),
                              "one_thread_gcups": round(cells_m / seq_s / 1e9, 3),
                              "eight_threads_reads_per_s": round(m / thr_s, 1),
                              "ms_per_read": round(1e3 * seq_s / m, 3),
+                             "ms_per_read_phases": dict(zip(
+                                 ["flatten", "h2d", "plan", "kernels", "d2h", "unflatten",
+                                  "total"], [round(float(x), 3) for x in ph1])),
                              "host_contexts": int(L.abea_host_contexts())}
     log(f"[bench] drop-ins: {res}")
     return res
