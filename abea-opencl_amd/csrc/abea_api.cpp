@@ -899,6 +899,8 @@ abea_status_t abea_plan_estimate_scalings(abea_plan_t *p, abea_scalings_t *out) 
   HIP_TRY(hipSetDevice(p->ctx->device));
   abea::MomArgs a{};
   a.n_reads = p->n;
+  // the batched form's queue is longest first; the plain form keeps input order
+  a.order = p->form == ABEA_PLAN_BATCHED ? p->queue.as<int32_t>() : nullptr;
   a.meta = p->meta.as<ReadMeta>();
   a.seq = p->batch.seq;
   a.seq_ptr = p->batch.seq_ptr;
@@ -1087,6 +1089,7 @@ struct HostRun {
   // outputs (pair counts, compacted pairs), and the compaction's offsets
   PinBuf h_seq, h_mean, h_np, h_pairs;
   DevBuf mean, coff, cpairs;
+  DevBuf mom_order;  // MOM's read order (longest first)
   // post-align and MOM entries
   DevBuf pmeta, rec, bmap, n_rec, epb, flag, status, scal_out, terms, pstat, items;
   // getevents entries
@@ -1508,8 +1511,18 @@ abea_status_t run_mom(int32_t n, char *const *read, const int32_t *read_len,
   host_post_layout(h, n, read, read_len, et, nullptr);
   if ((st = host_post_upload(r, h, model, s))) return st;
   if ((st = r->scal_out.ensure(sizeof(abea_scalings_t) * (size_t)std::max(n, 1)))) return st;
+  // longest serial chains first
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  auto chain = [&](int32_t i) {
+    return std::max<int64_t>((int64_t)et[i].n, std::max<int64_t>((int64_t)read_len[i] - 5, 0));
+  };
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return chain(x) > chain(y); });
+  if ((st = upload(r->mom_order, order.data(), order.size(), s))) return st;
   abea::MomArgs a{};
   a.n_reads = n;
+  a.order = r->mom_order.as<int32_t>();
   a.meta = r->pmeta.as<ReadMeta>();
   a.seq = r->seq.as<char>();
   a.seq_ptr = r->seq_ptr.as<int64_t>();

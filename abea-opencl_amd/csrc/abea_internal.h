@@ -126,6 +126,7 @@ struct RecalArgs {
 // estimate_scalings_using_mom, one wave per read.
 struct MomArgs {
   int32_t n_reads;
+  const int32_t *order;          // reads in the order the waves take them (null: input order)
   const ReadMeta *meta;          // n_e, n_k, ev_off
   const char *seq;
   const int64_t *seq_ptr;

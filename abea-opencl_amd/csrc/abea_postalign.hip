@@ -650,7 +650,9 @@ __device__ __forceinline__ bool exact_total(const ExactSum &es, double &total) {
 // estimate_scalings_using_mom, one wave per read.  Pass 1 sums the event
 // means and the k-mer levels (exact in parallel when provable, else again
 // serially); pass 2 runs the two serial sums of squares side by side, the
-// levels' in lane 0 and the events' in lane 1.  Each pass loads the next
+// levels' in lane 0 and the events' in lane 1.  The serial chains of the
+// longest read bound the launch, so the waves take the reads longest first
+// (order) when the caller gives that order.  Each pass loads the next
 // chunk's event means and bases while it works on the current one; the
 // level means sit in LDS.
 constexpr int kMomWaves = 8;
@@ -699,8 +701,9 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
   __syncthreads();
   double *st = stages[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
-  const int r = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  if (r >= a.n_reads) return;
+  const int q = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (q >= a.n_reads) return;
+  const int r = a.order ? uniform(a.order[q]) : q;
   const ReadMeta m = a.meta[r];
   MomRead d;
   d.events = a.events;
