@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
+#include <condition_variable>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -1294,6 +1295,114 @@ abea_status_t run_host_batch(HostRun *r, int form, const HostBatch &hb,
   return ABEA_OK;
 }
 
+// Concurrent abea_align() calls on one device are gathered into batches.
+// f5c calls align() from its worker threads, one read per call
+// (GPU/f5c.h:57-59), and one read is one wave walking its bands, so calls run
+// one by one leave the GPU nearly idle.  A call queues its read; if no batch
+// is running on its device, the call takes every queued read that shares the
+// oldest one's model, runs them as one batched forward (whose results equal
+// the per-read form's), hands each caller its pairs and wakes them.  Calls
+// that arrive meanwhile wait for the next batch, which one of them runs.  A
+// call alone runs a batch of one.
+struct AlignReq {
+  const char *seq = nullptr;
+  int32_t len = 0;
+  const abea_event_t *ev = nullptr;
+  int32_t n_e = 0;
+  abea_scalings_t sc{};
+  const abea_model_t *model = nullptr;
+  abea_aligned_pair_t *out = nullptr;
+  int32_t result = -1;
+  bool done = false;
+  std::string err;
+  double times[7] = {0, 0, 0, 0, 0, 0, 0};  // the batch's phases
+};
+
+struct Collector {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<AlignReq *> pending;
+  bool busy = false;  // a batch is running on this device
+  int callers = 0;    // calls inside abea_align on this device
+  int seen = 0;       // most callers at once since the last batch started
+  int target = 1;     // the same over the interval before (a batch's target size)
+};
+
+// How long a call about to run a batch waits for more calls to queue, when
+// fewer are queued than ran at once before: the callers of the last batch
+// come back a few microseconds after it ends, and a batch of one read leaves
+// the GPU nearly idle for a whole walk (milliseconds).
+constexpr auto kGatherWait = std::chrono::microseconds(200);
+
+constexpr int kMaxHostDevices = 64;
+Collector g_coll[kMaxHostDevices];
+
+// Runs one batch of queued reads; sets every request's result (pairs, or -1
+// and the error).  Catches everything: the waiting callers depend on it.
+void run_align_batch(const std::vector<AlignReq *> &q) noexcept {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int32_t n = (int32_t)q.size();
+  abea_status_t st = ABEA_OK;
+  try {
+    HostLease lease;
+    st = lease.acquire();
+    HostRun *r = lease.r;
+    HostBatch hb;
+    std::vector<abea_scalings_t> scal(n);
+    if (!st) {
+      hb.n = n;
+      hb.seq_ptr.resize(n);
+      hb.ev_ptr.resize(n);
+      hb.read_len.resize(n);
+      hb.n_events.resize(n);
+      for (int32_t i = 0; i < n; ++i) {
+        hb.seq_ptr[i] = hb.n_bases;
+        hb.ev_ptr[i] = hb.n_ev;
+        hb.read_len[i] = q[i]->len;
+        hb.n_events[i] = q[i]->n_e;
+        scal[i] = q[i]->sc;
+        hb.n_bases += std::max(q[i]->len, 0);
+        hb.n_ev += q[i]->n_e;
+      }
+      if (!(st = r->h_seq.ensure((size_t)std::max<int64_t>(hb.n_bases, 1))))
+        st = r->h_mean.ensure(sizeof(float) * (size_t)std::max<int64_t>(hb.n_ev, 1));
+    }
+    std::vector<int32_t> np;
+    std::vector<int64_t> coff;
+    if (!st) {
+      char *hs = r->h_seq.as<char>();
+      float *hm = r->h_mean.as<float>();
+      for (int32_t i = 0; i < n; ++i) {
+        if (q[i]->len > 0) std::memcpy(hs + hb.seq_ptr[i], q[i]->seq, (size_t)q[i]->len);
+        float *dst = hm + hb.ev_ptr[i];
+        for (int32_t j = 0; j < q[i]->n_e; ++j) dst[j] = q[i]->ev[j].mean;
+      }
+      t_host_ms[0] = ms_since(t0);
+      st = run_host_batch(r, ABEA_PLAN_BATCHED, hb, scal.data(), q[0]->model, np, coff);
+    }
+    if (!st) {
+      const auto t4 = std::chrono::steady_clock::now();
+      const abea_aligned_pair_t *hp = r->h_pairs.as<abea_aligned_pair_t>();
+      for (int32_t i = 0; i < n; ++i) {
+        int32_t k = np[i];
+        if ((int64_t)k > 2 * (int64_t)q[i]->n_e) k = 0;  // would overrun f5c's 2*n_events buffer
+        if (k) std::memcpy(q[i]->out, hp + coff[i], sizeof(abea_aligned_pair_t) * (size_t)k);
+        q[i]->result = k;
+      }
+      t_host_ms[5] = ms_since(t4);
+      t_host_ms[6] = ms_since(t0);
+      for (AlignReq *x : q) std::copy(t_host_ms, t_host_ms + 7, x->times);
+      return;
+    }
+  } catch (const std::exception &e) {
+    fail(ABEA_ERR_NOMEM, "abea_align: %s", e.what());
+  }
+  for (AlignReq *x : q) {
+    x->result = -1;
+    x->err = g_err;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1310,36 +1419,71 @@ int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_
     fail(ABEA_ERR_INVALID, "abea_align: %zu events (need < 2^31)", events.n);
     return -1;
   }
-  const auto t0 = std::chrono::steady_clock::now();
-  HostLease lease;
-  if (lease.acquire()) return -1;
-  HostRun *r = lease.r;
-  const int32_t n_e = (int32_t)events.n;
-  HostBatch hb;
-  hb.n = 1;
-  hb.seq_ptr = {0};
-  hb.ev_ptr = {0};
-  hb.read_len = {sequence_len};
-  hb.n_events = {n_e};
-  hb.n_bases = std::max(sequence_len, 0);
-  hb.n_ev = n_e;
-  if (r->h_seq.ensure((size_t)std::max<int64_t>(hb.n_bases, 1)) ||
-      r->h_mean.ensure(sizeof(float) * (size_t)std::max<int64_t>(hb.n_ev, 1)))
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxHostDevices) {
+    (void)hipGetLastError();
+    fail(ABEA_ERR_HIP, "abea_align: no usable current device (%d)", dev);
     return -1;
-  if (hb.n_bases) std::memcpy(r->h_seq.p, sequence, (size_t)hb.n_bases);
-  float *mean = r->h_mean.as<float>();
-  for (int32_t j = 0; j < n_e; ++j) mean[j] = events.event[j].mean;
-  t_host_ms[0] = ms_since(t0);
-  std::vector<int32_t> np;
-  std::vector<int64_t> coff;
-  if (run_host_batch(r, ABEA_PLAN_PLAIN, hb, &scaling, models, np, coff)) return -1;
-  const auto t4 = std::chrono::steady_clock::now();
-  int32_t k = np[0];
-  if ((int64_t)k > 2 * (int64_t)n_e) k = 0;  // would overrun f5c's 2*n_events buffer
-  if (k) std::memcpy(out_2, r->h_pairs.p, sizeof(abea_aligned_pair_t) * (size_t)k);
-  t_host_ms[5] = ms_since(t4);
-  t_host_ms[6] = ms_since(t0);
-  return k;
+  }
+  AlignReq req;
+  req.seq = sequence;
+  req.len = sequence_len;
+  req.ev = events.event;
+  req.n_e = (int32_t)events.n;
+  req.sc = scaling;
+  req.model = models;
+  req.out = out_2;
+  Collector &c = g_coll[dev];
+  try {
+    std::unique_lock<std::mutex> lk(c.mu);
+    c.pending.push_back(&req);
+    c.seen = std::max(c.seen, ++c.callers);
+    c.cv.notify_all();  // a call waiting to gather may now have enough
+    while (!req.done) {
+      if (c.busy) {
+        c.cv.wait(lk);
+        continue;
+      }
+      if ((int)c.pending.size() < c.target) {
+        const auto until = std::chrono::steady_clock::now() + kGatherWait;
+        while (!c.busy && !req.done && (int)c.pending.size() < c.target &&
+               c.cv.wait_until(lk, until) != std::cv_status::timeout) {
+        }
+        if (c.busy || req.done) continue;
+      }
+      c.target = std::max(1, c.seen);
+      c.seen = c.callers;
+      // run every queued read with the oldest one's model as one batch
+      std::vector<AlignReq *> batch, rest;
+      batch.reserve(c.pending.size());
+      rest.reserve(c.pending.size());
+      const abea_model_t *mdl = c.pending.front()->model;
+      for (AlignReq *q : c.pending) (q->model == mdl ? batch : rest).push_back(q);
+      c.pending.swap(rest);
+      c.busy = true;
+      lk.unlock();
+      run_align_batch(batch);
+      lk.lock();
+      for (AlignReq *q : batch) q->done = true;
+      c.busy = false;
+      c.cv.notify_all();
+    }
+    --c.callers;
+  } catch (const std::exception &e) {
+    // Only the queues' allocations can throw, before this call's read left
+    // the queue (the batch itself catches everything): take it out again.
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = std::find(c.pending.begin(), c.pending.end(), &req);
+    if (it != c.pending.end()) {
+      c.pending.erase(it);
+      --c.callers;
+    }
+    fail(ABEA_ERR_NOMEM, "abea_align: %s", e.what());
+    return -1;
+  }
+  std::copy(req.times, req.times + 7, t_host_ms);
+  if (req.result < 0) abea::set_last_error(req.err.c_str());
+  return req.result;
 }
 
 abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_len,
