@@ -104,10 +104,12 @@ const char *abea_last_error(void);
  * out_2 must hold 2 * events.n pairs (f5c's allocation,
  * GPU/align_cuda.c:496); a path that would not fit is reported as a failed
  * read (0) instead of overrunning.  sample_rate is unused, as in the
- * reference (GPU/align_cuda.cl:84-88).  Runs the plain per-read form on a
- * pooled context of the current device (abea_host_contexts()), so threads
- * may call it concurrently; returns -1 on a device error (abea_last_error()
- * tells which). */
+ * reference (GPU/align_cuda.cl:84-88).  Threads may call it concurrently:
+ * calls that overlap in time on the current device are gathered into one
+ * batched forward on a pooled context (abea_host_contexts()), with the same
+ * results as one call at a time; a call may wait up to 200 us for others to
+ * join its batch.  Returns -1 on a device error (abea_last_error() tells
+ * which). */
 int32_t abea_align(abea_aligned_pair_t *out_2, char *sequence, int32_t sequence_len,
                    abea_event_table events, abea_model_t *models, abea_scalings_t scaling,
                    float sample_rate);
