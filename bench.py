@@ -46,6 +46,7 @@ W = 100
 # 1 GiB copy's readings) and VALU wave-instructions (SQ_INSTS_VALU).  Keyed by
 # (kernel, waves per CU, traceback overlap).
 PROFILED = {
+    ("abea_core_kernel<1, true>", 16, 128): {"traffic": 46567172736, "valu": 2.0855e10},
     ("abea_core_kernel<1, true>", 24, 128): {"traffic": 49144735264, "valu": 2.0855e10},
     ("abea_core_kernel<1, true>", 32, 128): {"traffic": 49049624960, "valu": 2.0855e10},
 }
