@@ -62,19 +62,37 @@ def _forward(ctx, b, form=1):
     return plan, plan.results()
 
 
-def test_headline_tail_reads(ctx):
+@pytest.mark.parametrize("form", [1, 0], ids=["batched", "plain"])
+def test_headline_tail_reads(ctx, form):
     """The 8 longest read lengths of the headline workload (up to 63.5 kb,
     ~168 k bands) and one read at the workload's 200 kb clip (~530 k bands):
-    the longest walks and tracebacks of a few thousand rows per segment."""
+    the longest walks and tracebacks of a few thousand rows per segment, in
+    the batched form and in the reference's shape (one launch pair per
+    read)."""
     from abea_amd import synth
     L = np.sort(synth.workload_lengths("batched16384_ln8kb", seed=0))[-8:]
     assert L[-1] > 60000
     L = np.concatenate([L, [200000]])
     b = synth.generate(L, seed=101)
-    print(f"[tail] lengths {L.tolist()}, events {b.n_events.tolist()}", flush=True)
-    plan, res = _forward(ctx, b)
+    print(f"[tail] form {form}: lengths {L.tolist()}, events {b.n_events.tolist()}", flush=True)
+    plan, res = _forward(ctx, b, form)
     assert _oracle_batch_mismatches(b, res) == []
     plan.close()
+
+
+def test_plain_form_on_plain_workload_reads(ctx):
+    """96 reads drawn from the 4,096-read 5 kb workload (the one the plain
+    form is timed on) in the plain form, and the same reads in the batched
+    form: every field of every read equal to the oracle's."""
+    from abea_amd import synth
+    lens = synth.workload_lengths("plain4096_ln5kb", seed=1)
+    rng = np.random.default_rng(23)
+    b = synth.generate(rng.choice(lens, 96, replace=False), seed=23)
+    for form in (0, 1):
+        print(f"[plain] form {form}: {b.n_reads} reads, {int(b.read_len.sum())} bases", flush=True)
+        plan, res = _forward(ctx, b, form)
+        assert _oracle_batch_mismatches(b, res) == [], form
+        plan.close()
 
 
 def _fd_params(b, res):
