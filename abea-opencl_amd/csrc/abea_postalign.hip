@@ -652,10 +652,13 @@ __device__ __forceinline__ bool exact_total(const ExactSum &es, double &total) {
 // serially); pass 2 runs the two serial sums of squares side by side, the
 // levels' in lane 0 and the events' in lane 1.  The serial chains of the
 // longest read bound the launch, so the waves take the reads longest first
-// (order) when the caller gives that order.  Each pass loads the next
-// chunk's event means and bases while it works on the current one; the
-// level means sit in LDS.
+// (order) when the caller gives that order, and each pass keeps the loads of
+// the next kMomGroup chunks of 64 terms in flight while it works on the
+// current ones (one chunk ahead left a long read's wave waiting on memory
+// for most of each chunk); the level means sit in LDS.
 constexpr int kMomWaves = 8;
+constexpr int kMomGroup = 4;
+constexpr int kMomSpan = 64 * kMomGroup;  // terms per group
 
 struct MomRead {
   const abea_event_t *events;
@@ -679,13 +682,19 @@ __device__ __forceinline__ void mom_load(const MomRead &d, int i, MomLoad &L) {
   if (d.nk > 0) kmer_bytes(d.S, min(i, d.nk - 1), L.lo, L.hi);
 }
 
-// acc += v[0] + v[1] + ... + v[63], one add at a time in index order, the
-// LDS loads of 16 terms issued ahead of their adds.  The entries past a
-// chunk's last term are +0.0, which leaves acc unchanged: acc starts at +0
-// and is never -0 (a sum that cancels exactly rounds to +0).
-__device__ __forceinline__ void serial_sum64(const double *v, double &acc) {
+__device__ __forceinline__ void mom_load_group(const MomRead &d, int base, int lane,
+                                               MomLoad (&L)[kMomGroup]) {
 #pragma unroll
-  for (int j0 = 0; j0 < 64; j0 += 16) {
+  for (int g = 0; g < kMomGroup; ++g) mom_load(d, base + 64 * g + lane, L[g]);
+}
+
+// acc += v[0] + v[1] + ... + v[kMomSpan - 1], one add at a time in index
+// order, the LDS loads of 16 terms issued ahead of their adds.  The entries
+// past a read's last term are +0.0, which leaves acc unchanged: acc starts
+// at +0 and is never -0 (a sum that cancels exactly rounds to +0).
+__device__ __forceinline__ void serial_sum_span(const double *v, double &acc) {
+#pragma unroll 4
+  for (int j0 = 0; j0 < kMomSpan; j0 += 16) {
     double t[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) t[j] = v[j0 + j];
@@ -695,7 +704,7 @@ __device__ __forceinline__ void serial_sum64(const double *v, double &acc) {
 }
 
 __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
-  __shared__ double stages[kMomWaves][128];
+  __shared__ double stages[kMomWaves][2 * kMomSpan];
   __shared__ float levels[ABEA_NUM_KMER];
   for (int i = threadIdx.x; i < ABEA_NUM_KMER; i += blockDim.x) levels[i] = a.model[i].level_mean;
   __syncthreads();
@@ -717,28 +726,39 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
   const int n_max = max(n_e, nk);
 
   ExactSum ex, el;
-  MomLoad nxt;
-  mom_load(d, lane, nxt);
-  for (int base = 0; base < n_max; base += 64) {
-    const MomLoad cur = nxt;
-    mom_load(d, base + 64 + lane, nxt);
-    const int i = base + lane;
-    if (i < n_e) exact_add(ex, cur.x);
-    if (i < nk) exact_add(el, levels[rank_lohi(cur.lo, cur.hi)]);
+  MomLoad nxt[kMomGroup];
+  mom_load_group(d, 0, lane, nxt);
+  for (int base = 0; base < n_max; base += kMomSpan) {
+    MomLoad cur[kMomGroup];
+#pragma unroll
+    for (int g = 0; g < kMomGroup; ++g) cur[g] = nxt[g];
+    mom_load_group(d, base + kMomSpan, lane, nxt);
+#pragma unroll
+    for (int g = 0; g < kMomGroup; ++g) {
+      const int i = base + 64 * g + lane;
+      if (i < n_e) exact_add(ex, cur[g].x);
+      if (i < nk) exact_add(el, levels[rank_lohi(cur[g].lo, cur[g].hi)]);
+    }
   }
   double ev_sum, km_sum;
   const bool ex_ok = exact_total(ex, ev_sum), el_ok = exact_total(el, km_sum);
   if (!ex_ok || !el_ok) {
     double acc = 0.0;
-    mom_load(d, lane, nxt);
-    for (int base = 0; base < n_max; base += 64) {
-      const MomLoad cur = nxt;
-      mom_load(d, base + 64 + lane, nxt);
-      const int i = base + lane;
-      st[lane] = i < n_e ? (double)cur.x : 0.0;
-      st[64 + lane] = i < nk ? (double)levels[rank_lohi(cur.lo, cur.hi)] : 0.0;
+    mom_load_group(d, 0, lane, nxt);
+    for (int base = 0; base < n_max; base += kMomSpan) {
+      MomLoad cur[kMomGroup];
+#pragma unroll
+      for (int g = 0; g < kMomGroup; ++g) cur[g] = nxt[g];
+      mom_load_group(d, base + kMomSpan, lane, nxt);
+#pragma unroll
+      for (int g = 0; g < kMomGroup; ++g) {
+        const int i = base + 64 * g + lane;
+        st[64 * g + lane] = i < n_e ? (double)cur[g].x : 0.0;
+        st[kMomSpan + 64 * g + lane] =
+            i < nk ? (double)levels[rank_lohi(cur[g].lo, cur[g].hi)] : 0.0;
+      }
       wave_lds_sync();
-      serial_sum64(st + ((lane & 1) ? 64 : 0), acc);
+      serial_sum_span(st + ((lane & 1) ? kMomSpan : 0), acc);
       wave_lds_sync();
     }
     if (!ex_ok) ev_sum = readlane_f64(acc, 0);
@@ -747,24 +767,29 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
   const double shift = ev_sum / n_e - km_sum / nk;
 
   double acc = 0.0;
-  mom_load(d, lane, nxt);
-  for (int base = 0; base < n_max; base += 64) {
-    const MomLoad cur = nxt;
-    mom_load(d, base + 64 + lane, nxt);
-    const int i = base + lane;
-    double l2 = 0.0, d2 = 0.0;
-    if (i < nk) {
-      const double l = levels[rank_lohi(cur.lo, cur.hi)];
-      l2 = l * l;
+  mom_load_group(d, 0, lane, nxt);
+  for (int base = 0; base < n_max; base += kMomSpan) {
+    MomLoad cur[kMomGroup];
+#pragma unroll
+    for (int g = 0; g < kMomGroup; ++g) cur[g] = nxt[g];
+    mom_load_group(d, base + kMomSpan, lane, nxt);
+#pragma unroll
+    for (int g = 0; g < kMomGroup; ++g) {
+      const int i = base + 64 * g + lane;
+      double l2 = 0.0, d2 = 0.0;
+      if (i < nk) {
+        const double l = levels[rank_lohi(cur[g].lo, cur[g].hi)];
+        l2 = l * l;
+      }
+      if (i < n_e) {
+        const double dx = (double)cur[g].x - shift;
+        d2 = dx * dx;
+      }
+      st[64 * g + lane] = l2;
+      st[kMomSpan + 64 * g + lane] = d2;
     }
-    if (i < n_e) {
-      const double dx = (double)cur.x - shift;
-      d2 = dx * dx;
-    }
-    st[lane] = l2;
-    st[64 + lane] = d2;
     wave_lds_sync();
-    serial_sum64(st + ((lane & 1) ? 64 : 0), acc);
+    serial_sum_span(st + ((lane & 1) ? kMomSpan : 0), acc);
     wave_lds_sync();
   }
   const double km_sq = readlane_f64(acc, 0), ev_sq = readlane_f64(acc, 1);
