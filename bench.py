@@ -101,6 +101,28 @@ def best_ms(fn, torch, reps=3):
     return 1e3 * min(ts)
 
 
+def hbm_copy_gbs(torch, gib=2, reps=5):
+    """The box's own HBM rate: a device-to-device copy of `gib` GiB (read
+    once, written once), best of `reps`, hipEvent-timed on torch's stream.
+    The vendor figure (8 TB/s) stays the roofline's peak; this says what a
+    plain copy reaches on the same box."""
+    n = (gib << 30) // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = 1e30
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(2 * (gib << 30) / (best * 1e-3) / 1e9, 1)
+
+
 # --------------------------------------------------------------------- legs
 def headline_leg(args, ctx):
     import numpy as np
@@ -199,7 +221,8 @@ def headline_leg(args, ctx):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": prof["traffic"] if prof else None,
-                     "alg_bytes_per_launch": core_bytes},
+                     "alg_bytes_per_launch": core_bytes,
+                     "box_copy_gbs": None if args.profile else hbm_copy_gbs(torch)},
     }
     if prof:
         # VALU issue rate against the ceiling of one wave instruction per SIMD

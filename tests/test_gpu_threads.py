@@ -69,6 +69,59 @@ def test_align_from_threads_over_batches():
     assert free[-1] >= free[0] - (64 << 20), free
 
 
+def test_gathered_calls_with_two_models():
+    """Concurrent align() calls are gathered into batches of one model each:
+    threads that pass two different model arrays (equal contents) all get
+    the reference's pairs, and calls with a model whose contents differ get
+    that model's results (the oracle's)."""
+    _need_gpu()
+    import oracle
+    from abea_amd import _lib, load_golden_subset
+    L = _lib.lib()
+    b, golden, _ = load_golden_subset(GOLDEN)
+    model_a = np.ascontiguousarray(b.model, np.float32)
+    model_b = model_a.copy()
+    # a third model with shifted levels: its reads align differently
+    model_c = model_a.copy()
+    model_c[:, 0] += 1.5
+    aos = b.events_aos()
+    n = 6
+    seqs = [b.read_seq(i) for i in range(n)]
+    evs = [np.ascontiguousarray(aos[b.ev_ptr[i]:b.ev_ptr[i] + b.n_events[i]]) for i in range(n)]
+    want_c = []
+    for i in range(n):
+        r = oracle.align(seqs[i], b.ev_mean[b.ev_ptr[i]:b.ev_ptr[i] + b.n_events[i]], model_c,
+                         float(b.scale[i]), float(b.shift[i]))
+        want_c.append(r["pairs"])
+    errors = []
+
+    def worker(t):
+        try:
+            model = (model_a, model_b, model_c)[t % 3]
+            for j in range(2 * n):
+                i = (t + j) % n
+                ev = evs[i]
+                et = _lib.EventTable(ev.shape[0], 0, ev.shape[0],
+                                     ev.ctypes.data_as(C.POINTER(_lib.EventT)))
+                sc = _lib.ScalingsT(float(b.scale[i]), float(b.shift[i]), 0.0, 0.0)
+                out = np.zeros((2 * ev.shape[0], 2), np.int32)
+                got = L.abea_align(out.ctypes.data_as(C.c_void_p), seqs[i], len(seqs[i]), et,
+                                   model.ctypes.data_as(C.c_void_p), sc, 4000.0)
+                want = want_c[i] if t % 3 == 2 else golden[i]
+                if got != want.shape[0] or not np.array_equal(out[:got], want):
+                    errors.append((t, i, got, want.shape[0], L.abea_last_error()))
+        except Exception as e:  # noqa: BLE001
+            errors.append((t, repr(e)))
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(9)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert errors == [], errors[:4]
+    assert any(not np.array_equal(want_c[i], golden[i]) for i in range(n))
+
+
 def test_align_db_phase_times():
     """abea_host_last_times() reports the phases of the last align_cuda()
     drop-in call, as the reference's host prints them."""
