@@ -1483,7 +1483,22 @@ __global__ void abea_adjoint_finish_kernel(const unsigned long long *partial, in
                                            const double *special, double *grad_model) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kAdjBins; i += gridDim.x * blockDim.x) {
     unsigned long long h = 0ull, l = 0ull;
-    for (int b = 0; b < n_blocks; ++b) {
+    int b = 0;
+    // eight slices' loads in flight at a time (integer sums: any order)
+    for (; b + 8 <= n_blocks; b += 8) {
+      unsigned long long hv[8], lv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        hv[j] = partial[(size_t)(b + j) * 2 * kAdjBins + i];
+        lv[j] = partial[(size_t)(b + j) * 2 * kAdjBins + kAdjBins + i];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        h += hv[j];
+        l += lv[j];
+      }
+    }
+    for (; b < n_blocks; ++b) {
       h += partial[(size_t)b * 2 * kAdjBins + i];
       l += partial[(size_t)b * 2 * kAdjBins + kAdjBins + i];
     }
@@ -1682,8 +1697,9 @@ hipError_t launch_adjoint(const KernelArgs &a, const float *grad_score, float *g
   hipLaunchKernelGGL(abea_adjoint_clear_kernel, dim3(48), dim3(256), 0, s, special);
   hipLaunchKernelGGL(abea_adjoint_kernel, dim3(n_blocks), dim3(64 * wpb), 0, s, a, grad_score,
                      grad_scalings, partial, special, n);
-  hipLaunchKernelGGL(abea_adjoint_finish_kernel, dim3(48), dim3(256), 0, s, partial, n_blocks,
-                     special, grad_model);
+  // one 64-thread block per 64 bins: the sums spread over 192 CUs
+  hipLaunchKernelGGL(abea_adjoint_finish_kernel, dim3(kAdjBins / 64), dim3(64), 0, s, partial,
+                     n_blocks, special, grad_model);
   return hipGetLastError();
 }
 
