@@ -158,7 +158,9 @@ def headline_leg(args, ctx):
                         traceback_overlap=ovl)
     elif not args.profile:
         t = time.perf_counter()
-        plan.autotune(reps=1)
+        # two forwards per candidate: the shapes differ by 1-3%, about the
+        # noise of a single forward
+        plan.autotune(reps=2)
         log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s")
     tune = plan.tuning()
     log(f"[bench] tuning {tune}")
