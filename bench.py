@@ -47,6 +47,7 @@ W = 100
 # (kernel, waves per CU, traceback overlap).
 PROFILED = {
     ("abea_core_kernel<1, true>", 16, 128): {"traffic": 46567172736, "valu": 2.0855e10},
+    ("abea_core_kernel<1, true>", 20, 128): {"traffic": 49099957120, "valu": 2.0855e10},
     ("abea_core_kernel<1, true>", 24, 128): {"traffic": 49144735264, "valu": 2.0855e10},
     ("abea_core_kernel<1, true>", 32, 128): {"traffic": 49049624960, "valu": 2.0855e10},
 }
