@@ -344,7 +344,8 @@ abea_status_t abea_plan_set_replay(abea_plan_t *plan, const uint32_t *moves_host
  *                             level_log_stdv = log(level_stdv) the total
  *                             derivative is column 1 + column 2 / level_stdv.
  * Deterministic: the k-mer sums are exact fixed-point sums of the per-run
- * terms (integer atomics), so repeated calls give identical bits. */
+ * terms (integer adds, whose order does not matter), so repeated calls give
+ * identical bits. */
 abea_status_t abea_plan_adjoint(abea_plan_t *plan, const abea_output_t *fwd,
                                 const float *grad_score, float *grad_scalings,
                                 double *grad_model);
