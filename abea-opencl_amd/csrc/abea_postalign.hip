@@ -112,6 +112,22 @@ __device__ __forceinline__ int add_terms(Stage &st, bool m, const Terms &t, Norm
   return cnt;
 }
 
+// acc += v[0] + v[S] + v[2S] + ... over n entries (n <= 64), one add at a
+// time in order, the LDS loads of 16 entries issued ahead of their adds.
+// Entries n .. (n rounded up to 16) - 1 must hold +0.0, which leaves acc
+// unchanged: acc starts at +0 and is never -0 (a sum that cancels exactly
+// rounds to +0).
+template <int S>
+__device__ __forceinline__ void serial_sum_lds(const double *v, int n, double &acc) {
+  for (int j0 = 0; j0 < n; j0 += 16) {
+    double t[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t[j] = v[S * (j0 + j)];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc += t[j];
+  }
+}
+
 // One running sum over the lanes with m set, in lane order.
 __device__ __forceinline__ void add_in_order(Stage &st, bool m, double t, double &acc) {
   const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
@@ -413,22 +429,30 @@ __device__ __forceinline__ Normal normal_sums(Stage &st, const float4 *T, int k0
   double acc = 0.0;
   const int c = lane < 5 ? lane : 4;
   n_m = 0;
-  float4 nx = T[min(k0 + lane, kN)];
+  // the entries two chunks ahead load while this chunk is summed
+  float4 nx0 = T[min(k0 + lane, kN)], nx1 = T[min(k0 + 64 + lane, kN)];
   for (int base = k0; base <= kN; base += 64) {
-    const float4 q = nx;
-    nx = T[min(base + 64 + lane, kN)];
+    const float4 q = nx0;
+    nx0 = nx1;
+    nx1 = T[min(base + 128 + lane, kN)];
     const bool mm = base + lane <= kN && q.w != 0.f;
     const uint64_t mask = __builtin_amdgcn_ballot_w64(mm);
+    const int cnt = __builtin_popcountll(mask);
     const Terms t = normal_terms(q.x, q.y, q.z);
-    if (mm) {
+    if (lane >= cnt) {  // padding for the 16-entry reads (positions cnt..63)
+      double *z = st.t + kStage * lane;
+      *reinterpret_cast<double2 *>(z) = make_double2(0.0, 0.0);
+      *reinterpret_cast<double2 *>(z + 2) = make_double2(0.0, 0.0);
+      z[4] = 0.0;
+    }
+    if (mm) {  // the entries, compacted to positions 0..cnt-1
       double *w = st.t + kStage * lanes_below(mask);
       *reinterpret_cast<double2 *>(w) = make_double2(t.w, t.mw);
       *reinterpret_cast<double2 *>(w + 2) = make_double2(t.mmw, t.xw);
       w[4] = t.mxw;
     }
     wave_lds_sync();
-    const int cnt = __builtin_popcountll(mask);
-    for (int j = 0; j < cnt; ++j) acc += st.t[kStage * j + c];
+    serial_sum_lds<kStage>(st.t + c, cnt, acc);
     wave_lds_sync();
     n_m += cnt;
   }
@@ -445,17 +469,19 @@ __device__ __forceinline__ Normal normal_sums(Stage &st, const float4 *T, int k0
 __device__ __forceinline__ double var_sum(Stage &st, const float4 *T, int k0, int kN, int lane,
                                           double shift, double scale) {
   double acc = 0.0;
-  float4 nx = T[min(k0 + lane, kN)];
+  float4 nx0 = T[min(k0 + lane, kN)], nx1 = T[min(k0 + 64 + lane, kN)];
   for (int base = k0; base <= kN; base += 64) {
-    const float4 q = nx;
-    nx = T[min(base + 64 + lane, kN)];
+    const float4 q = nx0;
+    nx0 = nx1;
+    nx1 = T[min(base + 128 + lane, kN)];
     const bool mm = base + lane <= kN && q.w != 0.f;
     const uint64_t mask = __builtin_amdgcn_ballot_w64(mm);
-    const double t = var_term(q.x, q.y, q.z, shift, scale);
-    if (mm) st.t[lanes_below(mask)] = t;
-    wave_lds_sync();
     const int cnt = __builtin_popcountll(mask);
-    for (int j = 0; j < cnt; ++j) acc += st.t[j];
+    const double t = var_term(q.x, q.y, q.z, shift, scale);
+    if (lane >= cnt) st.t[lane] = 0.0;  // padding for the 16-entry reads
+    if (mm) st.t[lanes_below(mask)] = t;  // the entries, compacted
+    wave_lds_sync();
+    serial_sum_lds<1>(st.t, cnt, acc);
     wave_lds_sync();
   }
   return acc;
