@@ -252,6 +252,70 @@ def test_drop_in_align_and_align_db(ctx):
         libc.free(C.cast(outp[i], C.c_void_p))
 
 
+def test_drop_ins_on_edge_reads(ctx):
+    """The host drop-ins on reads that align to nothing and on an empty
+    batch: reads shorter than 6 bases, reads with no events, one base per
+    k-mer short of a 6-mer, mixed with golden reads in one align_cuda()
+    batch; every result equal to the oracle's (0 pairs for the degenerate
+    reads), and a batch of no reads succeeds."""
+    import oracle
+    from abea_amd import _lib
+    b, golden, _ = golden_batch()
+    L = _lib.lib()
+    model = np.ascontiguousarray(b.model, np.float32)
+    aos = b.events_aos()
+    rng = np.random.default_rng(5)
+    cases = []  # (seq, events AoS)
+    for i in range(3):
+        cases.append((b.read_seq(i), np.ascontiguousarray(aos[b.ev_ptr[i]:b.ev_ptr[i] + b.n_events[i]])))
+    some_ev = cases[0][1][:40].copy()
+    cases += [(b"ACGTA", some_ev), (b"", some_ev), (b"ACGTACGTAC" * 30, some_ev[:0]),
+              (b"ACGTAC", some_ev[:1]), (rng.choice(list(b"ACGT"), 300).astype(np.uint8).tobytes(),
+                                         some_ev[:2])]
+    n = len(cases)
+    want = []
+    for seq, ev in cases:
+        if len(seq) >= 6 and ev.shape[0] > 0:
+            r = oracle.align(seq, ev["mean"].astype(np.float32), model, float(b.scale[0]),
+                             float(b.shift[0]))
+            want.append(r["pairs"] if r["n_pairs"] <= 2 * ev.shape[0] else np.zeros((0, 2), np.int32))
+        else:
+            want.append(np.zeros((0, 2), np.int32))
+    keep = []
+    reads, ets = (C.c_char_p * n)(), (_lib.EventTable * n)()
+    sc = _lib.ScalingsT(float(b.scale[0]), float(b.shift[0]), 0.0, 0.0)
+    for i, (seq, ev) in enumerate(cases):
+        ev = np.ascontiguousarray(ev)
+        keep += [seq, ev]
+        et = _lib.EventTable(ev.shape[0], 0, ev.shape[0], ev.ctypes.data_as(C.POINTER(_lib.EventT)))
+        out = np.zeros((2 * ev.shape[0] + 1, 2), np.int32)
+        got = L.abea_align(out.ctypes.data_as(C.c_void_p), seq, len(seq), et,
+                           model.ctypes.data_as(C.c_void_p), sc, 4000.0)
+        assert got == want[i].shape[0], (i, got, L.abea_last_error())
+        assert np.array_equal(out[:got], want[i]), i
+        reads[i] = seq
+        ets[i] = et
+    rl = np.array([len(c[0]) for c in cases], np.int32)
+    scal = np.tile(np.array([b.scale[0], b.shift[0], 0, 0], np.float32), (n, 1))
+    outp = (C.POINTER(_lib.AlignedPair) * n)()
+    nout = np.zeros(n, np.int32)
+    st = L.abea_align_db(n, C.cast(reads, C.c_void_p), rl.ctypes.data_as(C.c_void_p),
+                         C.cast(ets, C.c_void_p), model.ctypes.data_as(C.c_void_p),
+                         scal.ctypes.data_as(C.c_void_p), C.cast(outp, C.c_void_p),
+                         nout.ctypes.data_as(C.c_void_p))
+    assert st == 0, L.abea_last_error()
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free.restype = None
+    for i in range(n):
+        assert nout[i] == want[i].shape[0], (i, nout[i])
+        got = np.ctypeslib.as_array(C.cast(outp[i], C.POINTER(C.c_int32)), (int(nout[i]) * 2,))
+        assert np.array_equal(got.reshape(-1, 2), want[i]), i
+        libc.free(C.cast(outp[i], C.c_void_p))
+    st = L.abea_align_db(0, None, None, None, model.ctypes.data_as(C.c_void_p), None, None, None)
+    assert st == 0, L.abea_last_error()
+
+
 def test_adjoint_matches_fp64_path_gradient(ctx):
     """GPU adjoint vs the oracle's FP64 partial derivatives along the same
     paths, and bitwise equal on a second call (fixed-point k-mer sums)."""
