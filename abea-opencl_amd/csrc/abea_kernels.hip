@@ -83,18 +83,60 @@ __device__ __forceinline__ int32_t order_key(uint32_t u) {
 
 constexpr uint32_t kNegInfBits = 0xff800000u;
 
-// 0/1 results of uniform compares, computed on the scalar unit.  Written out
-// so that the bit stays a scalar integer: a C++ bool of a uniform compare is
-// otherwise carried as a lane mask and turned back into an integer with two
-// vector instructions.
-__device__ __forceinline__ uint32_t s_lt_i32(int32_t a, int32_t b) {
-  uint32_t r;
-  asm("s_cmp_lt_i32 %1, %2\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(a), "s"(b) : "scc");
-  return r;
+// The band's two end slots, K (lower left) and K+99 (upper right), are one
+// even and one odd slot: the even one sits in the parity-0 register at lane
+// x0, the odd one in the parity-1 register at lane x1; np = 1 - (K & 1).
+// Lane numbers are used mod 64 (the lane-select operand's low six bits).
+struct EndLanes {
+  int x0, x1, np;
+};
+__device__ __forceinline__ EndLanes end_lanes(int K) {
+  const int p = K & 1, h = K >> 1;
+  return EndLanes{p ? h + 50 : h, p ? h : h + 49, 1 - p};
 }
-__device__ __forceinline__ uint32_t s_eq_u32(uint32_t a, uint32_t b) {
-  uint32_t r;
-  asm("s_cmp_eq_u32 %1, %2\n\ts_cselect_b32 %0, 1, 0" : "=s"(r) : "s"(a), "s"(b) : "scc");
+
+// Suzuki's rule (GPU/align_cuda.cl:398-410) for a band whose cells are all
+// <= 0 or -inf (no NaN, no -0): there x < y exactly when bits(x) > bits(y)
+// as unsigned integers, -inf having the largest bits.  a0 / a1 are the bits
+// of the end cells in the parity-0 / parity-1 registers; ll is a1 when K is
+// odd.  On odd bands both ends at -inf move right: the upper-right bits are
+// capped just below -inf's, so ll = -inf compares greater exactly when ur is
+// -inf too or finite.  Returns the move (1 = right) and advances the lanes:
+// a right move makes K's parity flip, x0 <- x1 + 1 and x1 <- x0 (with
+// h = K >> 1: even K has (x0, x1) = (h, h+49), odd K (h+50, h)).
+template <bool ODD>
+__device__ __forceinline__ uint32_t move_step(EndLanes &el, uint32_t a0, uint32_t a1) {
+  const int t = el.x1 + 1;
+  uint32_t r, ll, ur;
+  int x0n, x1n;
+  if (ODD)
+    asm("s_cmp_eq_u32 %[np], 0\n\t"
+        "s_cselect_b32 %[ll], %[a1], %[a0]\n\t"
+        "s_cselect_b32 %[ur], %[a0], %[a1]\n\t"
+        "s_min_u32 %[ur], %[ur], 0xff7fffff\n\t"
+        "s_cmp_gt_u32 %[ll], %[ur]\n\t"
+        "s_cselect_b32 %[x0n], %[t], %[x0]\n\t"
+        "s_cselect_b32 %[x1n], %[x0], %[x1]\n\t"
+        "s_cselect_b32 %[r], 1, 0"
+        : [r] "=&s"(r), [ll] "=&s"(ll), [ur] "=&s"(ur), [x0n] "=&s"(x0n), [x1n] "=&s"(x1n)
+        : [np] "s"(el.np), [a0] "s"(a0), [a1] "s"(a1), [t] "s"(t), [x0] "s"(el.x0),
+          [x1] "s"(el.x1)
+        : "scc");
+  else
+    asm("s_cmp_eq_u32 %[np], 0\n\t"
+        "s_cselect_b32 %[ll], %[a1], %[a0]\n\t"
+        "s_cselect_b32 %[ur], %[a0], %[a1]\n\t"
+        "s_cmp_gt_u32 %[ll], %[ur]\n\t"
+        "s_cselect_b32 %[x0n], %[t], %[x0]\n\t"
+        "s_cselect_b32 %[x1n], %[x0], %[x1]\n\t"
+        "s_cselect_b32 %[r], 1, 0"
+        : [r] "=&s"(r), [ll] "=&s"(ll), [ur] "=&s"(ur), [x0n] "=&s"(x0n), [x1n] "=&s"(x1n)
+        : [np] "s"(el.np), [a0] "s"(a0), [a1] "s"(a1), [t] "s"(t), [x0] "s"(el.x0),
+          [x1] "s"(el.x1)
+        : "scc");
+  el.x0 = x0n;
+  el.x1 = x1n;
+  el.np ^= (int)r;
   return r;
 }
 
@@ -196,12 +238,19 @@ __device__ __forceinline__ uint32_t shift_in4(uint32_t acc, uint64_t u0, uint64_
 // indices) before any of the round's stores.  The wave's later loads see the
 // stores without a cache operation: a CU's vector L1 stays coherent with its
 // own stores, and the walk reads the staged arrays through vector loads only.
-__device__ __forceinline__ void stage_read(const KernelArgs &a, const ReadMeta &m, int r,
+//
+// Returns whether every emission of the read is <= 0 (every staged
+// -0.918938f - log_stdv is): then, with transition log-probabilities <= 0,
+// every band cell is <= 0 or -inf, which the fast tiles' move decision
+// relies on.  A read with a k-mer whose emission can be positive (a model
+// stdv below 0.399) walks every band by the general band step.
+__device__ __forceinline__ bool stage_read(const KernelArgs &a, const ReadMeta &m, int r,
                                            int lane) {
   const char *S = a.seq + a.seq_ptr[r];
   const abea_scalings_t sc = a.scalings[r];
   KmerParam *km = a.km + m.km_off;
   uint16_t *kr = a.km_rank + m.km_off;
+  bool pos = false;  // a staged constant that is not <= 0 (positive, or NaN)
   for (int k0 = 0; k0 < m.n_k; k0 += 256) {
     uint32_t rank[4];
 #pragma unroll
@@ -222,6 +271,7 @@ __device__ __forceinline__ void stage_read(const KernelArgs &a, const ReadMeta &
         p.stdv = md[u].level_stdv;                        // scaling.var unused, GPU/align_cuda.cl:515
         p.rstdv = 1.0f / md[u].level_stdv;                // IEEE (correctly rounded) divide
         p.c = -0.918938f - md[u].level_log_stdv;          // cached log, GPU/align_cuda.cl:517-518
+        pos = pos || !(p.c <= 0.0f);
         km[k] = p;
         kr[k] = (uint16_t)rank[u];
       }
@@ -252,6 +302,7 @@ __device__ __forceinline__ void stage_read(const KernelArgs &a, const ReadMeta &
       if (e0 + 64 * u + lane < m.n_e) ev[e0 + 64 * u + lane] = v[u];
   }
   wave_sync();
+  return __builtin_amdgcn_ballot_w64(pos) == 0;
 }
 
 // The persistent core's queue head, cleared before each batched launch.
@@ -381,18 +432,16 @@ __device__ __forceinline__ void band_ends(const Slots &s, int h, uint32_t &llb, 
   urb = lane_bits(P ? s.M0 : s.M1, (h + 49 + P) & 63);
 }
 
-// Lane of slot k (k mod 128, two slots per lane).
-__device__ __forceinline__ int slot_lane(int k) { return (k >> 1) & 63; }
-
-// One interior band: every window cell is inside the matrix, no trim cell,
-// the last k-mer is not in the window.  Branch-free; the scalar state is the
-// lower-left k-mer K, the parity of K only selecting registers and lanes.
-// The events and k-mer parameters under the slots come from the tile's
-// gathered operands (tile_fast).
+// One band of a fast tile (band_fast): branch-free; the scalar state is the
+// two end lanes and the parity of K (EndLanes), plus K itself in edge tiles.
+// Interior bands have every window cell inside the matrix, no trim cell and
+// the last k-mer outside the window.  The events and k-mer parameters under
+// the slots come from the tile's gathered operands (tile_fast).
 // Masking: an in-window cell reads only in-window cells of the previous two
 // bands or the two slots just outside the window (K-1 and K+100), so those
 // two are set to -inf and the other out-of-window slots are left as they are
 // (never read by an in-window cell).
+//
 // Per-read values an edge tile needs.
 struct EdgeConst {
   int n_e, n_k, js;   // js: slot of the last k-mer (the end-search column)
@@ -401,26 +450,19 @@ struct EdgeConst {
 };
 
 template <bool ODD, bool EDGE>
-__device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i, int b, float ev0,
-                                          float ev1, const KmerParam &k0, const KmerParam &k1,
-                                          double lp_step, double lp_stay, double lp_skip,
-                                          const EdgeConst &ec, float &best,
+__device__ __forceinline__ void band_fast(Slots &s, int &K, EndLanes &el, uint32_t &gm, int b,
+                                          float ev0, float ev1, const KmerParam &k0,
+                                          const KmerParam &k1, double lp_step, double lp_stay,
+                                          double lp_skip, const EdgeConst &ec, float &best,
                                           int &best_e) {
-  // Suzuki's rule (GPU/align_cuda.cl:398-410) on the scalar unit: slot K
-  // (lane h, parity p) and slot K+99 (lane lu, the other parity) are read
-  // out of the two registers with lane indices chosen by p, then compared
-  // as the reference's floats through their order keys.
-  const int p = K & 1, h = slot_lane(K), lu = slot_lane(K + 99);
-  const uint32_t a0 = lane_bits(s.M0, p ? lu : h);
-  const uint32_t a1 = lane_bits(s.M1, p ? h : lu);
-  const uint32_t llb = p ? a1 : a0, urb = p ? a0 : a1;
-  uint32_t r = s_lt_i32(order_key(llb), order_key(urb));
-  if (ODD) {
-    // both ends outside the matrix: right on odd bands
-    r |= s_eq_u32(llb, kNegInfBits) & s_eq_u32(urb, kNegInfBits);
-  }
-  K += (int)r;
-  gm |= r << i;
+  // Suzuki's rule (GPU/align_cuda.cl:398-410) on the scalar unit: the end
+  // cells come out of the two registers at the lanes el holds, and the move
+  // is decided on their bits (move_step).  gm collects the tile's moves,
+  // first band in the highest bit; interior tiles recover K from them.
+  const uint32_t r =
+      move_step<ODD>(el, lane_bits(s.M0, el.x0 & 63), lane_bits(s.M1, el.x1 & 63));
+  if (EDGE) K += (int)r;
+  gm = (gm << 1) + r;
   Slots t = s;
   t.ev0 = ev0;
   t.ev1 = ev1;
@@ -433,11 +475,12 @@ __device__ __forceinline__ void band_fast(Slots &s, int &K, uint32_t &gm, int i,
   t.km1r = k1.rstdv;
   t.km1c = k1.c;
   const Cells c = compute_cells(t, lp_step, lp_stay, lp_skip);
-  const int p2 = K & 1;
-  const int lb = slot_lane(K - 1), lt = slot_lane(K + 100);
+  // slots K-1 and K+100 of the new K: for even K, K+100 in the parity-0
+  // register at lane x1 + 1 and K-1 in the parity-1 register at lane x0 - 1;
+  // for odd K, K-1 at lane x1 and K+100 at lane x0
   const float ninf = ABEA_NEG_INF;
-  float M0v = put_lane(c.M0, ninf, p2 ? lb : lt);
-  float M1 = put_lane(c.M1, ninf, p2 ? lt : lb);
+  float M0v = put_lane(c.M0, ninf, (el.x1 + el.np) & 63);
+  float M1 = put_lane(c.M1, ninf, (el.x0 - el.np) & 63);
   uint64_t u1 = c.u1, l1 = c.l1;
   if (EDGE) {
     // cell (e, n_k) lies outside the matrix, but its skip transition reads
@@ -481,7 +524,7 @@ struct ReadConst {
   double lp_stay, lp_step, lp_skip, lp_trim;
   const float *evp;      // the read's staged events
   const KmerParam *kmp;  // the read's staged k-mer records
-  uint32_t *trace;  // the read's tiles, offset by the lane
+  uint32_t *trace;  // the read's tiles (lane l's word of tile t at t * kTileWords + l)
   uint32_t *moves;
   int32_t *kck;
   c_u32 *replay;
@@ -544,7 +587,7 @@ __device__ __forceinline__ void band_general(Slots &s, Walk &w, const ReadConst 
   // the trim cell's from-code is U: bits (u1, l1) = (1, 0) in lane 63
   if (trim && lane == 63) s.acc = (s.acc & ~3u) | 2u;
   if ((b & 7) == 7) {
-    rc.trace[(b >> 3) * kTileWords] = s.acc;
+    rc.trace[(b >> 3) * kTileWords + lane] = s.acc;
     s.acc = 0;
   }
   if ((b & 31) == 31) {
@@ -583,35 +626,56 @@ struct TileOps {
 // [E0 - 126, E0 + 16] and k-mers K0 + off in [K0, K0 + 127].  Each read's
 // staged arrays carry kStagePad = 128 padding entries on both sides, so fast
 // tiles may start at -120 <= K0 <= n_k and 0 <= E0 <= n_e + 111.
+// A scalar value the compiler takes as computed: keeps a uniform sum from
+// being split into per-lane vector adds.
+__device__ __forceinline__ uint32_t s_opaque(uint32_t v) {
+  uint32_t r;
+  asm("s_mov_b32 %0, %1" : "=s"(r) : "s"(v));
+  return r;
+}
 struct TileAddr {
-  const float *ep0, *ep1;        // event of slot 2l / 2l+1 on the tile's first band
-  const KmerParam *kq0, *kq1;    // k-mer of slot 2l / 2l+1
+  const char *evq;    // the read's staged events, front padding included
+  const char *kq;     // k-mer record K0
+  uint32_t eo0, eo1;  // byte offset from evq of the event of slot 2l / 2l+1 on the tile's first band
+  uint32_t ko0, ko1;  // byte offset from kq of the k-mer of slot 2l / 2l+1
 };
 // Byte offsets are taken from the start of the read's front padding, so that
-// they stay non-negative: event b0 - 2 - K0 - off >= E0 - 126 >= -126.
+// they stay non-negative: event b0 - 2 - K0 - off >= E0 - 126 >= -126.  The
+// bases are wave-uniform and the offsets 32-bit, so the loads take a scalar
+// base and a vector offset.
 __device__ __forceinline__ TileAddr tile_addr(const ReadConst &rc, int b0, int K0, int lane) {
-  const uint32_t off0 = (uint32_t)(2 * lane - K0) & 127u, off1 = (uint32_t)(2 * lane + 1 - K0) & 127u;
-  const uint32_t eb = (uint32_t)(b0 - 2 - K0 + kStagePad);  // >= off + 2
-  const char *evq = (const char *)(rc.evp - kStagePad);
+  // 4 (j - K0) mod 512 for slot j = 2l and 2l + 1
+  const uint32_t q0 = (uint32_t)(8 * lane - 4 * K0) & 0x1fcu;
+  const uint32_t q1 = (q0 + 4u) & 0x1fcu;
+  const uint32_t eb4 = s_opaque(4u * (uint32_t)(b0 - 2 - K0 + kStagePad));  // >= q + 8
   TileAddr t;
-  t.ep0 = (const float *)(evq + 4u * (eb - off0));
-  t.ep1 = (const float *)(evq + 4u * (eb - off1));
-  t.kq0 = (const KmerParam *)((const char *)(rc.kmp + K0) + 16u * off0);
-  t.kq1 = (const KmerParam *)((const char *)(rc.kmp + K0) + 16u * off1);
+  t.evq = (const char *)(rc.evp - kStagePad);
+  t.kq = (const char *)(rc.kmp + K0);
+  t.eo0 = eb4 - q0;
+  t.eo1 = eb4 - q1;
+  t.ko0 = 4u * q0;
+  t.ko1 = 4u * q1;
   return t;
+}
+__device__ __forceinline__ const float *ev_at(const char *base, uint32_t off) {
+  return (const float *)(base + off);
+}
+__device__ __forceinline__ KmerParam ld_km(const char *base, uint32_t off) {
+  return *(const KmerParam *)(base + off);
 }
 
 // The first tile of a run gathers its own operands.
 __device__ __forceinline__ void tile_gather(TileOps &T, const Walk &w, const ReadConst &rc, int b0,
                                             int lane) {
   const TileAddr t = tile_addr(rc, b0, walk_k(w), lane);
+  const float *ep0 = ev_at(t.evq, t.eo0), *ep1 = ev_at(t.evq, t.eo1);
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    T.e0[i] = t.ep0[i];
-    T.e1[i] = t.ep1[i];
+    T.e0[i] = ep0[i];
+    T.e1[i] = ep1[i];
   }
-  T.k0 = *t.kq0;
-  T.k1 = *t.kq1;
+  T.k0 = ld_km(t.kq, t.ko0);
+  T.k1 = ld_km(t.kq, t.ko1);
 }
 
 // Edge tiles (EDGE) also cover bands whose window reaches outside the matrix
@@ -630,10 +694,11 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
   const int K0 = walk_k(w);
   int K = K0;
   uint32_t gm = 0;
+  EndLanes el = end_lanes(K0);
   const TileAddr t = tile_addr(rc, b0, K0, lane);
-  const float *ep0 = t.ep0, *ep1 = t.ep1;
-  N.k0 = *t.kq0;
-  N.k1 = *t.kq1;
+  const float *ep0 = ev_at(t.evq, t.eo0), *ep1 = ev_at(t.evq, t.eo1);
+  N.k0 = ld_km(t.kq, t.ko0);
+  N.k1 = ld_km(t.kq, t.ko1);
   // slot n_k & 127 holds k-mer n_k through the tile iff K0 >= n_k - 127
   const EdgeConst ec{rc.n_e, rc.n_k, rc.js, K0 >= rc.n_k - 127 ? (rc.n_k & 127) : -1,
                      rc.lp_trim};
@@ -641,11 +706,11 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
   int best_e = w.best_e;
 #pragma unroll
   for (int i = 0; i < 8; i += 2) {
-    band_fast<false, EDGE>(s, K, gm, i, b0 + i, T.e0[i], T.e1[i], T.k0, T.k1, rc.lp_step,
+    band_fast<false, EDGE>(s, K, el, gm, b0 + i, T.e0[i], T.e1[i], T.k0, T.k1, rc.lp_step,
                            rc.lp_stay, rc.lp_skip, ec, best, best_e);
     N.e0[i] = ep0[8 + i];
     N.e1[i] = ep1[8 + i];
-    band_fast<true, EDGE>(s, K, gm, i + 1, b0 + i + 1, T.e0[i + 1], T.e1[i + 1], T.k0, T.k1,
+    band_fast<true, EDGE>(s, K, el, gm, b0 + i + 1, T.e0[i + 1], T.e1[i + 1], T.k0, T.k1,
                           rc.lp_step, rc.lp_stay, rc.lp_skip, ec, best, best_e);
     N.e0[i + 1] = ep0[9 + i];
     N.e1[i + 1] = ep1[9 + i];
@@ -653,11 +718,13 @@ __device__ __forceinline__ void tile_fast(Slots &s, Walk &w, const ReadConst &rc
   if (EDGE) {
     w.best = best;
     w.best_e = best_e;
+  } else {
+    K = K0 + __builtin_popcount(gm);
   }
   w.koff = 16u * (uint32_t)(K + 128);
   w.eoff = 4u * (uint32_t)(b0 + 6 - K);  // E + 1 = b - 1 - K after band b = b0+7
-  w.mv |= gm << (b0 & 31);
-  rc.trace[(b0 >> 3) * kTileWords] = s.acc;
+  w.mv |= (__builtin_bitreverse32(gm) >> 24) << (b0 & 31);  // gm: band b0 in bit 7
+  (rc.trace + (b0 >> 3) * kTileWords)[(uint32_t)lane] = s.acc;
   s.acc = 0;
   if ((b0 & 31) == 24) {
     rc.moves[b0 >> 5] = w.mv;  // same value from every lane
@@ -721,7 +788,7 @@ __device__ __forceinline__ void restore_slots(Slots &s, Walk &w, const ReadConst
 }
 
 __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m, int r,
-                                          int lane) {
+                                          int lane, bool nonpos) {
   ReadConst rc;
   rc.n_e = m.n_e;
   rc.n_k = m.n_k;
@@ -732,7 +799,7 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   rc.lp_trim = a.lp_trim;
   rc.evp = a.ev + m.sev_off;
   rc.kmp = a.km + m.km_off;
-  rc.trace = a.trace + m.trace_off + lane;
+  rc.trace = a.trace + m.trace_off;
   rc.moves = a.moves + m.mv_off;
   rc.kck = a.kck + m.mv_off;
   rc.replay = a.replay ? (c_u32 *)(a.replay + m.mv_off) : nullptr;
@@ -788,8 +855,8 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
   // 0 <= E0 <= n_e + 111.
   auto tile_kind = [&](int b) {
     const int K0 = walk_k(w), E0 = walk_e(w);
-    if ((b & 7) != 0 || b + 8 > n_bands || rc.replay || K0 < -120 || K0 > n_k || E0 < 0 ||
-        E0 > n_e + 111)
+    if ((b & 7) != 0 || b + 8 > n_bands || rc.replay || !nonpos || K0 < -120 || K0 > n_k ||
+        E0 < 0 || E0 > n_e + 111)
       return 0;
     return (K0 >= 0 && K0 + 8 < n_k - ABEA_BANDWIDTH) ? 1 : 2;
   };
@@ -811,7 +878,8 @@ __device__ __forceinline__ void core_read(const KernelArgs &a, const ReadMeta &m
     ++b;
   }
   const int last = n_bands - 1;
-  if ((last & 7) != 7) rc.trace[(last >> 3) * kTileWords] = s.acc << (4 * (7 - (last & 7)));
+  if ((last & 7) != 7)
+    rc.trace[(last >> 3) * kTileWords + lane] = s.acc << (4 * (7 - (last & 7)));
   if ((last & 31) != 31) rc.moves[last >> 5] = w.mv;
   a.ends[r] = EndCell{w.best, w.best_e};  // same value from every lane
 }
@@ -1284,8 +1352,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5))) v
     else
       __builtin_amdgcn_s_setprio(0);
     if (m.n_k > 0 && m.n_e > 0) {
-      stage_read(a, m, r, lane);
-      core_read(a, m, r, lane);
+      const bool nonpos = stage_read(a, m, r, lane);
+      core_read(a, m, r, lane, nonpos);
     }
     if constexpr (FUSE) {
       wave_sync();  // this wave's trace, moves and end column are visible to it

@@ -41,15 +41,14 @@ sys.path.insert(0, os.path.join(ROOT, "abea-opencl_amd"))
 HBM_PEAK_GBS = 8000.0
 W = 100
 # Per launch of the core kernel on the headline workload, from rocprofv3
-# runs at the listed launch shape (profiles/abea_kernels_batched16384_ln8kb.md
-# gives the commands): HBM bytes (FETCH_SIZE + WRITE_SIZE passes, scaled by a
-# 1 GiB copy's readings) and VALU wave-instructions (SQ_INSTS_VALU).  Keyed by
-# (kernel, waves per CU, traceback overlap).
-PROFILED = {
-    ("abea_core_kernel<1, true>", 16, 128): {"traffic": 46567172736, "valu": 2.0855e10},
-    ("abea_core_kernel<1, true>", 20, 128): {"traffic": 49099957120, "valu": 2.0855e10},
-    ("abea_core_kernel<1, true>", 24, 128): {"traffic": 49144735264, "valu": 2.0855e10},
-    ("abea_core_kernel<1, true>", 32, 128): {"traffic": 49049624960, "valu": 2.0855e10},
+# (profiles/abea_kernels_batched16384_ln8kb.md gives the commands): VALU
+# wave-instructions (SQ_INSTS_VALU), the same at every launch shape, and HBM
+# bytes (FETCH_SIZE + WRITE_SIZE passes, scaled by a 1 GiB copy's readings)
+# per launch shape (kernel, waves per CU, traceback overlap).
+PROFILED_VALU = {"abea_core_kernel<1, true>": 2.05535e10}
+PROFILED_TRAFFIC = {
+    ("abea_core_kernel<1, true>", 16, 128): 46036583104,
+    ("abea_core_kernel<1, true>", 20, 128): 48777037088,
 }
 N_SIMD = 1024          # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # peak engine clock
@@ -198,7 +197,8 @@ def headline_leg(args, ctx):
     core_bytes = walk_b + (tb_b if tune["fuse_traceback"] else 0)
     achieved = core_bytes / (kt[1] * 1e-3) / 1e9
     kname = kernel_name(tune)
-    prof = PROFILED.get((kname, tune["core_waves_per_cu"], tune["traceback_overlap"]))
+    traffic = PROFILED_TRAFFIC.get((kname, tune["core_waves_per_cu"], tune["traceback_overlap"]))
+    valu = PROFILED_VALU.get(kname)
     out = {
         "metric": "ABEA GCUPS (batched persistent form, 1 GPU)",
         "value": round(gcups, 3),
@@ -223,20 +223,20 @@ def headline_leg(args, ctx):
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": prof["traffic"] if prof else None,
+                     "traffic": traffic,
                      "alg_bytes_per_launch": core_bytes,
                      "box_copy_gbs": None if args.profile else hbm_copy_gbs(torch)},
     }
-    if prof:
+    if valu:
         # VALU issue rate against the ceiling of one wave instruction per SIMD
         # per 4 cycles at the peak clock
-        rate = prof["valu"] / (kt[1] * 1e-3)
+        rate = valu / (kt[1] * 1e-3)
         peak = N_SIMD * CLOCK_HZ / 4
-        out["issue"] = {"bound": "valu", "valu_insts_per_launch": prof["valu"],
-                        "valu_insts_per_band": round(prof["valu"] / int(b.n_bands.sum()), 2),
+        out["issue"] = {"bound": "valu", "valu_insts_per_launch": valu,
+                        "valu_insts_per_band": round(valu / int(b.n_bands.sum()), 2),
                         "achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
                         "unit": "G wave-instructions/s", "frac": round(rate / peak, 4)}
-    else:
+    if traffic is None:
         out["roofline"]["traffic_note"] = "no pmc profile at this launch shape"
     keep = {"pairs": res["pairs"], "path_from": res["path_from"], "path_len": res["path_len"],
             "n_pairs": res["n_pairs"], "score": res["score"], "avg": res["avg"],

@@ -523,6 +523,43 @@ def test_emission_sum_fallback_paths(ctx):
         assert compare_to_oracle(batch, res) == []
 
 
+def test_positive_emissions_vs_oracle(ctx):
+    """A model stdv below 0.399 makes -0.918938 - log_stdv positive, so a
+    cell can exceed 0.  The fast tiles decide band moves on the bits of cells
+    that are <= 0; reads containing such a k-mer walk every band by the
+    general step instead.  AAAAAA gets stdv 0.05 (about 40% of 2 kb reads
+    contain it), and poly-A reads with events within 0.005 pA of its mean
+    have emissions near +2.07 (the 40-base one scores +79); every read,
+    either way, must match the oracle bit for bit in both forms."""
+    from abea_amd import synth
+    base = synth.workload("cpu64x2kb", seed=71, n=16)
+    model = base.model.copy()
+    model[0, 1] = np.float32(0.05)
+    model[0, 2] = np.float32(np.log(np.float32(0.05)))
+    seqs = [base.read_seq(i) for i in range(16)]
+    evs = [base.read_events(i) for i in range(16)]
+    rng = np.random.default_rng(71)
+    for L in (40, 300, 1500):
+        seqs.append(b"A" * L)
+        evs.append((model[0, 0] + rng.normal(0, 0.005, int(1.6 * L))).astype(np.float32))
+    rl = np.array([len(q) for q in seqs], np.int64)
+    ne = np.array([len(e) for e in evs], np.int64)
+    sp = np.concatenate([[0], np.cumsum(rl)[:-1]])
+    ep = np.concatenate([[0], np.cumsum(ne)[:-1]])
+    n = len(seqs)
+    b = synth.Batch(model, np.frombuffer(b"".join(seqs), np.uint8).copy(), sp, rl,
+                    np.concatenate(evs).astype(np.float32), ep, ne,
+                    np.concatenate([base.scale, np.ones(3, np.float32)]),
+                    np.concatenate([base.shift, np.zeros(3, np.float32)]),
+                    np.ones(n, np.float32), np.zeros(n, np.float32))
+    with_a = sum(b"AAAAAA" in q for q in seqs)
+    assert 3 < with_a < n, with_a
+    for form in (1, 0):
+        res = run(ctx, b, form)
+        assert compare_to_oracle(b, res) == [], form
+        assert res["score"][16] > 0  # a path of cells above 0
+
+
 def test_replay_f5c_dump(ctx, tmp_path):
     """A two-batch f5c dump (the golden reads, then the same reads in reverse
     order) replayed through the batched plan with event_t records: every
