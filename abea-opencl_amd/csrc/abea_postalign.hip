@@ -685,6 +685,9 @@ __device__ __forceinline__ bool exact_total(const ExactSum &es, double &total) {
 constexpr int kMomWaves = 8;
 constexpr int kMomGroup = 4;
 constexpr int kMomSpan = 64 * kMomGroup;  // terms per group
+// the two chains' stages start 4 LDS banks apart (not 2 KB, the same bank),
+// so the chain lanes' loads of one step do not conflict
+constexpr int kMomSecond = kMomSpan + 2;
 
 struct MomRead {
   const abea_event_t *events;
@@ -730,7 +733,7 @@ __device__ __forceinline__ void serial_sum_span(const double *v, double &acc) {
 }
 
 __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
-  __shared__ double stages[kMomWaves][2 * kMomSpan];
+  __shared__ double stages[kMomWaves][kMomSecond + kMomSpan];
   __shared__ float levels[ABEA_NUM_KMER];
   for (int i = threadIdx.x; i < ABEA_NUM_KMER; i += blockDim.x) levels[i] = a.model[i].level_mean;
   __syncthreads();
@@ -780,11 +783,11 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
       for (int g = 0; g < kMomGroup; ++g) {
         const int i = base + 64 * g + lane;
         st[64 * g + lane] = i < n_e ? (double)cur[g].x : 0.0;
-        st[kMomSpan + 64 * g + lane] =
+        st[kMomSecond + 64 * g + lane] =
             i < nk ? (double)levels[rank_lohi(cur[g].lo, cur[g].hi)] : 0.0;
       }
       wave_lds_sync();
-      serial_sum_span(st + ((lane & 1) ? kMomSpan : 0), acc);
+      serial_sum_span(st + ((lane & 1) ? kMomSecond : 0), acc);
       wave_lds_sync();
     }
     if (!ex_ok) ev_sum = readlane_f64(acc, 0);
@@ -812,10 +815,10 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
         d2 = dx * dx;
       }
       st[64 * g + lane] = l2;
-      st[kMomSpan + 64 * g + lane] = d2;
+      st[kMomSecond + 64 * g + lane] = d2;
     }
     wave_lds_sync();
-    serial_sum_span(st + ((lane & 1) ? kMomSpan : 0), acc);
+    serial_sum_span(st + ((lane & 1) ? kMomSecond : 0), acc);
     wave_lds_sync();
   }
   const double km_sq = readlane_f64(acc, 0), ev_sq = readlane_f64(acc, 1);

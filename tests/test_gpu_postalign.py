@@ -186,6 +186,57 @@ def test_mom_golden_and_synthetic(ctx):
         assert got.tobytes() == expect.tobytes()
 
 
+def test_mom_mixed_lengths_and_serial_sums(ctx):
+    """MOM on reads of 2 to 12 kb, some of whose event or level sums fail the
+    exactness test (a 1e-30 event, a k-mer of level 1e-30) and so take the
+    serial pass, among reads whose sums run in parallel.  The batched plan
+    (queue longest first) and the host entry point (its own order) both
+    match the oracle bit for bit."""
+    import oracle
+    from abea_amd import engine, synth
+    L, lib = _lib()
+    lengths = np.array([12000, 7500, 7000, 6800, 6600, 4000, 3800] + [2000 + 37 * i
+                                                                      for i in range(23)])
+    b = synth.generate(lengths, seed=41, skip=0.015)
+    model = b.model.copy()
+    model[0, 0] = np.float32(1e-30)  # AAAAAA
+    b.model = model
+    ev = b.ev_mean.copy()
+    for i in (2, 9, 20):  # one tiny event
+        ev[int(b.ev_ptr[i]) + 5] = np.float32(1e-30)
+    b.ev_mean = ev
+    seq = b.seq.copy()
+    for i in (4, 13):  # AAAAAA
+        p = int(b.seq_ptr[i]) + 100
+        seq[p:p + 6] = np.frombuffer(b"AAAAAA", np.uint8)
+    b.seq = seq
+    n = b.n_reads
+    want = np.stack([oracle.estimate_scalings_mom(b.read_seq(i), b.read_events(i), b.model)
+                     for i in range(n)])
+    plan = engine.Plan(ctx, engine.DeviceBatch.upload(b, 0), 1)
+    got = plan.estimate_scalings()[:n].cpu().numpy()
+    plan.close()
+    assert got.tobytes() == want.tobytes()
+
+    ev_all = b.events_aos()
+    seqs = [C.create_string_buffer(b.read_seq(i), len(b.read_seq(i)) + 1) for i in range(n)]
+    read = (C.c_char_p * n)(*[C.cast(s, C.c_char_p) for s in seqs])
+    read_len = np.ascontiguousarray(b.read_len, np.int32)
+    ets, keeps = [], []
+    for i in range(n):
+        et, k = _et(ev_all[int(b.ev_ptr[i]):int(b.ev_ptr[i]) + int(b.n_events[i])])
+        ets.append(et)
+        keeps.append(k)
+    et_arr = (L.EventTable * n)(*ets)
+    scal = (L.ScalingsT * n)()
+    vp = lambda x: C.cast(x, C.c_void_p)  # noqa: E731
+    mdl = np.ascontiguousarray(b.model, np.float32)
+    assert lib.abea_estimate_scalings_db(n, vp(read), read_len.ctypes.data, vp(et_arr),
+                                         mdl.ctypes.data, vp(scal)) == 0
+    got2 = np.array([[s.scale, s.shift, s.var, s.log_var] for s in scal], np.float32)
+    assert got2.tobytes() == want.tobytes()
+
+
 def _first_diff(got: bytes, want: bytes, item: int = 32):
     """None when equal, else (index, got item, want item) of the first
     differing record."""
