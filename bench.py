@@ -49,6 +49,8 @@ PROFILED_VALU = {"abea_core_kernel<1, true>": 2.05535e10}
 PROFILED_TRAFFIC = {
     ("abea_core_kernel<1, true>", 16, 128): 46036583104,
     ("abea_core_kernel<1, true>", 20, 128): 48777037088,
+    ("abea_core_kernel<1, true>", 24, 128): 48763236352,
+    ("abea_core_kernel<1, true>", 32, 128): 48762137568,
 }
 N_SIMD = 1024          # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # peak engine clock
