@@ -26,8 +26,6 @@ at K >= -120).
 """
 import numpy as np
 
-NEG_INF_BITS = 0xFF800000
-
 
 def f32_bits(x):
     return int(np.array([x], np.float32).view(np.uint32)[0])
