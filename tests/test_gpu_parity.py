@@ -146,7 +146,7 @@ def test_tunings_agree(ctx):
     # again; 1024 starts nearly every lane at the true end cell
     for wpb, wpc, overlap, fuse in [(1, 8, 32, 0), (2, 16, 128, 0), (4, 32, 1024, 0),
                                     (4, 16, 64, 0), (1, 16, 32, 1), (1, 20, 128, 1),
-                                    (2, 24, 128, 1),
+                                    (1, 16, 96, 1), (2, 24, 128, 1),
                                     (4, 32, 256, 1), (1, 8, 1024, 1)]:
         r = run(ctx, b, 1, tune=dict(core_waves_per_block=wpb, core_waves_per_cu=wpc,
                                      traceback_overlap=overlap, use_graph=0,
