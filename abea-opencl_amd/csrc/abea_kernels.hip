@@ -74,8 +74,9 @@ __device__ __forceinline__ uint32_t lane_bits(float v, int lane) {
 
 // Signed-integer order key of a float's bits: for non-NaN values without -0,
 // key(x) < key(y) exactly when x < y.  Band cells are never NaN or -0 (they
-// are -inf or rounded sums), so the band-move comparison runs on the scalar
-// unit.
+// are -inf or rounded sums), so the general band step compares them on the
+// scalar unit through these keys, whatever their sign (the fast tiles use the
+// shorter move_step, for reads whose cells are all <= 0).
 __device__ __forceinline__ int32_t order_key(uint32_t u) {
   const int32_t i = (int32_t)u;
   return i ^ ((i >> 31) & 0x7fffffff);
