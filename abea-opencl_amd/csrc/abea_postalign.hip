@@ -196,40 +196,51 @@ __device__ __forceinline__ int2 load_pair(const abea_aligned_pair_t *P, int i) {
   return q[i];
 }
 
-// A chunk's raw loads: its pairs (and the next chunk's first pair) and each
-// pair's k-mer bases.  Loads are unconditional, from clamped (in-bounds)
-// indices, so that they can be issued ahead: a load under a branch would be
-// waited for at the branch's join.  Validity is applied where they are used.
+// A chunk's raw loads: its pairs and each pair's k-mer bases.  Loads are
+// unconditional, from clamped (in-bounds) indices, so that they can be
+// issued ahead: a load under a branch would be waited for at the branch's
+// join.  Validity is applied where they are used.
 struct ChunkLoad {
-  int2 c, nx;
-  uint32_t lo, hi;
+  int2 c;
+  uint32_t lo, hi4;  // bases k..k+3 and k+2..k+5 of each pair's k-mer k
+};
+
+// A chunk's gathered recalibration inputs: each lane's event mean and its
+// k-mer's model row.
+struct ChunkGather {
+  float x = 0.f, mu = 0.f, sd = 0.f;
 };
 
 __device__ __forceinline__ void load_pairs(const PairView &v, int base, int lane, ChunkLoad &L) {
   L.c = load_pair(v.P, min(base + lane, v.np - 1));
-  L.nx = load_pair(v.P, min(base + 64, v.np - 1));
 }
 
+// Two 4-byte loads inside the read (k <= len - 6), with no sub-word load
+// whose widening would be done, and waited for, as soon as it is issued.
 __device__ __forceinline__ void load_kmers(const PairView &v, ChunkLoad &L) {
-  kmer_bytes(v.S, min(max(L.c.x, 0), v.nk - 1), L.lo, L.hi);
+  const char *q = v.S + min(max(L.c.x, 0), v.nk - 1);
+  __builtin_memcpy(&L.lo, q, 4);
+  __builtin_memcpy(&L.hi4, q + 2, 4);
 }
 
 // All lanes must call.  Pairs p < end belong to this work item; the path's
 // pair p+1 exists when p + 1 < np.  Neighbouring pairs come from the
-// neighbouring lanes (DPP lane shifts); lane 0 takes pair p-1 from the carry.
-// A pair is bad when its step from pair p-1 is not D, U or L, its k-mer or
-// event is out of range, or its record index e - e0 is outside [0, np).
+// neighbouring lanes (DPP lane shifts); lane 0 takes pair p-1 from the carry,
+// lane 63 takes pair p+1 from nx (the next chunk's first pair, clamped to the
+// path's last as the loads are).  A pair is bad when its step from pair p-1
+// is not D, U or L, its k-mer or event is out of range, or its record index
+// e - e0 is outside [0, np).
 __device__ __forceinline__ PairStep pair_step(const PairView &v, int base, int lane, int end,
-                                              int e0, const ChunkLoad &L, bool need_events,
-                                              Carry &cy) {
+                                              int e0, const ChunkLoad &L, int2 nx,
+                                              bool need_events, Carry &cy) {
   PairStep s{};
   const int p = base + lane;
   s.valid = p < end;
   // neighbours from the loaded pairs themselves: pair p+1 may lie past this
   // work item's end (it belongs to the next one), but it is the path's
   const int2 c = s.valid ? L.c : make_int2(-1, -1);
-  const int nk_ = __builtin_amdgcn_update_dpp(L.nx.x, L.c.x, 0x130, 0xf, 0xf, false);  // wave_shl:1
-  const int ne_ = __builtin_amdgcn_update_dpp(L.nx.y, L.c.y, 0x130, 0xf, 0xf, false);
+  const int nk_ = __builtin_amdgcn_update_dpp(nx.x, L.c.x, 0x130, 0xf, 0xf, false);  // wave_shl:1
+  const int ne_ = __builtin_amdgcn_update_dpp(nx.y, L.c.y, 0x130, 0xf, 0xf, false);
   const int ak = __builtin_amdgcn_update_dpp(cy.k, L.c.x, 0x138, 0xf, 0xf, false);  // wave_shr:1
   const int ae = __builtin_amdgcn_update_dpp(cy.e, L.c.y, 0x138, 0xf, 0xf, false);
   s.k = c.x;
@@ -249,18 +260,20 @@ __device__ __forceinline__ PairStep pair_step(const PairView &v, int base, int l
   s.last = s.is_new && !(p + 1 < v.np && ne_ != c.y && nk_ == c.x);
   s.none = live && !s.is_new && (p + 1 == v.np || nk_ != c.x);
   s.lo = live ? L.lo : 0;
-  s.hi = live ? L.hi : 0;
+  s.hi = live ? L.hi4 >> 16 : 0;
   s.rank = live ? rank_lohi(s.lo, s.hi) : 0;
 
-  // rank of the previous record: that of the last new pair before this one
-  const int incl = wave_scan_max(s.is_new ? lane : -1);
+  // rank of the previous record: that of the last new pair before this one.
+  // Ranks are below 4096, so the prefix maximum of lane * 4096 + rank over
+  // the new pairs finds the last new lane at or before each lane and carries
+  // its rank in the low 12 bits.
+  const int incl = wave_scan_max(s.is_new ? (lane << 12) | (int)s.rank : -1);
   const int excl = __builtin_amdgcn_update_dpp(-1, incl, 0x138, 0xf, 0xf, false);  // wave_shr:1
-  const int from = __builtin_amdgcn_ds_bpermute(max(excl, 0) << 2, (int)s.rank);
-  const int prank = excl >= 0 ? from : cy.rank;
+  const int prank = excl >= 0 ? (excl & 0xfff) : cy.rank;
   s.m = s.first && (prank < 0 || (uint32_t)prank != s.rank);
 
   const int last_new = __builtin_amdgcn_readlane(incl, 63);
-  if (last_new >= 0) cy.rank = __builtin_amdgcn_readlane((int)s.rank, last_new);
+  if (last_new >= 0) cy.rank = last_new & 0xfff;
   cy.k = __builtin_amdgcn_readlane(L.c.x, 63);
   cy.e = __builtin_amdgcn_readlane(L.c.y, 63);
   cy.is_new = __builtin_amdgcn_readlane((int)s.is_new, 63);
@@ -279,13 +292,38 @@ __device__ __forceinline__ void record_words(const PairStep &s, uint4 &w0, uint4
   w1.w = s.m ? 'M' : 'E';               // hmm_state, pad
 }
 
+// Stores that every lane of a chunk issues: raw buffer stores (vector
+// memory) whose lanes with nothing to write pass a byte offset outside the
+// buffer's range, which the hardware drops.  Each chunk then issues the same
+// number of memory operations whichever lanes write, so the compiler can
+// wait for a load by count without also waiting for younger stores.
+constexpr uint32_t kSkip = 0x80000000u;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// A raw buffer of `bytes` bytes at p (descriptor word 3: 32-bit data format,
+// the gfx9-family value for untyped buffer access).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t store_buffer(void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void store_b32(uint32_t v, __amdgpu_buffer_rsrc_t b, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, b, (int)off, 0, 0);
+}
+__device__ __forceinline__ void store_b64(u32x2 v, __amdgpu_buffer_rsrc_t b, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b64(v, b, (int)off, 0, 0);
+}
+__device__ __forceinline__ void store_b128(u32x4 v, __amdgpu_buffer_rsrc_t b, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b128(v, b, (int)off, 0, 0);
+}
+
 // A chunk's records are consecutive (its new pairs have consecutive events):
 // assemble them in LDS and store them as one contiguous block, 16 bytes per
-// lane per store.
-__device__ __forceinline__ void write_records(Stage &st, abea_event_alignment_t *R, int e0,
+// lane per store, into the read's records (buffer rb, record e - e0 of event
+// e).  A chunk without new pairs issues the same two stores, all lanes
+// skipped.
+__device__ __forceinline__ void write_records(Stage &st, __amdgpu_buffer_rsrc_t rb, int e0,
                                               const PairStep &s, int lane) {
   const uint64_t nm = __builtin_amdgcn_ballot_w64(s.is_new);
-  if (!nm) return;
   uint4 *L = reinterpret_cast<uint4 *>(st.t);
   if (s.is_new) {
     uint4 w0, w1;
@@ -295,11 +333,12 @@ __device__ __forceinline__ void write_records(Stage &st, abea_event_alignment_t 
     L[2 * j + 1] = w1;
   }
   wave_lds_sync();
-  const int e_first = __builtin_amdgcn_readlane(s.e, __builtin_ctzll(nm));
-  uint4 *dst = reinterpret_cast<uint4 *>(R + (e_first - e0));
+  const int e_first = nm ? __builtin_amdgcn_readlane(s.e, __builtin_ctzll(nm)) : e0;
+  const uint32_t o = 32u * (uint32_t)(e_first - e0);
   const int words = 2 * __builtin_popcountll(nm);
-  if (lane < words) dst[lane] = L[lane];
-  if (64 + lane < words) dst[64 + lane] = L[64 + lane];
+  const uint4 q0 = L[lane], q1 = L[64 + lane];
+  store_b128(u32x4{q0.x, q0.y, q0.z, q0.w}, rb, lane < words ? o + 16u * lane : kSkip);
+  store_b128(u32x4{q1.x, q1.y, q1.z, q1.w}, rb, 64 + lane < words ? o + 16u * (64 + lane) : kSkip);
   wave_lds_sync();
 }
 
@@ -319,11 +358,12 @@ __device__ __forceinline__ int last_record_rank(const PairView &v, int q) {
 }
 
 // Walk kernel: one wave per (read, segment of kSegPairs pairs).  Writes the
-// map entries, the records and, per k-mer on the path, the (event mean,
-// level mean, level stdv, 'M') of the k-mer's first record -- each k-mer of
-// the path has exactly one pair that writes it: the first pair of its events
-// or, for a k-mer without events, the pair that passes it.  A bad path sets
-// the read's status word.
+// map entries (MAP), the records (REC) and, per k-mer on the path (RECAL),
+// the (event mean, level mean, level stdv, 'M') of the k-mer's first record
+// -- each k-mer of the path has exactly one pair that writes it: the first
+// pair of its events or, for a k-mer without events, the pair that passes
+// it.  A bad path sets the read's status word.
+template <bool REC, bool MAP, bool RECAL>
 __global__ __launch_bounds__(256) void abea_postalign_walk_kernel(PostArgs a) {
   __shared__ Stage stages[kWavesPerBlock];
   Stage &st = stages[threadIdx.x >> 6];
@@ -342,26 +382,29 @@ __global__ __launch_bounds__(256) void abea_postalign_walk_kernel(PostArgs a) {
   v.nk = uniform(m.n_k);
   v.n_e = uniform(m.n_e);
   v.S = a.seq + a.seq_ptr[r];
-  const bool recal = a.recalibrate != 0;
   // pairs on a read without k-mers (or, to recalibrate, without events)
   // cannot be a path
-  if (v.nk <= 0 || (recal && v.n_e <= 0)) {
+  if (v.nk <= 0 || (RECAL && v.n_e <= 0)) {
     if (lane == 0) a.pstat[r] = 1;
     return;
   }
   const int end = min(begin + kSegPairs, np);
   const int64_t evb = m.ev_off;
-  abea_event_alignment_t *R = a.rec ? a.rec + m.pair_off : nullptr;
-  abea_index_pair_t *B = a.bmap ? a.bmap + m.bmap_off : nullptr;
-  float4 *T = recal ? a.terms + m.bmap_off : nullptr;
+  const __amdgpu_buffer_rsrc_t rb =
+      store_buffer(REC ? a.rec + m.pair_off : nullptr, REC ? 32u * (uint32_t)np : 0u);
+  const __amdgpu_buffer_rsrc_t mb =
+      store_buffer(MAP ? a.bmap + m.bmap_off : nullptr, MAP ? 8u * (uint32_t)v.nk : 0u);
+  const __amdgpu_buffer_rsrc_t tb =
+      store_buffer(RECAL ? a.terms + m.bmap_off : nullptr, RECAL ? 16u * (uint32_t)v.nk : 0u);
   const int2 p0 = load_pair(v.P, 0);
   const int e0 = uniform(p0.y);
 
-  if (begin == 0 && B) {  // k-mers before the path's first and after its last
+  if (MAP && begin == 0) {  // k-mers before the path's first and after its last
+    int2 *B2 = reinterpret_cast<int2 *>(a.bmap + m.bmap_off);
     const int k0 = uniform(p0.x), kN = uniform(load_pair(v.P, np - 1).x);
     const int2 none = make_int2(-1, -1);
-    for (int k = lane; k < min(k0, v.nk); k += 64) reinterpret_cast<int2 *>(B)[k] = none;
-    for (int k = max(kN + 1, 0) + lane; k < v.nk; k += 64) reinterpret_cast<int2 *>(B)[k] = none;
+    for (int k = lane; k < min(k0, v.nk); k += 64) B2[k] = none;
+    for (int k = max(kN + 1, 0) + lane; k < v.nk; k += 64) B2[k] = none;
   }
 
   Carry cy;
@@ -373,50 +416,84 @@ __global__ __launch_bounds__(256) void abea_postalign_walk_kernel(PostArgs a) {
     cy.rank = uniform(last_record_rank(v, begin - 1));
   }
 
-  // Software pipeline, in issue order: the gathers of chunk c (model row and
-  // event mean of each lane's pair) are issued at the end of iteration c-1,
-  // before the k-mer loads of chunk c+1 and the pair loads of chunk c+2, so
-  // that waiting for them (the load counter is in order) does not wait for
-  // the younger prefetches.
-  auto gather = [&](const ChunkLoad &L, float &x, float &mu, float &sd) {
-    if (!recal) return;
-    const abea_model_t md = a.model[rank_lohi(L.lo, L.hi)];
-    mu = md.level_mean;
-    sd = md.level_stdv;
-    x = event_mean(a.events, a.ev_mean, evb + min(max(L.c.y, 0), v.n_e - 1));
+  // Software pipeline: each load level waits for the one before it (pairs,
+  // then their k-mer bases, then the gathers of model rows and event means),
+  // so each runs one chunk ahead of the next.  The step for chunk c issues
+  // the gathers of chunk c+1, the pair loads of chunk c+3 and the k-mer
+  // loads of chunk c+2, then works on chunk c and issues its stores.  Every
+  // load it waits for was issued before the previous chunk's work, and with
+  // the same memory operations in every step (the stores above) and a loop
+  // with one exit, the waits count exactly and never wait for the previous
+  // chunk's stores.  The loop runs whole groups of four steps, the buffers'
+  // roles rotated so that no buffer is copied (a copy of a register still
+  // being loaded waits for the load); steps past the segment's end have no
+  // valid pairs and store nothing.  After a bad pair the read's walk stops
+  // storing (dead) and its status word is set at the end.
+  auto gather = [&](const ChunkLoad &L, ChunkGather &g) {
+    if constexpr (RECAL) {
+      const abea_model_t md = a.model[rank_lohi(L.lo, L.hi4 >> 16)];
+      g.mu = md.level_mean;
+      g.sd = md.level_stdv;
+      g.x = event_mean(a.events, a.ev_mean, evb + min(max(L.c.y, 0), v.n_e - 1));
+    }
   };
-  ChunkLoad cur, nxt;
-  float gx = 0.f, gmu = 0.f, gsd = 0.f;
-  load_pairs(v, begin, lane, cur);
-  load_kmers(v, cur);
-  gather(cur, gx, gmu, gsd);
-  load_pairs(v, begin + 64, lane, nxt);
-  for (int base = begin; base < end; base += 64) {
-    ChunkLoad after;
-    load_kmers(v, nxt);
-    load_pairs(v, base + 128, lane, after);
-    const PairStep s = pair_step(v, base, lane, end, e0, cur, recal, cy);
-    if (__builtin_amdgcn_ballot_w64(s.bad)) {
-      if (lane == 0) a.pstat[r] = 1;
-      return;
+  // The stores of one chunk; lanes whose flags are clear store nothing.
+  auto stores = [&](const PairStep &s, const ChunkGather &g) {
+    if constexpr (REC) write_records(st, rb, e0, s, lane);
+    if constexpr (MAP) {
+      // {first, last} event, or {-1, -1} for a k-mer without events, as one
+      // entry; a first or last event alone as one word
+      const bool whole = (s.first && s.last) || s.none;
+      const uint32_t ev = (uint32_t)s.e;
+      store_b64(s.none ? u32x2{~0u, ~0u} : u32x2{ev, ev}, mb, whole ? 8u * s.k : kSkip);
+      store_b32(ev, mb, s.first != s.last ? 8u * s.k + (s.last ? 4u : 0u) : kSkip);
     }
-    if (B) {
-      int2 *B2 = reinterpret_cast<int2 *>(B);
-      if (s.first && s.last) {
-        B2[s.k] = make_int2(s.e, s.e);
-      } else {
-        if (s.first) B[s.k].start = s.e;
-        if (s.last) B[s.k].stop = s.e;
-      }
-      if (s.none) B2[s.k] = make_int2(-1, -1);
+    if constexpr (RECAL) {
+      const u32x4 t = s.m ? u32x4{__builtin_bit_cast(uint32_t, g.x),
+                                  __builtin_bit_cast(uint32_t, g.mu),
+                                  __builtin_bit_cast(uint32_t, g.sd), 0x3f800000u}
+                          : u32x4{0u, 0u, 0u, 0u};
+      store_b128(t, tb, (s.first || s.none) ? 16u * s.k : kSkip);
     }
-    if (R) write_records(st, R, e0, s, lane);
-    if (recal && (s.first || s.none))
-      T[s.k] = s.m ? make_float4(gx, gmu, gsd, 1.f) : make_float4(0.f, 0.f, 0.f, 0.f);
-    gather(nxt, gx, gmu, gsd);
-    cur = nxt;
-    nxt = after;
+  };
+  bool dead = false;
+  // Chunk `base` (loads in A, gathers in GA; B, C, D the next three chunks).
+  auto step = [&](int base, const ChunkLoad &A, ChunkLoad &B, ChunkLoad &C, ChunkLoad &D,
+                  const ChunkGather &GA, ChunkGather &GB) {
+    // keep each step's work inside the step: the scheduler would otherwise
+    // start the next step's arithmetic on loads issued moments before
+    __builtin_amdgcn_sched_barrier(0);
+    gather(B, GB);
+    load_pairs(v, base + 192, lane, D);
+    load_kmers(v, C);
+    // the next chunk's first pair (lane 0 loads min(base + 64, np - 1))
+    const int2 nx =
+        make_int2(__builtin_amdgcn_readlane(B.c.x, 0), __builtin_amdgcn_readlane(B.c.y, 0));
+    PairStep s = pair_step(v, base, lane, end, e0, A, nx, RECAL, cy);
+    dead = dead || __builtin_amdgcn_ballot_w64(s.bad) != 0;
+    s.is_new = s.is_new && !dead;
+    s.first = s.first && !dead;
+    s.last = s.last && !dead;
+    s.none = s.none && !dead;
+    s.m = s.m && !dead;
+    stores(s, GA);
+  };
+  ChunkLoad c0, c1, c2, c3;
+  ChunkGather g0, g1;
+  load_pairs(v, begin, lane, c0);
+  load_pairs(v, begin + 64, lane, c1);
+  load_kmers(v, c0);
+  gather(c0, g0);
+  load_pairs(v, begin + 128, lane, c2);
+  load_kmers(v, c1);
+  stores(PairStep{}, g0);  // the memory operations a step leaves behind
+  for (int base = begin; base < end; base += 256) {
+    step(base, c0, c1, c2, c3, g0, g1);
+    step(base + 64, c1, c2, c3, c0, g1, g0);
+    step(base + 128, c2, c3, c0, c1, g0, g1);
+    step(base + 192, c3, c0, c1, c2, g1, g0);
   }
+  if (dead && lane == 0) a.pstat[r] = 1;
 }
 
 // The five normal-equation sums over the 'M' entries of T[k0..kN] (the
@@ -839,7 +916,19 @@ hipError_t launch_postalign(const PostArgs &a, hipStream_t s) {
   if (a.n_reads <= 0) return hipSuccess;
   if (a.n_items > 0) {
     const int wb = (a.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(abea_postalign_walk_kernel, dim3(wb), dim3(64 * kWavesPerBlock), 0, s, a);
+    auto walk = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(wb), dim3(64 * kWavesPerBlock), 0, s, a);
+    };
+    switch ((a.rec ? 4 : 0) | (a.bmap ? 2 : 0) | (a.recalibrate ? 1 : 0)) {
+      case 0: walk(abea_postalign_walk_kernel<false, false, false>); break;
+      case 1: walk(abea_postalign_walk_kernel<false, false, true>); break;
+      case 2: walk(abea_postalign_walk_kernel<false, true, false>); break;
+      case 3: walk(abea_postalign_walk_kernel<false, true, true>); break;
+      case 4: walk(abea_postalign_walk_kernel<true, false, false>); break;
+      case 5: walk(abea_postalign_walk_kernel<true, false, true>); break;
+      case 6: walk(abea_postalign_walk_kernel<true, true, false>); break;
+      default: walk(abea_postalign_walk_kernel<true, true, true>); break;
+    }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
