@@ -317,14 +317,15 @@ __device__ __forceinline__ void store_b128(u32x4 v, __amdgpu_buffer_rsrc_t b, ui
 }
 
 // A chunk's records are consecutive (its new pairs have consecutive events):
-// assemble them in LDS and store them as one contiguous block, 16 bytes per
-// lane per store, into the read's records (buffer rb, record e - e0 of event
-// e).  A chunk without new pairs issues the same two stores, all lanes
-// skipped.
-__device__ __forceinline__ void write_records(Stage &st, __amdgpu_buffer_rsrc_t rb, int e0,
+// assemble them in the wave's LDS stage L (128 words of 16 bytes) and store
+// them as one contiguous block, 16 bytes per lane per store, into the read's
+// records (buffer rb, record e - e0 of event e).  A chunk without new pairs
+// issues the same two stores, all lanes skipped.  The records are output
+// only: the stores are marked non-temporal (streamed past the caches).
+constexpr int kStoreNT = 2;  // cache-policy bits of a buffer store: nt
+__device__ __forceinline__ void write_records(uint4 *L, __amdgpu_buffer_rsrc_t rb, int e0,
                                               const PairStep &s, int lane) {
   const uint64_t nm = __builtin_amdgcn_ballot_w64(s.is_new);
-  uint4 *L = reinterpret_cast<uint4 *>(st.t);
   if (s.is_new) {
     uint4 w0, w1;
     record_words(s, w0, w1);
@@ -337,8 +338,12 @@ __device__ __forceinline__ void write_records(Stage &st, __amdgpu_buffer_rsrc_t 
   const uint32_t o = 32u * (uint32_t)(e_first - e0);
   const int words = 2 * __builtin_popcountll(nm);
   const uint4 q0 = L[lane], q1 = L[64 + lane];
-  store_b128(u32x4{q0.x, q0.y, q0.z, q0.w}, rb, lane < words ? o + 16u * lane : kSkip);
-  store_b128(u32x4{q1.x, q1.y, q1.z, q1.w}, rb, 64 + lane < words ? o + 16u * (64 + lane) : kSkip);
+  __builtin_amdgcn_raw_buffer_store_b128(u32x4{q0.x, q0.y, q0.z, q0.w}, rb,
+                                         (int)(lane < words ? o + 16u * lane : kSkip), 0,
+                                         kStoreNT);
+  __builtin_amdgcn_raw_buffer_store_b128(u32x4{q1.x, q1.y, q1.z, q1.w}, rb,
+                                         (int)(64 + lane < words ? o + 16u * (64 + lane) : kSkip),
+                                         0, kStoreNT);
   wave_lds_sync();
 }
 
@@ -357,16 +362,29 @@ __device__ __forceinline__ int last_record_rank(const PairView &v, int q) {
   return (int)rank_lohi(lo, hi);
 }
 
-// Walk kernel: one wave per (read, segment of kSegPairs pairs).  Writes the
-// map entries (MAP), the records (REC) and, per k-mer on the path (RECAL),
-// the (event mean, level mean, level stdv, 'M') of the k-mer's first record
-// -- each k-mer of the path has exactly one pair that writes it: the first
-// pair of its events or, for a k-mer without events, the pair that passes
-// it.  A bad path sets the read's status word.
+// Walk kernel: one wave per (read, segment of kSegPairs pairs), kWalkWaves
+// waves per workgroup.  Writes the map entries (MAP), the records (REC) and,
+// per k-mer on the path (RECAL), the (event mean, level mean, level stdv,
+// 'M') of the k-mer's first record -- each k-mer of the path has exactly one
+// pair that writes it: the first pair of its events or, for a k-mer without
+// events, the pair that passes it.  A bad path sets the read's status word.
+// With RECAL the workgroup first copies the model's level means and stdvs
+// (32 KB) into LDS, where the pairs' k-mer ranks gather them: gathered from
+// global memory, 64 scattered rows per chunk cost more than the rest of the
+// chunk's loads together.
+// Eight waves share a table: 48 KB of LDS per workgroup with the record
+// stages, three workgroups (24 waves) per CU at up to 80 VGPRs per wave.
+constexpr int kWalkWaves = 8;
 template <bool REC, bool MAP, bool RECAL>
-__global__ __launch_bounds__(256) void abea_postalign_walk_kernel(PostArgs a) {
-  __shared__ Stage stages[kWavesPerBlock];
-  Stage &st = stages[threadIdx.x >> 6];
+__global__ __launch_bounds__(64 * kWalkWaves, 6) void abea_postalign_walk_kernel(PostArgs a) {
+  __shared__ uint4 rec_stage[REC ? kWalkWaves : 1][128];
+  __shared__ float2 levels[RECAL ? ABEA_NUM_KMER : 1];
+  if constexpr (RECAL) {
+    for (int i = threadIdx.x; i < ABEA_NUM_KMER; i += 64 * kWalkWaves)
+      levels[i] = make_float2(a.model[i].level_mean, a.model[i].level_stdv);
+    __syncthreads();
+  }
+  uint4 *st = rec_stage[REC ? threadIdx.x >> 6 : 0];
   const int lane = threadIdx.x & 63;
   const int w = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (w >= a.n_items) return;
@@ -431,9 +449,9 @@ __global__ __launch_bounds__(256) void abea_postalign_walk_kernel(PostArgs a) {
   // storing (dead) and its status word is set at the end.
   auto gather = [&](const ChunkLoad &L, ChunkGather &g) {
     if constexpr (RECAL) {
-      const abea_model_t md = a.model[rank_lohi(L.lo, L.hi4 >> 16)];
-      g.mu = md.level_mean;
-      g.sd = md.level_stdv;
+      const float2 md = levels[rank_lohi(L.lo, L.hi4 >> 16)];
+      g.mu = md.x;
+      g.sd = md.y;
       g.x = event_mean(a.events, a.ev_mean, evb + min(max(L.c.y, 0), v.n_e - 1));
     }
   };
@@ -915,9 +933,9 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
 hipError_t launch_postalign(const PostArgs &a, hipStream_t s) {
   if (a.n_reads <= 0) return hipSuccess;
   if (a.n_items > 0) {
-    const int wb = (a.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int wb = (a.n_items + kWalkWaves - 1) / kWalkWaves;
     auto walk = [&](auto kernel) {
-      hipLaunchKernelGGL(kernel, dim3(wb), dim3(64 * kWavesPerBlock), 0, s, a);
+      hipLaunchKernelGGL(kernel, dim3(wb), dim3(64 * kWalkWaves), 0, s, a);
     };
     switch ((a.rec ? 4 : 0) | (a.bmap ? 2 : 0) | (a.recalibrate ? 1 : 0)) {
       case 0: walk(abea_postalign_walk_kernel<false, false, false>); break;
