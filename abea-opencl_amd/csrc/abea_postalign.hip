@@ -517,23 +517,28 @@ __global__ __launch_bounds__(64 * kWalkWaves, 6) void abea_postalign_walk_kernel
 // The five normal-equation sums over the 'M' entries of T[k0..kN] (the
 // k-mers' first records, in record order).  Each chunk's terms are staged in
 // LDS, compacted to the 'M' entries, one record per 48 bytes; lane c < 5 then
-// runs sum c over them in order (one LDS read and one add per record), and
-// lanes 0..4 hold the five sums at the end.  n_m receives the count.
+// runs sum c over them in order (one LDS read and one add per record, the
+// other lanes masked off), and lanes 0..4 hold the five sums at the end.
+// n_m receives the count.
+//
+// Both passes keep the entries of the next three chunks loading: three
+// buffers, each reloaded with the chunk three ahead as soon as its own
+// chunk's terms are formed, in whole groups of three chunks (chunks past kN
+// have no entries), so that no buffer is copied -- a copy of a register
+// still being loaded waits for the load.
 __device__ __forceinline__ Normal normal_sums(Stage &st, const float4 *T, int k0, int kN,
                                               int lane, int &n_m) {
   double acc = 0.0;
-  const int c = lane < 5 ? lane : 4;
   n_m = 0;
-  // the entries two chunks ahead load while this chunk is summed
-  float4 nx0 = T[min(k0 + lane, kN)], nx1 = T[min(k0 + 64 + lane, kN)];
-  for (int base = k0; base <= kN; base += 64) {
-    const float4 q = nx0;
-    nx0 = nx1;
-    nx1 = T[min(base + 128 + lane, kN)];
+  auto step = [&](int base, float4 &q) {
     const bool mm = base + lane <= kN && q.w != 0.f;
     const uint64_t mask = __builtin_amdgcn_ballot_w64(mm);
     const int cnt = __builtin_popcountll(mask);
     const Terms t = normal_terms(q.x, q.y, q.z);
+    // q is dead from here: the reload may take its registers (issued any
+    // earlier, it would need others, and the loop would copy them back)
+    __builtin_amdgcn_sched_barrier(0);
+    q = T[min(base + 192 + lane, kN)];
     if (lane >= cnt) {  // padding for the 16-entry reads (positions cnt..63)
       double *z = st.t + kStage * lane;
       *reinterpret_cast<double2 *>(z) = make_double2(0.0, 0.0);
@@ -547,9 +552,16 @@ __device__ __forceinline__ Normal normal_sums(Stage &st, const float4 *T, int k0
       w[4] = t.mxw;
     }
     wave_lds_sync();
-    serial_sum_lds<kStage>(st.t + c, cnt, acc);
+    if (lane < 5) serial_sum_lds<kStage>(st.t + lane, cnt, acc);
     wave_lds_sync();
     n_m += cnt;
+  };
+  float4 b0 = T[min(k0 + lane, kN)], b1 = T[min(k0 + 64 + lane, kN)],
+         b2 = T[min(k0 + 128 + lane, kN)];
+  for (int base = k0; base <= kN; base += 192) {
+    step(base, b0);
+    step(base + 64, b1);
+    step(base + 128, b2);
   }
   Normal nm;
   nm.a00 = readlane_f64(acc, 0);
@@ -560,26 +572,32 @@ __device__ __forceinline__ Normal normal_sums(Stage &st, const float4 *T, int k0
   return nm;
 }
 
-// Sum of the squared weighted residuals over the same entries, in order.
+// Sum of the squared weighted residuals over the same entries, in order
+// (lane 0 runs it), broadcast.
 __device__ __forceinline__ double var_sum(Stage &st, const float4 *T, int k0, int kN, int lane,
                                           double shift, double scale) {
   double acc = 0.0;
-  float4 nx0 = T[min(k0 + lane, kN)], nx1 = T[min(k0 + 64 + lane, kN)];
-  for (int base = k0; base <= kN; base += 64) {
-    const float4 q = nx0;
-    nx0 = nx1;
-    nx1 = T[min(base + 128 + lane, kN)];
+  auto step = [&](int base, float4 &q) {
     const bool mm = base + lane <= kN && q.w != 0.f;
     const uint64_t mask = __builtin_amdgcn_ballot_w64(mm);
     const int cnt = __builtin_popcountll(mask);
     const double t = var_term(q.x, q.y, q.z, shift, scale);
+    __builtin_amdgcn_sched_barrier(0);  // q is dead from here (see normal_sums)
+    q = T[min(base + 192 + lane, kN)];
     if (lane >= cnt) st.t[lane] = 0.0;  // padding for the 16-entry reads
     if (mm) st.t[lanes_below(mask)] = t;  // the entries, compacted
     wave_lds_sync();
-    serial_sum_lds<1>(st.t, cnt, acc);
+    if (lane == 0) serial_sum_lds<1>(st.t, cnt, acc);
     wave_lds_sync();
+  };
+  float4 b0 = T[min(k0 + lane, kN)], b1 = T[min(k0 + 64 + lane, kN)],
+         b2 = T[min(k0 + 128 + lane, kN)];
+  for (int base = k0; base <= kN; base += 192) {
+    step(base, b0);
+    step(base + 64, b1);
+    step(base + 128, b2);
   }
-  return acc;
+  return readlane_f64(acc, 0);
 }
 
 // Finish kernel: one wave per read.  Counts and status, then (with
