@@ -812,16 +812,20 @@ struct MomRead {
 
 struct MomLoad {
   float x;
-  uint32_t lo, hi;
+  uint32_t lo, hi4;  // bases i..i+3 and i+2..i+5 of k-mer i
 };
 
-// Unconditional loads from clamped indices (see load_pairs); a read with no
-// events or no k-mers loads nothing of that kind.
+// Unconditional loads from clamped indices (see load_pairs), the same three
+// in every group, so that waiting for one group does not wait for the next
+// (the load counter is waited for in order, by count).  A read with no
+// events or no k-mers reads a stand-in (the model table) for that kind; the
+// values are not used.  The k-mer's bases come as two 4-byte loads inside
+// the read (see load_kmers).
 __device__ __forceinline__ void mom_load(const MomRead &d, int i, MomLoad &L) {
-  L.x = 0.f;
-  L.lo = L.hi = 0;
-  if (d.n_e > 0) L.x = event_mean(d.events, d.ev_mean, d.evb + min(i, d.n_e - 1));
-  if (d.nk > 0) kmer_bytes(d.S, min(i, d.nk - 1), L.lo, L.hi);
+  L.x = event_mean(d.events, d.ev_mean, d.evb + max(min(i, d.n_e - 1), 0));
+  const char *q = d.S + max(min(i, d.nk - 1), 0);
+  __builtin_memcpy(&L.lo, q, 4);
+  __builtin_memcpy(&L.hi4, q + 2, 4);
 }
 
 __device__ __forceinline__ void mom_load_group(const MomRead &d, int base, int lane,
@@ -864,63 +868,72 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
   d.n_e = uniform(m.n_e);
   d.nk = uniform(m.n_k);
   d.len = d.nk + ABEA_KMER_SIZE - 1;
+  if (d.n_e <= 0) {  // the loads' stand-ins (see mom_load)
+    d.events = nullptr;
+    d.ev_mean = &a.model[0].level_mean;
+    d.evb = 0;
+  }
+  if (d.nk <= 0) d.S = reinterpret_cast<const char *>(a.model);
   const int n_e = d.n_e, nk = d.nk;
   const int n_max = max(n_e, nk);
 
+  // Each pass runs over the read in whole pairs of groups (positions past
+  // the read's last term are masked): two load buffers trade roles, one
+  // group's loads in flight while the other group is used, and a buffer is
+  // reloaded only after its last use (the scheduling barrier), so that no
+  // buffer is copied -- a copy of a register still being loaded waits for
+  // the load.
+  MomLoad ga[kMomGroup], gb[kMomGroup];
+  auto pass = [&](auto &&body) {
+    mom_load_group(d, 0, lane, ga);
+    for (int base = 0; base < n_max; base += 2 * kMomSpan) {
+      mom_load_group(d, base + kMomSpan, lane, gb);
+      body(base, ga);
+      __builtin_amdgcn_sched_barrier(0);
+      mom_load_group(d, base + 2 * kMomSpan, lane, ga);
+      body(base + kMomSpan, gb);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+
   ExactSum ex, el;
-  MomLoad nxt[kMomGroup];
-  mom_load_group(d, 0, lane, nxt);
-  for (int base = 0; base < n_max; base += kMomSpan) {
-    MomLoad cur[kMomGroup];
-#pragma unroll
-    for (int g = 0; g < kMomGroup; ++g) cur[g] = nxt[g];
-    mom_load_group(d, base + kMomSpan, lane, nxt);
+  pass([&](int base, const MomLoad (&cur)[kMomGroup]) {
 #pragma unroll
     for (int g = 0; g < kMomGroup; ++g) {
       const int i = base + 64 * g + lane;
       if (i < n_e) exact_add(ex, cur[g].x);
-      if (i < nk) exact_add(el, levels[rank_lohi(cur[g].lo, cur[g].hi)]);
+      if (i < nk) exact_add(el, levels[rank_lohi(cur[g].lo, cur[g].hi4 >> 16)]);
     }
-  }
+  });
   double ev_sum, km_sum;
   const bool ex_ok = exact_total(ex, ev_sum), el_ok = exact_total(el, km_sum);
   if (!ex_ok || !el_ok) {
     double acc = 0.0;
-    mom_load_group(d, 0, lane, nxt);
-    for (int base = 0; base < n_max; base += kMomSpan) {
-      MomLoad cur[kMomGroup];
-#pragma unroll
-      for (int g = 0; g < kMomGroup; ++g) cur[g] = nxt[g];
-      mom_load_group(d, base + kMomSpan, lane, nxt);
+    pass([&](int base, const MomLoad (&cur)[kMomGroup]) {
 #pragma unroll
       for (int g = 0; g < kMomGroup; ++g) {
         const int i = base + 64 * g + lane;
         st[64 * g + lane] = i < n_e ? (double)cur[g].x : 0.0;
         st[kMomSecond + 64 * g + lane] =
-            i < nk ? (double)levels[rank_lohi(cur[g].lo, cur[g].hi)] : 0.0;
+            i < nk ? (double)levels[rank_lohi(cur[g].lo, cur[g].hi4 >> 16)] : 0.0;
       }
       wave_lds_sync();
       serial_sum_span(st + ((lane & 1) ? kMomSecond : 0), acc);
       wave_lds_sync();
-    }
+    });
     if (!ex_ok) ev_sum = readlane_f64(acc, 0);
     if (!el_ok) km_sum = readlane_f64(acc, 1);
   }
   const double shift = ev_sum / n_e - km_sum / nk;
 
   double acc = 0.0;
-  mom_load_group(d, 0, lane, nxt);
-  for (int base = 0; base < n_max; base += kMomSpan) {
-    MomLoad cur[kMomGroup];
-#pragma unroll
-    for (int g = 0; g < kMomGroup; ++g) cur[g] = nxt[g];
-    mom_load_group(d, base + kMomSpan, lane, nxt);
+  pass([&](int base, const MomLoad (&cur)[kMomGroup]) {
 #pragma unroll
     for (int g = 0; g < kMomGroup; ++g) {
       const int i = base + 64 * g + lane;
       double l2 = 0.0, d2 = 0.0;
       if (i < nk) {
-        const double l = levels[rank_lohi(cur[g].lo, cur[g].hi)];
+        const double l = levels[rank_lohi(cur[g].lo, cur[g].hi4 >> 16)];
         l2 = l * l;
       }
       if (i < n_e) {
@@ -933,7 +946,7 @@ __global__ __launch_bounds__(64 * kMomWaves) void abea_mom_kernel(MomArgs a) {
     wave_lds_sync();
     serial_sum_span(st + ((lane & 1) ? kMomSecond : 0), acc);
     wave_lds_sync();
-  }
+  });
   const double km_sq = readlane_f64(acc, 0), ev_sq = readlane_f64(acc, 1);
   const double scale = (ev_sq / n_e) / (km_sq / nk);
   if (lane == 0) {

@@ -138,6 +138,27 @@ def test_synthetic_and_edges_vs_oracle(ctx, form):
     assert compare_to_oracle(b, res) == []
 
 
+def test_mom_on_edge_reads(ctx):
+    """MOM over the edge reads, among them one of 5 bases (no k-mers) and one
+    with no events, whose loads read stand-ins: every read's scalings equal
+    the oracle's bit for bit, except where f5c's arithmetic divides by zero
+    (no events, no k-mers), where both are non-finite in the same fields (the
+    sign of a NaN differs between the host and the GPU)."""
+    import oracle
+    from abea_amd import engine
+    b = edge_batch()
+    plan = engine.Plan(ctx, engine.DeviceBatch.upload(b, 0), 1)
+    got = plan.estimate_scalings()[:b.n_reads].cpu().numpy()
+    plan.close()
+    with np.errstate(all="ignore"):
+        want = np.stack([oracle.estimate_scalings_mom(b.read_seq(i), b.read_events(i), b.model)
+                         for i in range(b.n_reads)])
+    degenerate = (b.n_events == 0) | (b.read_len < 6)
+    assert degenerate.sum() == 2
+    assert got[~degenerate].tobytes() == want[~degenerate].tobytes()
+    assert np.array_equal(np.isfinite(got[degenerate]), np.isfinite(want[degenerate]))
+
+
 def test_tunings_agree(ctx):
     """Every launch shape the autotuner may pick gives identical results."""
     b = edge_batch()
