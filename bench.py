@@ -42,15 +42,26 @@ HBM_PEAK_GBS = 8000.0
 W = 100
 # Per launch of the core kernel on the headline workload, from rocprofv3
 # (profiles/abea_kernels_batched16384_ln8kb.md gives the commands): VALU
-# wave-instructions (SQ_INSTS_VALU), the same at every launch shape, and HBM
-# bytes (FETCH_SIZE + WRITE_SIZE passes, scaled by a 1 GiB copy's readings)
-# per launch shape (kernel, waves per CU, traceback overlap).
-PROFILED_VALU = {"abea_core_kernel<1, true>": 2.05535e10}
-PROFILED_TRAFFIC = {
+# wave-instructions (SQ_INSTS_VALU), which depend on the traceback overlap
+# only, and HBM bytes (FETCH_SIZE + WRITE_SIZE passes, scaled by a 1 GiB
+# copy's readings) per launch shape (kernel, waves per CU, traceback overlap).
+# The autotuner's unfused candidates (a separate post kernel) have no entry.
+PROFILED_VALU = {  # (kernel, traceback overlap)
+    ("abea_core_kernel<1, true>", 128): 2.05535e10,
+    ("abea_core_kernel<2, true>", 128): 2.05537e10,
+    ("abea_core_kernel<4, true>", 128): 2.05537e10,
+    ("abea_core_kernel<1, true>", 64): 2.06509e10,
+    ("abea_core_kernel<1, true>", 256): 2.06105e10,
+}
+PROFILED_TRAFFIC = {  # (kernel, waves per CU, traceback overlap)
     ("abea_core_kernel<1, true>", 16, 128): 46036583104,
     ("abea_core_kernel<1, true>", 20, 128): 48777037088,
     ("abea_core_kernel<1, true>", 24, 128): 48763236352,
     ("abea_core_kernel<1, true>", 32, 128): 48762137568,
+    ("abea_core_kernel<2, true>", 32, 128): 48772339919,
+    ("abea_core_kernel<4, true>", 32, 128): 48716171416,
+    ("abea_core_kernel<1, true>", 24, 64): 48962427589,
+    ("abea_core_kernel<1, true>", 24, 256): 51101081867,
 }
 N_SIMD = 1024          # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # peak engine clock
@@ -200,7 +211,7 @@ def headline_leg(args, ctx):
     achieved = core_bytes / (kt[1] * 1e-3) / 1e9
     kname = kernel_name(tune)
     traffic = PROFILED_TRAFFIC.get((kname, tune["core_waves_per_cu"], tune["traceback_overlap"]))
-    valu = PROFILED_VALU.get(kname)
+    valu = PROFILED_VALU.get((kname, tune["traceback_overlap"]))
     out = {
         "metric": "ABEA GCUPS (batched persistent form, 1 GPU)",
         "value": round(gcups, 3),
