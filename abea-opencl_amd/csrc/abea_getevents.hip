@@ -150,8 +150,11 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
   const bool exact = !t.bad && (t.gx >= 1000 || t.abs_s < ldexp(1.0, 51 + t.gx)) &&
                      (t.gq >= 1000 || t.abs_q < ldexp(1.0, 51 + t.gq));
   // tame: an exact read whose samples are all below 2^60 in magnitude (ulp
-  // below 2^37), so that no window sum of squares overflows float
-  const bool tame = exact && t.hx <= 36;
+  // below 2^37), so that no window sum of squares overflows float, and whose
+  // samples and float squares are all zero or multiples of 2^-125 (ulp at
+  // least 2^-125), so that every non-zero window sum of either is at least
+  // 2^-125 in magnitude
+  const bool tame = exact && t.hx <= 36 && t.gx >= -125 && t.gq >= -125;
   if (lane == 0) {
     a.exact[r] = exact ? (tame ? 2 : 1) : 0;
     a.first_bad[r] = INT32_MAX;
@@ -192,20 +195,30 @@ __global__ __launch_bounds__(256) void abea_ge_read_scan_kernel(GeArgs a) {
 
 // Division by d = 3 or 6 (3 times a power of two) for the statistics of tame
 // reads (below): a product by y = RN(1/d) and one fused residual correction,
-// Markstein's final step.  RN(1/3) = (1/3)(1 - 2^-54) in double, so x*y
+// Markstein's final step.  In double, RN(1/3) = (1/3)(1 - 2^-54), so x*y
 // rounds to within an ulp of x/d, and the corrected quotient is the correctly
-// rounded one when it is a normal number.  A float quotient is taken the
-// same way in double and then rounded to float: the double quotient is
-// exact enough that the second rounding gives the float quotient.  Both are
-// checked against IEEE division (tests/native/division_check.c): the float
-// route exhaustively over all finite floats (equal but for the sign of a
-// zero quotient, which no statistic sees: it enters only squares and |m2 -
-// m1|), the double one on the inputs a tame read gives.
+// rounded one when it is a normal number (div3k).  Float quotients take the
+// same step in float (div3f): equal to IEEE float division over every finite
+// float for d = 3, and for d = 6 over every float of magnitude at least
+// 2^-125 (smaller ones, whose quotients are subnormal, can round the other
+// way), in both cases but for the sign of a zero quotient, which no statistic
+// sees (it enters only squares and |m2 - m1|).  A tame read's window sums
+// and sums of squares are zero or at least 2^-125 in magnitude; the variance
+// may be FLT_MIN, so its quotient by 6 goes through double (div3k, then
+// rounded to float: the double quotient is exact enough that the second
+// rounding gives the float quotient).  tests/native/div3_check.c compares
+// each route with IEEE division.
 constexpr double kRcp3 = 1.0 / 3.0, kRcp6 = 1.0 / 6.0;
+constexpr float kRcp3f = 1.0f / 3.0f, kRcp6f = 1.0f / 6.0f;
 
 __device__ __forceinline__ double div3k(double x, double d, double y) {
   const double q0 = x * y;
   return __builtin_fma(__builtin_fma(-q0, d, x), y, q0);
+}
+
+__device__ __forceinline__ float div3f(float x, float d, float y) {
+  const float q0 = x * y;
+  return __builtin_fmaf(__builtin_fmaf(-q0, d, x), y, q0);
 }
 
 // Correctly rounded square root of a double in [2^-767, 2^1023): the
@@ -225,18 +238,42 @@ __device__ __forceinline__ double sqrt_mid(double x) {
   return __builtin_fma(d, h, g);
 }
 
+// Correctly rounded a / b for a in {0} u [2^-149, 2^128) and b in [2^-65,
+// 2^65]: the sequence LLVM lowers a double division to on AMDGPU (hardware
+// reciprocal estimate, two Newton steps, quotient and one fused correction)
+// without its v_div_scale, v_div_fmas and v_div_fixup steps.  For these
+// operands v_div_scale returns its input unchanged (no operand or quotient is
+// near the ends of the double range), so v_div_fmas is a plain fma, and
+// v_div_fixup changes no finite, non-zero-divisor result; a = 0 gives +0, as
+// IEEE division does.
+__device__ __forceinline__ double div_mid(double a, double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double r1 = __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+  const double r2 = __builtin_fma(r1, __builtin_fma(-b, r1, 1.0), r1);
+  const double m = a * r2;
+  return __builtin_fma(__builtin_fma(-b, m, a), r2, m);
+}
+
 // The statistic from the window before the sample (s1, q1: sums of the
 // samples and of their float squares, double) and the window from it (s2, q2,
 // rounded to float as the C source does).  TAME: the inputs of a tame read,
-// whose divisions by w take the route above (the sums are finite, and zero
-// or at least 2^-149 in magnitude); otherwise IEEE division.
+// whose divisions by w take the routes above; otherwise IEEE division.  The
+// final quotient's numerator |m2 - m1| is a float, and its divisor the
+// square root of a float of at least FLT_MIN / 6, so it is in div_mid's
+// range.
 template <bool TAME>
 __device__ __forceinline__ float tstat_tail(double s1, double q1, float s2, float q2, int w) {
   const float wf = (float)w;
   const double wd = (double)w;
   const double yd = w == kWShort ? kRcp3 : kRcp6;
+  const float yf = w == kWShort ? kRcp3f : kRcp6f;
   auto d64 = [&](double x) { return TAME ? div3k(x, wd, yd) : x / wd; };
-  auto d32 = [&](float x) { return TAME ? (float)div3k((double)x, wd, yd) : x / wf; };
+  // window sums and sums of squares (zero or at least 2^-125 in a tame read)
+  auto d32 = [&](float x) { return TAME ? div3f(x, wf, yf) : x / wf; };
+  // the variance, at least FLT_MIN: the float route only for w = 3
+  auto d32v = [&](float x) {
+    return TAME ? (w == kWShort ? div3f(x, wf, yf) : (float)div3k((double)x, wd, yd)) : x / wf;
+  };
   const float m1 = (float)d64(s1);
   const float m2 = d32(s2);
   double cv = d64(q1);
@@ -245,8 +282,8 @@ __device__ __forceinline__ float tstat_tail(double s1, double q1, float s2, floa
   cv -= (double)(m2 * m2);
   const float var = fmaxf((float)cv, FLT_MIN);
   const float dm = m2 - m1;
-  const double v = (double)d32(var);
-  return (float)(fabs((double)dm) / (TAME ? sqrt_mid(v) : sqrt(v)));
+  const double v = (double)d32v(var);
+  return (float)(TAME ? div_mid(fabs((double)dm), sqrt_mid(v)) : fabs((double)dm) / sqrt(v));
 }
 
 // The statistic from the prefix sums (a read whose sums are not exact).
@@ -267,7 +304,7 @@ struct GeRead {
   const double *S, *Q;   // prefix sums (reads whose sums are not exact)
   int32_t n;
   bool exact;            // window sums of the samples are exact in any order
-  bool tame;             // exact, and every sample below 2^60 in magnitude
+  bool tame;             // exact, samples below 2^60, samples and squares on a 2^-125 grid
   int64_t avail;         // samples readable from x on: this read's and the later reads'
 };
 

@@ -178,7 +178,8 @@ struct GeArgs {
   GeChunkSum *csum;           // per prefix chunk
   double *carry;              // per prefix chunk: S and Q before it (2 doubles)
   int32_t *exact;             // per read: 1 when the parallel sums are exact, 2 when also
-                              // every sample is below 2^60 in magnitude (tame)
+                              // every sample is below 2^60 in magnitude and every sample
+                              // and float square is a multiple of 2^-125 (tame)
   const int64_t *det_off;     // n_reads + 1: read r's detector chunks
   const int32_t *det_read;    // the read of each detector chunk
   int32_t *peaks;             // det_chunk boundaries per detector chunk, in emission order

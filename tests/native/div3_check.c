@@ -1,14 +1,20 @@
 /* Checks the divisions by w = 3 and 6 that event detection takes for the
- * t-statistics of tame reads (abea_getevents.hip, div3k):
- *   q = fma(fma(-q0, d, x), y, q0),  q0 = x * y,  y = RN(1/d)   (double)
- * (1) as a double quotient, against IEEE double division, on doubles that
+ * t-statistics of tame reads (abea_getevents.hip, div3k and div3f):
+ *   q = fma(fma(-q0, d, x), y, q0),  q0 = x * y,  y = RN(1/d)
+ * (1) in double (div3k), against IEEE double division, on doubles that
  *     are sums of floats: all 2^24 signed significands at several exponents
  *     and random ones over the whole float range;
- * (2) as a float quotient (a float taken to double, divided as above, rounded
- *     to float), against IEEE float division: all signed significands at
- *     the smallest normal, a middle and the largest exponent, and all
- *     subnormals.  (The same comparison over all 2^32 floats differs only
- *     for -0, whose quotient comes out +0.)
+ * (2) a float taken to double, divided as in (1), rounded to float, against
+ *     IEEE float division: all signed significands at the smallest normal, a
+ *     middle and the largest exponent, and all subnormals.  (The same
+ *     comparison over all 2^32 floats differs only for -0, whose quotient
+ *     comes out +0.)
+ * (3) in float (div3f), against IEEE float division: for d = 3 all signed
+ *     significands at the subnormal and the lowest, middle and highest
+ *     exponents; for d = 6 the same at the exponents of magnitudes from
+ *     2^-125 up.  (Over all 2^32 floats, d = 3 differs only for -0; d = 6
+ *     also differs for some magnitudes below 2^-125, whose quotients are
+ *     subnormal, and those are not taken this way.)
  * Zero quotients compare equal whatever their sign.  Exit status 0 = no
  * mismatch. */
 #include <math.h>
@@ -25,6 +31,11 @@ static uint64_t next(void) { rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17
 static double div3k(double x, double d, double y) {
   double q0 = x * y;
   return fma(fma(-q0, d, x), y, q0);
+}
+
+static float div3f(float x, float d, float y) {
+  float q0 = x * y;
+  return fmaf(fmaf(-q0, d, x), y, q0);
 }
 
 int main(void) {
@@ -46,6 +57,26 @@ int main(void) {
           ++tested;
           if (fb(q) != fb(ref) && !(q == 0.0f && ref == 0.0f)) {
             if (bad < 5) printf("float route: d=%g x=%a got %a want %a\n", d, x, q, ref);
+            ++bad;
+          }
+        }
+    /* (3) float route: every exponent listed, both signs, all significands;
+     * for d = 6 only magnitudes from 2^-125 up (biased exponent 2) */
+    volatile float yfv = 1.0f / df;
+    const float yf = yfv;
+    const uint32_t fexps3[] = {0, 1, 2, 3, 64, 127, 128, 200, 253, 254};
+    const uint32_t fexps6[] = {2, 3, 4, 64, 127, 128, 200, 253, 254};
+    const uint32_t *fe = di == 0 ? fexps3 : fexps6;
+    const unsigned nfe = di == 0 ? sizeof fexps3 / sizeof fexps3[0] : sizeof fexps6 / sizeof fexps6[0];
+    for (unsigned e = 0; e < nfe; ++e)
+      for (uint32_t m = 0; m < (1u << 23); ++m)
+        for (uint32_t s = 0; s < 2; ++s) {
+          const float x = bf((s << 31) | (fe[e] << 23) | m);
+          volatile float ref = x / df;
+          const float q = div3f(x, df, yf);
+          ++tested;
+          if (fb(q) != fb(ref) && !(q == 0.0f && ref == 0.0f)) {
+            if (bad < 5) printf("float route (div3f): d=%g x=%a got %a want %a\n", d, x, q, ref);
             ++bad;
           }
         }

@@ -23,9 +23,10 @@ def test_reciprocal_fma_quotient_equals_ieee_division(tmp_path):
 
 def test_division_by_three_and_six_equals_ieee_division(tmp_path):
     """Event detection divides the window sums of tame reads by w = 3 and 6
-    with a correctly rounded reciprocal and one fused correction, in double
-    (and rounds to float where the C source divides in float); the check
-    compares both routes with IEEE division (tests/native/div3_check.c)."""
+    with a correctly rounded reciprocal and one fused correction: in double
+    where the C source divides doubles, in float (or in double, rounded to
+    float, for the variance over 6) where it divides floats; the check
+    compares every route with IEEE division (tests/native/div3_check.c)."""
     src = os.path.join(ROOT, "tests", "native", "div3_check.c")
     exe = str(tmp_path / "div3_check")
     with open("/proc/cpuinfo") as f:
