@@ -136,11 +136,18 @@ using abea::transition_constants;
 
 // Event-detection scratch of a context (grows, never shrinks).
 struct GeScratch {
-  DevBuf raw_off, chunk_off, chunk_read, det_off, det_read, S, Q, ev_base, prev_last, csum, carry,
-      exact, peaks, n_peaks, entry, exit, first_bad, reruns;
-  std::vector<int64_t> raw_off_h, chunk_off_h, det_off_h;
-  std::vector<int32_t> chunk_read_h, det_read_h;
+  // the call's chunk tables (read offsets, prefix and detector chunk offsets,
+  // then the read of each chunk), staged in pinned memory and copied to the
+  // device as one block
+  PinBuf tab_h;
+  DevBuf tab;
+  hipEvent_t tab_copied = nullptr;  // the last call's copy out of tab_h
+  DevBuf S, Q, ev_base, prev_last, csum, carry, exact, peaks, n_peaks, entry, exit, first_bad,
+      reruns;
   int32_t n_reads = 0;
+  ~GeScratch() {
+    if (tab_copied) (void)hipEventDestroy(tab_copied);
+  }
 };
 
 struct abea_ctx {
@@ -308,31 +315,24 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
   const int32_t D = ctx->ge_chunk > 0 ? ctx->ge_chunk
                                       : pick_det_chunk(n, raw_off_h, ctx->n_cu, ctx->ge_overlap);
   ctx->ge_last_chunk = D;
-  g.raw_off_h.assign(raw_off_h, raw_off_h + n + 1);
-  g.chunk_off_h.assign((size_t)n + 1, 0);
-  g.det_off_h.assign((size_t)n + 1, 0);
-  g.chunk_read_h.clear();
-  g.det_read_h.clear();
+  if (!raw && raw_off_h[n] > 0) return fail(ABEA_ERR_INVALID, "getevents: null raw signal");
+  int64_t nc = 0, nd = 0;
   for (int32_t r = 0; r < n; ++r) {
     const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
-    const int64_t nch = (len + abea::kGeChunk - 1) / abea::kGeChunk;
-    const int64_t ndc = (len + D - 1) / D;
-    g.chunk_off_h[r + 1] = g.chunk_off_h[r] + nch;
-    g.det_off_h[r + 1] = g.det_off_h[r] + ndc;
-    g.chunk_read_h.insert(g.chunk_read_h.end(), (size_t)nch, r);
-    g.det_read_h.insert(g.det_read_h.end(), (size_t)ndc, r);
+    nc += (len + abea::kGeChunk - 1) / abea::kGeChunk;
+    nd += (len + D - 1) / D;
   }
-  if (!raw && raw_off_h[n] > 0) return fail(ABEA_ERR_INVALID, "getevents: null raw signal");
-  const int64_t nc = g.chunk_off_h[n], nd = g.det_off_h[n];
   const int64_t extent = raw_off_h[n];
   const size_t nc1 = (size_t)std::max<int64_t>(nc, 1), nd1 = (size_t)std::max<int64_t>(nd, 1);
+  // the tables: three int64 arrays of n + 1, then the int32 chunk-to-read maps
+  const size_t off_bytes = sizeof(int64_t) * (size_t)(n + 1);
+  const size_t tab_bytes = 3 * off_bytes + sizeof(int32_t) * (size_t)(nc + nd);
   hipStream_t s = ctx->stream;
   abea_status_t st;
-  if ((st = g.raw_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
-      (st = g.chunk_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
-      (st = g.det_off.ensure(sizeof(int64_t) * (size_t)(n + 1))) ||
-      (st = g.chunk_read.ensure(sizeof(int32_t) * nc1)) ||
-      (st = g.det_read.ensure(sizeof(int32_t) * nd1)) ||
+  if (!g.tab_copied) HIP_TRY(hipEventCreateWithFlags(&g.tab_copied, hipEventDisableTiming));
+  // the previous call's copy may still read the staging block
+  HIP_TRY(hipEventSynchronize(g.tab_copied));
+  if ((st = g.tab_h.ensure(tab_bytes)) || (st = g.tab.ensure(tab_bytes)) ||
       (st = g.S.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
       (st = g.Q.ensure(sizeof(double) * (size_t)(extent + n + 1))) ||
       (st = g.ev_base.ensure(sizeof(int32_t) * nd1)) ||
@@ -347,37 +347,49 @@ abea_status_t ge_enqueue(abea_ctx *ctx, int32_t n, const int64_t *raw_off_h, con
       (st = g.first_bad.ensure(sizeof(int32_t) * (size_t)n)) ||
       (st = g.reruns.ensure(sizeof(int32_t) * (size_t)n)))
     return st;
-  HIP_TRY(hipMemcpyAsync(g.raw_off.p, g.raw_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(g.chunk_off.p, g.chunk_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(g.det_off.p, g.det_off_h.data(), sizeof(int64_t) * (size_t)(n + 1),
-                         hipMemcpyHostToDevice, s));
-  if (nc)
-    HIP_TRY(hipMemcpyAsync(g.chunk_read.p, g.chunk_read_h.data(), sizeof(int32_t) * (size_t)nc,
-                           hipMemcpyHostToDevice, s));
-  if (nd)
-    HIP_TRY(hipMemcpyAsync(g.det_read.p, g.det_read_h.data(), sizeof(int32_t) * (size_t)nd,
-                           hipMemcpyHostToDevice, s));
+  int64_t *raw_off_t = g.tab_h.as<int64_t>();
+  int64_t *chunk_off_t = raw_off_t + (n + 1);
+  int64_t *det_off_t = chunk_off_t + (n + 1);
+  int32_t *chunk_read_t = reinterpret_cast<int32_t *>(det_off_t + (n + 1));
+  int32_t *det_read_t = chunk_read_t + nc;
+  std::memcpy(raw_off_t, raw_off_h, off_bytes);
+  chunk_off_t[0] = 0;
+  det_off_t[0] = 0;
+  for (int32_t r = 0; r < n; ++r) {
+    const int64_t len = raw_off_h[r + 1] - raw_off_h[r];
+    const int64_t nch = (len + abea::kGeChunk - 1) / abea::kGeChunk;
+    const int64_t ndc = (len + D - 1) / D;
+    for (int64_t k = 0; k < nch; ++k) chunk_read_t[chunk_off_t[r] + k] = r;
+    for (int64_t k = 0; k < ndc; ++k) det_read_t[det_off_t[r] + k] = r;
+    chunk_off_t[r + 1] = chunk_off_t[r] + nch;
+    det_off_t[r + 1] = det_off_t[r] + ndc;
+  }
+  HIP_TRY(hipMemcpyAsync(g.tab.p, g.tab_h.p, tab_bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(g.tab_copied, s));
+  const int64_t *raw_off_d = g.tab.as<int64_t>();
+  const int64_t *chunk_off_d = raw_off_d + (n + 1);
+  const int64_t *det_off_d = chunk_off_d + (n + 1);
+  const int32_t *chunk_read_d = reinterpret_cast<const int32_t *>(det_off_d + (n + 1));
+  const int32_t *det_read_d = chunk_read_d + nc;
   abea::GeArgs a{};
   a.n_reads = n;
   a.overlap = ctx->ge_overlap;
   a.det_chunk = D;
   a.n_chunks = nc;
   a.n_det = nd;
-  a.raw_off = g.raw_off.as<int64_t>();
+  a.raw_off = raw_off_d;
   a.raw = raw;
   a.S = g.S.as<double>();
   a.Q = g.Q.as<double>();
   a.ev_base = g.ev_base.as<int32_t>();
   a.prev_last = g.prev_last.as<int32_t>();
-  a.chunk_off = g.chunk_off.as<int64_t>();
-  a.chunk_read = g.chunk_read.as<int32_t>();
+  a.chunk_off = chunk_off_d;
+  a.chunk_read = chunk_read_d;
   a.csum = g.csum.as<abea::GeChunkSum>();
   a.carry = g.carry.as<double>();
   a.exact = g.exact.as<int32_t>();
-  a.det_off = g.det_off.as<int64_t>();
-  a.det_read = g.det_read.as<int32_t>();
+  a.det_off = det_off_d;
+  a.det_read = det_read_d;
   a.peaks = g.peaks.as<int32_t>();
   a.n_peaks = g.n_peaks.as<int32_t>();
   a.entry = g.entry.as<abea::GeState>();
