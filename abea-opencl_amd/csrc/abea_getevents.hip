@@ -93,8 +93,22 @@ __device__ __forceinline__ void reduce_sum_info(GeChunkSum &c) {
   c.hx = wave_max_i32(c.hx);
 }
 
+// ulp exponent of the float whose magnitude has bits u (ulp_exp's values)
+__device__ __forceinline__ int ulp_exp_bits(uint32_t u) {
+  if (u == 0) return 1000;
+  const int e = (int)(u >> 23);
+  if (e >= 255) return 2000;
+  return e == 0 ? -149 : e - 150;
+}
+
 // One wave per prefix chunk (4 per block): the chunk's sums and exactness
-// data, read 64 consecutive samples per step.
+// data, read 64 consecutive samples per step.  The ulp exponent of a finite
+// float grows with the bits of its magnitude, so each lane keeps the
+// smallest and largest magnitude bits of its samples and of their squares
+// (zero counted as the largest unsigned value in the minimum) and converts
+// them once: the smallest ulp exponent of the non-zero values, the largest
+// one, and whether an inf or NaN (magnitude bits >= 0x7f800000) occurred.
+// The squares are non-negative, so their sum of magnitudes is their sum.
 __global__ __launch_bounds__(256) void abea_ge_chunk_sum_kernel(GeArgs a) {
   const int lane = threadIdx.x & 63;
   const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -104,21 +118,31 @@ __global__ __launch_bounds__(256) void abea_ge_chunk_sum_kernel(GeArgs a) {
   const int64_t n = a.raw_off[r + 1] - off;
   const int64_t c0 = (c - a.chunk_off[r]) * kGeChunk, c1 = min(c0 + kGeChunk, n);
   const float *x = a.raw + off;
-  GeChunkSum cs{0.0, 0.0, 0.0, 0.0, 1000, 1000, 0, -1000};
+  double s = 0.0, q = 0.0, abs_s = 0.0;
+  uint32_t lo_x = 0xffffffffu, hi_x = 0u, lo_q = 0xffffffffu, hi_q = 0u;
 #pragma unroll 4
   for (int64_t i = c0 + lane; i < c1; i += 64) {
     const float v = x[i];
     const float sq = v * v;
-    const int ex = ulp_exp(v), eq = ulp_exp(sq);
-    cs.bad |= (ex == 2000) | (eq == 2000);
-    cs.gx = min(cs.gx, ex);
-    cs.hx = max(cs.hx, ex == 1000 ? -1000 : ex);
-    cs.gq = min(cs.gq, eq);
-    cs.s += (double)v;
-    cs.q += (double)sq;
-    cs.abs_s += fabs((double)v);
-    cs.abs_q += (double)sq;
+    const uint32_t ux = __builtin_bit_cast(uint32_t, v) & 0x7fffffffu;
+    const uint32_t uq = __builtin_bit_cast(uint32_t, sq) & 0x7fffffffu;
+    lo_x = min(lo_x, ux - 1u);  // zero wraps to the largest value
+    hi_x = max(hi_x, ux);
+    lo_q = min(lo_q, uq - 1u);
+    hi_q = max(hi_q, uq);
+    s += (double)v;
+    q += (double)sq;
+    abs_s += fabs((double)v);
   }
+  GeChunkSum cs;
+  cs.s = s;
+  cs.q = q;
+  cs.abs_s = abs_s;
+  cs.abs_q = q;
+  cs.gx = lo_x == 0xffffffffu ? 1000 : ulp_exp_bits(lo_x + 1u);
+  cs.gq = lo_q == 0xffffffffu ? 1000 : ulp_exp_bits(lo_q + 1u);
+  cs.bad = (hi_x >= 0x7f800000u) | (hi_q >= 0x7f800000u);
+  cs.hx = hi_x == 0u ? -1000 : ulp_exp_bits(hi_x);
   reduce_sum_info(cs);
   if (lane == 0) a.csum[c] = cs;
 }

@@ -3,7 +3,8 @@ oracle restatement (oracle/getevents.c): event tables byte for byte.
 
 Covers signal lengths around the window and chunk sizes, empty and tiny
 signals, the serial prefix-sum path (terms too fine for an exact parallel
-sum), exact signals too large for the statistics' fast divisions, forced
+sum), exact signals too large or too fine for the statistics' fast
+divisions, all-zero and subnormal signals, forced
 speculation misses (no lead-in), dyadic step signals with known cuts, reads
 at a non-zero offset, and the host drop-ins."""
 import ctypes as C
@@ -173,6 +174,28 @@ def test_step_signals_and_offset_reads(ctx):
     for c, g in zip(cuts, [got[0], got[2], got[3]]):
         assert np.array_equal(g["start"][1:].astype(np.int64), c[:-1])
         assert g.shape[0] == max(len(c), 1)
+
+
+def test_tiny_scales_and_the_tame_boundary(ctx):
+    """Signals scaled by powers of two on either side of the tame reads'
+    limit (samples and float squares on a 2^-125 grid): x 2^-50 stays tame
+    (squares near 2^-86), x 2^-62 and x 2^-80 are exact but not tame
+    (squares below the grid; subnormal at 2^-80); an all-zero read and a
+    zero read with a few subnormal samples are exact.  Each read equals the
+    oracle; the tame one is cut where the unscaled signal is."""
+    base = _signals([5000, 3000, 4000], seed=31)
+    sigs = [(base[0] * np.float32(2.0 ** -50)).astype(np.float32),
+            (base[1] * np.float32(2.0 ** -62)).astype(np.float32),
+            (base[2] * np.float32(2.0 ** -80)).astype(np.float32),
+            np.zeros(3000, np.float32), np.zeros(2500, np.float32)]
+    sigs[4][::301] = np.float32(2.0 ** -149)
+    got = _run_device(ctx, sigs)
+    for i, (s, g) in enumerate(zip(sigs, got)):
+        want = oracle.getevents(s)
+        assert _events_equal(g, want), f"read {i}: {_first_diff(g, want)}"
+    unscaled = _run_device(ctx, [base[0]])[0]
+    assert np.array_equal(got[0]["start"], unscaled["start"])
+    assert np.array_equal(got[0]["length"], unscaled["length"])
 
 
 def test_mixed_kinds_in_one_call(ctx):
