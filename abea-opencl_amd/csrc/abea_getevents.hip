@@ -412,15 +412,58 @@ __device__ __forceinline__ float ge_x(const GeRead &rd, int32_t j) {
   return (uint32_t)j < (uint32_t)rd.n ? v : 0.0f;
 }
 
-// One sample of the detector; boundaries at samples >= keep_from go to pk.
+// Where a lane's boundaries go: its chunk's list pk, entry by entry
+// (DirectSink), or through a buffer of its own in LDS that goes out kStage
+// entries (one 128-byte line) at a time (StagedSink).  Stored one by one,
+// the boundaries of a CU's lanes touch as many lines as there are lanes, and
+// the caches write each line back partly filled, many times over.
+struct DirectSink {
+  int32_t *pk;
+  __device__ __forceinline__ void put(int32_t k, int32_t, int32_t p) { pk[k] = p; }
+  __device__ __forceinline__ void settle(int32_t, int32_t) {}
+  __device__ __forceinline__ void finish(int32_t) {}
+};
+
+constexpr int kStage = 32;              // entries per write-out (a power of two)
+constexpr int kStageLane = kStage + 1;  // LDS words per lane (odd: no bank conflicts)
+
+// Entries below the multiple of kStage at or under the count are in pk; the
+// rest, at most kStage + 1 while a step adds its (at most two) entries, are
+// in the lane's buffer from its start.
+struct StagedSink {
+  int32_t *pk;  // the chunk's list
+  int32_t *lb;  // the lane's kStageLane words of LDS
+  // entry k of a step that began with `before` entries
+  __device__ __forceinline__ void put(int32_t k, int32_t before, int32_t p) {
+    lb[k - (before & ~(kStage - 1))] = p;
+  }
+  __device__ __forceinline__ void settle(int32_t before, int32_t cnt) {
+    const int32_t done = before & ~(kStage - 1);
+    if ((cnt & ~(kStage - 1)) != done) {
+#pragma unroll
+      for (int j = 0; j < kStage; ++j) pk[done + j] = lb[j];
+      lb[0] = lb[kStage];
+    }
+  }
+  __device__ __forceinline__ void finish(int32_t cnt) {
+    const int32_t done = cnt & ~(kStage - 1);
+    for (int32_t j = 0; j < cnt - done; ++j) pk[done + j] = lb[j];
+  }
+};
+
+// One sample of the detector; boundaries at samples >= keep_from go to the
+// sink.
+template <class Sink>
 __device__ __forceinline__ void ge_step(GeState &st, float t1, float t2, int32_t i,
-                                        int32_t keep_from, int32_t *pk, int32_t &cnt,
+                                        int32_t keep_from, Sink &pk, int32_t &cnt,
                                         GeState *entry) {
   int32_t p0 = 0, p1 = 0;
   const int ne = detect_step(st, t1, t2, i, p0, p1);
-  if (i >= keep_from) {
-    if (ne > 0) pk[cnt++] = p0;
-    if (ne > 1) pk[cnt++] = p1;
+  if (i >= keep_from && ne > 0) {
+    const int32_t before = cnt;
+    pk.put(cnt++, before, p0);
+    if (ne > 1) pk.put(cnt++, before, p1);
+    pk.settle(before, cnt);
   }
   if (entry && i + 1 == keep_from) *entry = st;
 }
@@ -448,9 +491,9 @@ __device__ __forceinline__ float keep_if(float v, bool c) {
 // interleaves (each is a long chain of dependent double operations), then
 // the six detector steps.  CHECK: an iteration may load past the read's end
 // (within 18 samples of it).  TAME: the read is tame (tstat_tail).
-template <bool CHECK, bool TAME>
+template <bool CHECK, bool TAME, class Sink>
 __device__ void run_exact(GeState &st, const GeRead &rd, int32_t from, int32_t to,
-                          int32_t keep_from, int32_t *pk, int32_t &cnt, GeState *entry) {
+                          int32_t keep_from, Sink &pk, int32_t &cnt, GeState *entry) {
   const int32_t n = rd.n;
   auto win = [&](int32_t i, int w, double &s, double &q) {
     s = 0.0;
@@ -514,8 +557,9 @@ __device__ void run_exact(GeState &st, const GeRead &rd, int32_t from, int32_t t
 // samples >= keep_from go to pk (count in *np), the state before keep_from
 // to *entry.  The statistics come with the samples: from sliding window sums
 // (exact reads) or from the prefix sums.
+template <class Sink>
 __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_t to,
-                             int32_t keep_from, int32_t *pk, int32_t *np, GeState *entry) {
+                             int32_t keep_from, Sink &pk, int32_t *np, GeState *entry) {
   int32_t cnt = 0;
   if (entry && from == keep_from) *entry = st;
   if (rd.exact) {
@@ -527,14 +571,14 @@ __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_
     const bool check = (int64_t)to + 3 * kWLong > rd.avail;
     if (rd.tame) {
       if (check)
-        run_exact<true, true>(st, rd, from, to, keep_from, pk, cnt, entry);
+        run_exact<true, true, Sink>(st, rd, from, to, keep_from, pk, cnt, entry);
       else
-        run_exact<false, true>(st, rd, from, to, keep_from, pk, cnt, entry);
+        run_exact<false, true, Sink>(st, rd, from, to, keep_from, pk, cnt, entry);
     } else {
       if (check)
-        run_exact<true, false>(st, rd, from, to, keep_from, pk, cnt, entry);
+        run_exact<true, false, Sink>(st, rd, from, to, keep_from, pk, cnt, entry);
       else
-        run_exact<false, false>(st, rd, from, to, keep_from, pk, cnt, entry);
+        run_exact<false, false, Sink>(st, rd, from, to, keep_from, pk, cnt, entry);
     }
   } else {
     for (int32_t i = from; i < to; ++i) {
@@ -543,6 +587,7 @@ __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_
       ge_step(st, t1, t2, i, keep_from, pk, cnt, entry);
     }
   }
+  pk.finish(cnt);
   *np = cnt;
 }
 
@@ -551,6 +596,7 @@ __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_
 // boundaries (D >= 8): a detector that emits is back to no peak and needs 3
 // more samples to emit again.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void abea_ge_detect_kernel(GeArgs a) {
+  __shared__ int32_t stage[kStageLane * 256];  // each lane's boundary buffer
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_det) return;
   const int r = a.det_read[c];
@@ -562,7 +608,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void a
   GeState st = fresh_state();
   const GeRead rd{a.raw + off, a.S + off + r, a.Q + off + r, n, a.exact[r] != 0,
                   a.exact[r] == 2, a.raw_off[a.n_reads] - off};
-  run_detector(st, rd, from, c1, c0, a.peaks + c * a.det_chunk, a.n_peaks + c, a.entry + c);
+  StagedSink sink{a.peaks + c * a.det_chunk, stage + kStageLane * threadIdx.x};
+  run_detector(st, rd, from, c1, c0, sink, a.n_peaks + c, a.entry + c);
   a.exit[c] = st;
 }
 
@@ -597,8 +644,8 @@ __global__ __launch_bounds__(256) void abea_ge_fix_kernel(GeArgs a) {
         st = a.exit[c];
       } else {
         ++reruns;
-        run_detector(st, rd, c0, min(c0 + a.det_chunk, n), c0, a.peaks + c * a.det_chunk,
-                     a.n_peaks + c, nullptr);
+        DirectSink sink{a.peaks + c * a.det_chunk};
+        run_detector(st, rd, c0, min(c0 + a.det_chunk, n), c0, sink, a.n_peaks + c, nullptr);
       }
     }
   }
