@@ -271,13 +271,13 @@ void *abea_ctx_stream(abea_ctx_t *c) { return c ? (void *)c->stream : nullptr; }
 namespace {
 
 // Detector chunk size for a call.  The detector kernel is bound by VALU issue
-// and runs four 4-wave blocks per CU, one lane per chunk, each lane walking
+// and runs kGeDetBlocksPerCU 4-wave blocks per CU, one lane per chunk, each lane walking
 // its chunk plus the lead-in; the kernel takes about (rounds of resident
 // blocks) x (chunk + lead-in) sample steps.  The chunk is chosen to fill the
 // last round, among multiples of 32 from kGeDetChunkMin to kGeDetChunkMax
 // (ties to the larger chunk: fewer chunks to check and re-run).
 int32_t pick_det_chunk(int32_t n, const int64_t *raw_off_h, int n_cu, int32_t overlap) {
-  const int64_t slots = (int64_t)std::max(n_cu, 1) * 4;  // resident 256-lane blocks
+  const int64_t slots = (int64_t)std::max(n_cu, 1) * abea::kGeDetBlocksPerCU;  // resident blocks
   int32_t best = abea::kGeDetChunk;
   double best_cost = 1e300;
   for (int32_t D = abea::kGeDetChunkMin; D <= abea::kGeDetChunkMax; D += 32) {

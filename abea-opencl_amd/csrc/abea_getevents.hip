@@ -595,7 +595,7 @@ __device__ void run_detector(GeState &st, const GeRead &rd, int32_t from, int32_
 // chunk.  A chunk of D samples holds at most 2 * (ceil(D / 3) + 1) <= D
 // boundaries (D >= 8): a detector that emits is back to no peak and needs 3
 // more samples to emit again.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void abea_ge_detect_kernel(GeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGeDetBlocksPerCU))) void abea_ge_detect_kernel(GeArgs a) {
   __shared__ int32_t stage[kStageLane * 256];  // each lane's boundary buffer
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_det) return;

@@ -145,6 +145,9 @@ constexpr int kGeChunk = 2048;
 constexpr int kGeDetChunk = 512;
 constexpr int kGeDetChunkMin = 256, kGeDetChunkMax = 4096;
 constexpr int kGeOverlap = 64;  // samples a detector chunk's speculative run starts early
+// resident 256-lane detector blocks per CU (waves per SIMD): the detector
+// kernel's occupancy bound, and the unit the chunk size is chosen for
+constexpr int kGeDetBlocksPerCU = 4;
 struct GeDet {
   int32_t pos;        // -1: no peak yet
   float val;          // running minimum, or the peak's value
