@@ -728,11 +728,19 @@ __device__ __forceinline__ abea_event_t make_event_exact(const GeArgs &a, const 
                                                          int64_t end) {
   double ds, dq;
   if (start <= end && end - start <= 2 * kGeChunk) {
-    // four samples per step into four partial sums (exact in any order),
-    // their loads in flight together
-    // (the last step's missing samples count as +0.0, which changes no sum)
+    // four partial sums (exact in any order).  The first 16 samples, which
+    // hold most events whole, load together; the rest four per step.
+    // Missing samples count as +0.0, which changes no sum.
     double s[4] = {0.0, 0.0, 0.0, 0.0}, q[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t k = start; k < end; k += 4) {
+    float v0[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v0[u] = start + u < end ? x[start + u] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      s[u & 3] += (double)v0[u];
+      q[u & 3] += (double)(v0[u] * v0[u]);
+    }
+    for (int64_t k = start + 16; k < end; k += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const float v = k + u < end ? x[k + u] : 0.0f;
