@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1228,6 +1229,7 @@ struct HostBatch {
   std::vector<int64_t> seq_ptr, ev_ptr;
   std::vector<int32_t> read_len, n_events;
   int64_t n_bases = 0, n_ev = 0;
+  bool staged = false;  // bases and event means already copied to r->seq / r->mean
 };
 
 // Runs a flattened batch.  On return the reads' pairs sit back to back in the
@@ -1240,10 +1242,10 @@ abea_status_t run_host_batch(HostRun *r, int form, const HostBatch &hb,
   hipStream_t s = r->ctx->stream;
   abea_status_t st;
   HIP_TRY(hipEventRecord(r->ev[0], s));
-  if ((st = upload(r->seq, r->h_seq.as<char>(), (size_t)hb.n_bases, s)) ||
+  if ((!hb.staged && (st = upload(r->seq, r->h_seq.as<char>(), (size_t)hb.n_bases, s))) ||
       (st = upload(r->seq_ptr, hb.seq_ptr.data(), hb.seq_ptr.size(), s)) ||
       (st = upload(r->read_len, hb.read_len.data(), hb.read_len.size(), s)) ||
-      (st = upload(r->mean, r->h_mean.as<float>(), (size_t)hb.n_ev, s)) ||
+      (!hb.staged && (st = upload(r->mean, r->h_mean.as<float>(), (size_t)hb.n_ev, s))) ||
       (st = upload(r->ev_ptr, hb.ev_ptr.data(), hb.ev_ptr.size(), s)) ||
       (st = upload(r->n_events, hb.n_events.data(), hb.n_events.size(), s)) ||
       (st = upload(r->scal, scal, (size_t)n, s)) ||
@@ -1542,17 +1544,53 @@ abea_status_t abea_align_db(int32_t n_bam_rec, char **read, const int32_t *read_
     return st;
   // flatten, as GPU/align_cuda.c:432-492 does, into pinned staging: the bases
   // (no NUL terminators needed) and the event means, the one field of the
-  // event records the alignment reads
+  // event records the alignment reads.  Each thread copies its reads to the
+  // device in four parts as it goes, so the transfer runs under the flatten
+  // instead of after it.
+  HIP_TRY(hipSetDevice(r->ctx->device));
+  if ((st = r->seq.ensure((size_t)std::max<int64_t>(hb.n_bases, 1))) ||
+      (st = r->mean.ensure(sizeof(float) * (size_t)std::max<int64_t>(hb.n_ev, 1))))
+    return st;
   char *hs = r->h_seq.as<char>();
   float *hm = r->h_mean.as<float>();
+  char *ds = r->seq.as<char>();
+  float *dm = r->mean.as<float>();
+  const hipStream_t cs = r->ctx->stream;
+  std::atomic<int> copy_err{0};
+  const int dev = r->ctx->device;
   parallel_ranges(n, wpre, [&](int32_t b, int32_t e) {
-    for (int32_t i = b; i < e; ++i) {
-      if (read_len[i] > 0) std::memcpy(hs + hb.seq_ptr[i], read[i], (size_t)read_len[i]);
-      const abea_event_t *src = et[i].event;
-      float *dst = hm + hb.ev_ptr[i];
-      for (size_t j = 0; j < et[i].n; ++j) dst[j] = src[j].mean;
+    if (hipSetDevice(dev) != hipSuccess) copy_err = 1;  // this thread's current device
+    int32_t i = b;
+    for (int part = 1; part <= 4 && i < e; ++part) {
+      int32_t j = e;
+      if (part < 4) {
+        const int64_t target = wpre[b] + (wpre[e] - wpre[b]) * part / 4;
+        j = (int32_t)(std::lower_bound(wpre.begin() + i, wpre.begin() + e, target) -
+                      wpre.begin());
+        j = std::max(j, i + 1);
+      }
+      for (int32_t k = i; k < j; ++k) {
+        if (read_len[k] > 0) std::memcpy(hs + hb.seq_ptr[k], read[k], (size_t)read_len[k]);
+        const abea_event_t *src = et[k].event;
+        float *dst = hm + hb.ev_ptr[k];
+        for (size_t q = 0; q < et[k].n; ++q) dst[q] = src[q].mean;
+      }
+      const int64_t sb = hb.seq_ptr[i], se = j < n ? hb.seq_ptr[j] : hb.n_bases;
+      const int64_t eb = hb.ev_ptr[i], ee = j < n ? hb.ev_ptr[j] : hb.n_ev;
+      if (se > sb && hipMemcpyAsync(ds + sb, hs + sb, (size_t)(se - sb), hipMemcpyHostToDevice,
+                                    cs) != hipSuccess)
+        copy_err = 1;
+      if (ee > eb && hipMemcpyAsync(dm + eb, hm + eb, sizeof(float) * (size_t)(ee - eb),
+                                    hipMemcpyHostToDevice, cs) != hipSuccess)
+        copy_err = 1;
+      i = j;
     }
   });
+  if (copy_err) {
+    (void)hipGetLastError();
+    return fail(ABEA_ERR_HIP, "abea_align_db: copy of the flattened inputs failed");
+  }
+  hb.staged = true;
   t_host_ms[0] = ms_since(t0);
   std::vector<int32_t> np;
   std::vector<int64_t> coff;
