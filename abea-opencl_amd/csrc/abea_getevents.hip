@@ -343,48 +343,53 @@ __device__ __forceinline__ GeState fresh_state() {
 }
 
 // One sample of both detectors (short first); returns how many boundaries
-// it emitted (0-2) in p0, p1.
+// it emitted (0-2) in p0, p1.  The C source's branches become selects: a
+// detector not masked at i either has no peak (pos == -1: a lower x is the
+// new minimum, else a rise of more than the peak height starts a peak at
+// i) or has one (a higher x moves it to i; then the short detector masks
+// the long one past its peak once the peak clears the threshold, a fall of
+// more than the peak height below a peak above the threshold validates it,
+// and a valid peak more than w/2 samples back is emitted, the detector
+// starting afresh from x).  NaN statistics compare false, as in C.
 __device__ __forceinline__ int detect_step(GeState &st, float xs, float xl, int32_t i,
                                            int32_t &p0, int32_t &p1) {
   int ne = 0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     GeDet &d = st.d[k];
-    if (d.masked_to >= i) continue;
     const float x = k == 0 ? xs : xl;
     const float thr = k == 0 ? kThrShort : kThrLong;
     const int32_t w = k == 0 ? kWShort : kWLong;
-    if (d.pos == -1) {
-      if (x < d.val) {
-        d.val = x;
-      } else if (x - d.val > kPeakHeight) {
-        d.val = x;
-        d.pos = i;
-      }
-    } else {
-      if (x > d.val) {
-        d.val = x;
-        d.pos = i;
-      }
-      if (k == 0 && d.val > thr) {
-        GeDet &l = st.d[1];
-        l.masked_to = d.pos + w;
-        l.pos = -1;
-        l.val = FLT_MAX;
-        l.valid = 0;
-      }
-      if (d.val - x > kPeakHeight && d.val > thr) d.valid = 1;
-      if (d.valid && i - d.pos > w / 2) {
-        if (ne == 0)
-          p0 = d.pos;
-        else
-          p1 = d.pos;
-        ++ne;
-        d.pos = -1;
-        d.val = x;
-        d.valid = 0;
-      }
+    const bool act = d.masked_to < i;
+    const bool np = d.pos == -1;
+    const bool lower = x < d.val;
+    const bool rise = !lower && x - d.val > kPeakHeight;
+    const bool higher = x > d.val;
+    const bool take = act && (np ? (lower || rise) : higher);
+    const bool at_i = act && (np ? rise : higher);
+    const float val = take ? x : d.val;
+    const int32_t pos = at_i ? i : d.pos;
+    const bool peak = act && !np;
+    if (k == 0) {
+      const bool mask = peak && val > thr;
+      GeDet &l = st.d[1];
+      l.masked_to = mask ? pos + w : l.masked_to;
+      l.pos = mask ? -1 : l.pos;
+      l.val = mask ? FLT_MAX : l.val;
+      l.valid = mask ? 0 : l.valid;
     }
+    const int32_t valid = (peak && val - x > kPeakHeight && val > thr) ? 1 : d.valid;
+    const bool emit = peak && valid && i - pos > w / 2;
+    if (emit) {
+      if (ne == 0)
+        p0 = pos;
+      else
+        p1 = pos;
+      ++ne;
+    }
+    d.pos = emit ? -1 : pos;
+    d.val = emit ? x : val;
+    d.valid = emit ? 0 : valid;
   }
   return ne;
 }
