@@ -4,7 +4,7 @@ oracle restatement (oracle/getevents.c): event tables byte for byte.
 Covers signal lengths around the window and chunk sizes, empty and tiny
 signals, the serial prefix-sum path (terms too fine for an exact parallel
 sum), exact signals too large or too fine for the statistics' fast
-divisions, all-zero and subnormal signals, forced
+divisions, all-zero and subnormal signals, inf and NaN samples, forced
 speculation misses (no lead-in), dyadic step signals with known cuts, reads
 at a non-zero offset, and the host drop-ins."""
 import ctypes as C
@@ -196,6 +196,28 @@ def test_tiny_scales_and_the_tame_boundary(ctx):
     unscaled = _run_device(ctx, [base[0]])[0]
     assert np.array_equal(got[0]["start"], unscaled["start"])
     assert np.array_equal(got[0]["length"], unscaled["length"])
+
+
+def test_non_finite_samples(ctx):
+    """A read with an inf sample and one with a NaN sample fail the exactness
+    test and take the serial prefix sums; their events equal the oracle's,
+    field by field, a NaN matching a NaN of either sign (x86 and the GPU
+    make NaNs of different signs)."""
+    base = _signals([4000, 4000, 3000], seed=41)
+    sigs = [base[0].copy(), base[1].copy(), base[2]]
+    sigs[0][1500] = np.float32(np.inf)
+    sigs[1][2500] = np.float32(np.nan)
+    got = _run_device(ctx, sigs)
+    for i, (s, g) in enumerate(zip(sigs, got)):
+        want = oracle.getevents(s)
+        assert g.shape == want.shape, f"read {i}: {g.shape} vs {want.shape}"
+        assert np.array_equal(g["start"], want["start"]), f"read {i}"
+        assert np.array_equal(g["length"].view(np.uint32), want["length"].view(np.uint32)), i
+        for f in ("mean", "stdv"):
+            a, b = g[f], want[f]
+            same = (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+            assert same.all(), f"read {i} {f}: first at {np.nonzero(~same)[0][:3]}"
+    assert np.isinf(got[0]["mean"]).sum() == 1 and np.isnan(got[1]["mean"]).sum() == 1
 
 
 def test_mixed_kinds_in_one_call(ctx):
