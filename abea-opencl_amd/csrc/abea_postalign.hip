@@ -761,17 +761,20 @@ __global__ __launch_bounds__(64) void abea_recal_records_kernel(RecalArgs a) {
 // magnitudes stays below 2^(52+g) (one bit of margin for the rounding of that
 // bound), every partial sum in any order is a double exactly, so the result
 // equals the serial sum.
+// The ulp exponent of a finite float grows with the bits of its magnitude, so
+// a lane keeps the smallest and largest magnitude bits of its terms (zero
+// counted as the largest unsigned value in the minimum) and converts them
+// once: the smallest ulp exponent of the non-zero terms, and whether an inf
+// or NaN (magnitude bits >= 0x7f800000) occurred.
 struct ExactSum {
   double s = 0.0, bound = 0.0;  // this lane's partial sums
-  int gmin = 1 << 20;
-  bool finite = true;
+  uint32_t lo = 0xffffffffu, hi = 0u;
 };
 
 __device__ __forceinline__ void exact_add(ExactSum &es, float x) {
-  const uint32_t u = __builtin_bit_cast(uint32_t, x);
-  const int eb = (int)((u >> 23) & 0xff);
-  if ((u & 0x7fffffffu) != 0) es.gmin = min(es.gmin, eb ? eb - 150 : -149);  // ulp exponent
-  es.finite = es.finite && eb != 0xff;
+  const uint32_t u = __builtin_bit_cast(uint32_t, x) & 0x7fffffffu;
+  es.lo = min(es.lo, u - 1u);  // zero wraps to the largest value
+  es.hi = max(es.hi, u);
   es.s += (double)x;
   es.bound += fabs((double)x);
 }
@@ -781,8 +784,13 @@ __device__ __forceinline__ void exact_add(ExactSum &es, float x) {
 __device__ __forceinline__ bool exact_total(const ExactSum &es, double &total) {
   total = wave_sum_exact_f64(es.s);
   const double bound = wave_sum_exact_f64(es.bound);
-  const int gmin = -__builtin_amdgcn_readlane(wave_scan_max(-es.gmin), 63);
-  const bool finite = __builtin_amdgcn_ballot_w64(!es.finite) == 0;
+  int g = 1 << 20;  // no non-zero term
+  if (es.lo != 0xffffffffu) {
+    const int eb = (int)((es.lo + 1u) >> 23);
+    g = eb ? eb - 150 : -149;  // ulp exponent
+  }
+  const int gmin = -__builtin_amdgcn_readlane(wave_scan_max(-g), 63);
+  const bool finite = __builtin_amdgcn_ballot_w64(es.hi >= 0x7f800000u) == 0;
   return finite && bound <= ldexp(1.0, min(52 + gmin, 1000));
 }
 
