@@ -48,16 +48,6 @@ constexpr int kWShort = 3, kWLong = 6;
 // --------------------------------------------------------------------------
 // prefix sums
 
-// ulp exponent of a float (-149 for subnormals); 1000 for zero (no
-// constraint), and 2000 for inf/NaN (never exact)
-__device__ __forceinline__ int ulp_exp(float f) {
-  const uint32_t u = __builtin_bit_cast(uint32_t, f) & 0x7fffffffu;
-  if (u == 0) return 1000;
-  const int e = (int)(u >> 23);
-  if (e == 255) return 2000;
-  return e == 0 ? -149 : e - 150;
-}
-
 __device__ __forceinline__ double wave_scan_add_f64(double v) {
   v += dpp_f64(v, 0x111, 0xf);
   v += dpp_f64(v, 0x112, 0xf);
@@ -93,7 +83,9 @@ __device__ __forceinline__ void reduce_sum_info(GeChunkSum &c) {
   c.hx = wave_max_i32(c.hx);
 }
 
-// ulp exponent of the float whose magnitude has bits u (ulp_exp's values)
+// ulp exponent of the float whose magnitude has bits u (-149 for
+// subnormals); 1000 for zero (no constraint), and 2000 for inf/NaN (never
+// exact)
 __device__ __forceinline__ int ulp_exp_bits(uint32_t u) {
   if (u == 0) return 1000;
   const int e = (int)(u >> 23);
